@@ -1,0 +1,131 @@
+/*
+ * plan_hip.h — C ABI of the MI355X (gfx950) clump-finding library.
+ *
+ * The reference (astroboylrx/PLAN) exposes this path as public C++ members
+ * of its BHtree / ParticleSet / PlanetesimalList templates; there is no
+ * plugin registry or FFI.  Each entry point below replaces one of those
+ * members for a caller that hands over plain arrays.  All functions return
+ * 0 on success and a nonzero PLAN_E_* code on failure, with the message in
+ * plan_last_error(); none of them exits the process (the reference calls
+ * exit(2..5) on fatal errors, src/global.cpp:469-512, src/tree.hpp:1341).
+ *
+ * One plan_ctx owns one HIP stream on one device and all device memory of
+ * one snapshot; it is not thread-safe, separate contexts may run
+ * concurrently.  Device buffers are reused across calls.
+ */
+#ifndef PLAN_HIP_H
+#define PLAN_HIP_H
+
+#include <stdint.h>
+
+#include "plan_params.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    PLAN_OK = 0,
+    PLAN_E_ARG = 1,      /* invalid argument (reference: exit(4), src/global.cpp:204,215) */
+    PLAN_E_STATE = 2,    /* call out of order (no particles / no tree yet) */
+    PLAN_E_DEVICE = 3,   /* HIP runtime error or out of device memory */
+    PLAN_E_INTERNAL = 4  /* unexpected condition inside the pipeline */
+};
+
+typedef struct plan_ctx plan_ctx;
+
+/* Tree summary after plan_build_tree (BHtree members, src/tree.hpp:2461-2531). */
+typedef struct plan_tree_info {
+    uint32_t num_particles;   /* after sink merging */
+    uint32_t num_nodes;
+    uint32_t num_leaf_nodes;
+    uint32_t num_sink_particles;
+    uint32_t depth;           /* deepest node level (root = 1) */
+    uint32_t reserved;
+    double   root_center[3];
+    double   half_width;
+    int32_t  level_count[32];
+} plan_tree_info;
+
+/* Create a context on HIP device `device`. */
+int plan_ctx_create(plan_ctx **out, int device);
+void plan_ctx_destroy(plan_ctx *ctx);
+const char *plan_last_error(const plan_ctx *ctx);
+
+/* Load the particle set of one snapshot, already in the reference's
+ * ReadLisFile order (sorted by cpu_id, id_in_run; id = index):
+ * replaces ParticleSet<3>::ReadLisFile's result (src/tree.hpp:1540-1621).
+ * pos, vel: N x 3 row-major; density: the .lis grid density (ath_density);
+ * property_index: species; id: permanent particle id.  params carry the
+ * header box/time and the parameter-file keys (plan_params.h). */
+int plan_set_particles(plan_ctx *ctx, const plan_params *params, uint32_t N, const double *pos, const double *vel,
+                       const double *density, const int32_t *property_index, const uint32_t *id);
+/* Same from the .lis-native float32 fields (converted to binary64 on the
+ * device exactly as the reference's static_cast<double>, src/tree.hpp:1397-1404). */
+int plan_set_particles_f32(plan_ctx *ctx, const plan_params *params, uint32_t N, const float *pos, const float *vel,
+                           const float *density, const int32_t *property_index, const uint32_t *id);
+
+/* ParticleSet<3>::MakeGhostParticles (src/tree.hpp:1626-1754), on the device.
+ * T_out receives N + number of ghosts.  With params->no_ghost set it is a no-op. */
+int plan_make_ghost_particles(plan_ctx *ctx, uint32_t *T_out);
+
+/* BHtree<3>::BuildTree(paras, particle_set, quiet, check_pos) (src/tree.hpp:2759-2908).
+ * Consumes the particle set like the reference (src/tree.hpp:2820). */
+int plan_build_tree(plan_ctx *ctx, int check_pos, plan_tree_info *info);
+
+/* Step 1 of FindPlanetesimals (src/tree.hpp:3664-3689): new_density of every
+ * tree particle from its K neighbours.  kernel: 0 median spherical
+ * (src/tree.hpp:3473-3491), 1 pseudo quadratic spline (src/tree.hpp:3427-3434),
+ * -1 the reference's automatic choice (src/tree.hpp:3576-3612). */
+int plan_knn_density(plan_ctx *ctx, uint32_t K, int kernel);
+
+/* Steps 2-3 (src/tree.hpp:3711-3809): densest of the H nearest neighbours
+ * for every particle with new_density >= threshold, then each one's peak.
+ * Returns the number of such particles and of naive groups. */
+int plan_hop(plan_ctx *ctx, uint32_t H, double threshold, uint32_t *num_above, uint32_t *num_groups);
+
+/* BHtree<3>::FindPlanetesimals(ds, loop_count) (src/tree.hpp:3576-4339) on a
+ * built tree: densities, HOP, merging, unbinding, rebinding, filters,
+ * angular momentum.  *out is allocated by the library (plan_catalog_free). */
+int plan_find_planetesimals(plan_ctx *ctx, plan_catalog **out);
+void plan_catalog_free(plan_catalog *cat);
+
+/* ---- inspection (tests, tools) ------------------------------------------------ */
+
+/* tree particles in tree order; any pointer may be NULL.  pos/vel: n x 3. */
+int plan_get_tree_particles(plan_ctx *ctx, double *pos, double *vel, double *mass, double *ath_density,
+                            double *new_density, uint32_t *original_id, uint32_t *sink_count);
+/* sorted unique Morton keys as words (bits 0-31, 32-63, 64-95) */
+int plan_get_keys(plan_ctx *ctx, uint32_t *w0, uint32_t *w1, uint32_t *w2);
+/* node arrays (num_nodes); center: num_nodes x 3 */
+int plan_get_nodes(plan_ctx *ctx, double *center, double *half_width, uint32_t *begin, uint32_t *end,
+                   uint32_t *parent, uint32_t *first_daughter, uint8_t *num_daughters, uint8_t *level,
+                   uint16_t *orthant, int32_t *node2leaf, int32_t *leaf_nodes);
+/* densest neighbour and peak per tree particle after plan_hop (peak = n for
+ * particles below the threshold) */
+int plan_get_hop(plan_ctx *ctx, uint32_t *densest, uint32_t *peak);
+/* K-nearest-neighbour queries with the reference's candidate-set rule
+ * (KNN_Search_OpenMP, src/tree.hpp:3198-3251).  Either particle queries
+ * (particle_idx non-NULL, start radius from the tree, est may be NULL) or
+ * point queries (pts: nq x 3 with est[nq] > 0).  Outputs nq x K indices and
+ * squared distances in ascending order, and the final search radius. */
+int plan_knn_query(plan_ctx *ctx, uint32_t nq, const uint32_t *particle_idx, const double *pts, const double *est,
+                   uint32_t K, uint32_t *idx, double *d2, double *rstop);
+/* Densities at points (IsSaddlePointDeepEnough's query, src/tree.hpp:4880-4892). */
+int plan_density_at(plan_ctx *ctx, uint32_t nq, const double *pts, const double *est, uint32_t K, int kernel,
+                    double *rho);
+/* BallSearch (src/tree.hpp:3297-3304): indices in traversal order; returns
+ * the count in *count (writes at most cap). */
+int plan_ball_query(plan_ctx *ctx, const double *center, double radius, uint32_t *out, uint32_t cap,
+                    uint32_t *count);
+/* wall-clock milliseconds of the last pipeline stages: ghosts, build, density,
+ * hop, clumps, total (hipEvent timing on the context stream; clumps = host). */
+int plan_stage_times(plan_ctx *ctx, double *ms, int n);
+/* device milliseconds of the last density KNN launch (single kernel) */
+int plan_last_density_kernel_ms(plan_ctx *ctx, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PLAN_HIP_H */

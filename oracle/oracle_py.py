@@ -1,0 +1,218 @@
+"""ctypes binding of the CPU restatement (liboracle).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg as the parity checker / CPU baseline.  The
+product under plan_amd/ never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from plan_amd.abi import PlanCatalog, PlanDerived, catalog_to_py, make_params
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "libplan_oracle.so")
+_lib = None
+
+P = np.ctypeslib.ndpointer
+_d = P(dtype=np.float64, flags="C_CONTIGUOUS")
+_u32 = P(dtype=np.uint32, flags="C_CONTIGUOUS")
+_i32 = P(dtype=np.int32, flags="C_CONTIGUOUS")
+_u8 = P(dtype=np.uint8, flags="C_CONTIGUOUS")
+_u16 = P(dtype=np.uint16, flags="C_CONTIGUOUS")
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = C.CDLL(_LIB)
+        vp = C.c_void_p
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_derive.argtypes = [vp, C.c_uint32, vp]
+        L.orc_make_ghosts.restype = C.c_uint32
+        L.orc_make_ghosts.argtypes = [vp, C.c_uint32, _d, _d, _d, _i32, _u32, vp, vp, vp, vp, vp]
+        L.orc_tree_build.restype = vp
+        L.orc_tree_build.argtypes = [vp, C.c_uint32, C.c_uint32, _d, _d, _d, _i32, _u32]
+        L.orc_tree_free.argtypes = [vp]
+        for f in ("orc_tree_num_particles", "orc_tree_num_nodes", "orc_tree_num_leaves", "orc_tree_num_sinks"):
+            getattr(L, f).restype = C.c_uint32
+            getattr(L, f).argtypes = [vp]
+        L.orc_tree_root.argtypes = [vp, _d, _d]
+        L.orc_tree_keys.argtypes = [vp, _u32, _u32, _u32, _u32]
+        L.orc_tree_particles.argtypes = [vp, _d, _d, _d, _d, _d, _u32, _u8]
+        L.orc_tree_nodes.argtypes = [vp, _d, _d, _u32, _u32, _u32, _u32, _u8, _u8, _u16, _i32, _i32]
+        L.orc_knn.restype = C.c_uint32
+        L.orc_knn.argtypes = [vp, _d, C.c_uint32, C.c_uint32, vp, _u32, _d, _d, _d, _u32, _u32]
+        L.orc_ball.restype = C.c_uint32
+        L.orc_ball.argtypes = [vp, _d, C.c_double, _u32, C.c_uint32]
+        L.orc_find_planetesimals.argtypes = [vp]
+        L.orc_densities.argtypes = [vp, _d]
+        L.orc_hop.argtypes = [vp, _u32, _u32]
+        L.orc_stage_info.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                     C.POINTER(C.c_uint32)]
+        L.orc_catalog.restype = C.POINTER(PlanCatalog)
+        L.orc_catalog.argtypes = [vp]
+        L.orc_catalog_free.argtypes = [C.POINTER(PlanCatalog)]
+        L.orc_unbind_once.restype = C.c_uint32
+        L.orc_unbind_once.argtypes = [vp, C.c_uint32, _d, _d, _d, _u32, _d]
+        _lib = L
+    return _lib
+
+
+def set_threads(n: int):
+    lib().orc_set_threads(int(n))
+
+
+def derive(params: dict, n: int) -> PlanDerived:
+    p = make_params(params)
+    d = PlanDerived()
+    lib().orc_derive(C.byref(p), n, C.byref(d))
+    return d
+
+
+def make_ghosts(params: dict, pos, vel, dens, prop, ids):
+    """pos/vel: (N,3) float64; returns arrays for the T particles."""
+    p = make_params(params)
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    vel = np.ascontiguousarray(vel, dtype=np.float64)
+    dens = np.ascontiguousarray(dens, dtype=np.float64)
+    prop = np.ascontiguousarray(prop, dtype=np.int32)
+    ids = np.ascontiguousarray(ids, dtype=np.uint32)
+    n = pos.shape[0]
+    L = lib()
+    T = L.orc_make_ghosts(C.byref(p), n, pos.ravel(), vel.ravel(), dens, prop, ids, None, None, None, None, None)
+    op = np.empty((T, 3)); ov = np.empty((T, 3)); od = np.empty(T)
+    opr = np.empty(T, dtype=np.int32); oid = np.empty(T, dtype=np.uint32)
+    L.orc_make_ghosts(C.byref(p), n, pos.ravel(), vel.ravel(), dens, prop, ids,
+                      op.ctypes.data, ov.ctypes.data, od.ctypes.data, opr.ctypes.data, oid.ctypes.data)
+    return op, ov, od, opr, oid
+
+
+class OracleTree:
+    """A built oracle tree over T particles (N real)."""
+
+    def __init__(self, params: dict, n_real: int, pos, vel, dens, prop, ids):
+        self.params = params
+        self._p = make_params(params)
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        vel = np.ascontiguousarray(vel, dtype=np.float64)
+        self.h = lib().orc_tree_build(C.byref(self._p), n_real, pos.shape[0], pos.ravel(), vel.ravel(),
+                                      np.ascontiguousarray(dens, dtype=np.float64),
+                                      np.ascontiguousarray(prop, dtype=np.int32),
+                                      np.ascontiguousarray(ids, dtype=np.uint32))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_tree_free(self.h)
+            self.h = None
+
+    @property
+    def n(self):
+        return lib().orc_tree_num_particles(self.h)
+
+    def root(self):
+        c = np.zeros(3); hw = np.zeros(1)
+        lib().orc_tree_root(self.h, c, hw)
+        return c, float(hw[0])
+
+    def keys(self):
+        n = self.n
+        w = [np.empty(n, dtype=np.uint32) for _ in range(4)]
+        lib().orc_tree_keys(self.h, *w)
+        return w  # w0, w1, w2, particle index
+
+    def particles(self):
+        n = self.n
+        pos = np.empty((n, 3)); vel = np.empty((n, 3)); m = np.empty(n); ath = np.empty(n); rho = np.empty(n)
+        oid = np.empty(n, dtype=np.uint32); sink = np.empty(n, dtype=np.uint8)
+        lib().orc_tree_particles(self.h, pos.ravel(), vel.ravel(), m, ath, rho, oid, sink)
+        return dict(pos=pos, vel=vel, mass=m, ath=ath, rho=rho, oid=oid, sink=sink)
+
+    def nodes(self):
+        L = lib()
+        nn = L.orc_tree_num_nodes(self.h)
+        nl = L.orc_tree_num_leaves(self.h)
+        out = dict(center=np.empty((nn, 3)), hw=np.empty(nn), begin=np.empty(nn, np.uint32),
+                   end=np.empty(nn, np.uint32), parent=np.empty(nn, np.uint32),
+                   first_daughter=np.empty(nn, np.uint32), num_daughters=np.empty(nn, np.uint8),
+                   level=np.empty(nn, np.uint8), orthant=np.empty(nn, np.uint16),
+                   node2leaf=np.empty(nn, np.int32), leaf_nodes=np.empty(max(nl, 1), np.int32))
+        L.orc_tree_nodes(self.h, out["center"].ravel(), out["hw"], out["begin"], out["end"], out["parent"],
+                         out["first_daughter"], out["num_daughters"], out["level"], out["orthant"],
+                         out["node2leaf"], out["leaf_nodes"])
+        out["leaf_nodes"] = out["leaf_nodes"][:nl]
+        return out
+
+    def num_sinks(self):
+        return lib().orc_tree_num_sinks(self.h)
+
+    def knn(self, q, K, est=None):
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, 3)
+        nq = q.shape[0]
+        idx = np.empty((nq, K), np.uint32); d2 = np.empty((nq, K)); rk = np.empty(nq); rs = np.empty(nq)
+        nc = np.empty(nq, np.uint32); nd = np.empty(nq, np.uint32)
+        e = None
+        if est is not None:
+            est = np.ascontiguousarray(est, dtype=np.float64)
+            e = est.ctypes.data
+        bad = lib().orc_knn(self.h, q.ravel(), nq, K, e, idx.ravel(), d2.ravel(), rk, rs, nc, nd)
+        if bad:
+            raise RuntimeError(f"oracle knn: start node missing for query {bad - 1}")
+        return dict(idx=idx, d2=d2, rk=rk, rstop=rs, ncand=nc, ndrop=nd)
+
+    def ball(self, c, r, cap=1 << 22):
+        out = np.empty(cap, np.uint32)
+        n = lib().orc_ball(self.h, np.ascontiguousarray(c, dtype=np.float64), float(r), out, cap)
+        return out[:min(n, cap)]
+
+    def find_planetesimals(self):
+        lib().orc_find_planetesimals(self.h)
+
+    def densities(self):
+        r = np.empty(self.n)
+        lib().orc_densities(self.h, r)
+        return r
+
+    def hop(self):
+        n = self.n
+        dn = np.empty(n, np.uint32); pk = np.empty(n, np.uint32)
+        lib().orc_hop(self.h, dn, pk)
+        return dn, pk
+
+    def stage_info(self):
+        k = C.c_int(); h = C.c_uint32(); a = C.c_uint32(); b = C.c_uint32()
+        lib().orc_stage_info(self.h, C.byref(k), C.byref(h), C.byref(a), C.byref(b))
+        return dict(kernel=k.value, hop=h.value, naive_clumps=a.value, after_merge=b.value)
+
+    def catalog(self):
+        L = lib()
+        c = L.orc_catalog(self.h)
+        try:
+            return catalog_to_py(c.contents)
+        finally:
+            L.orc_catalog_free(c)
+
+
+def run_pipeline(snap, params=None, threads: int | None = None):
+    """Ghosts -> tree -> FindPlanetesimals on a synth.Snapshot; returns the tree."""
+    params = params or snap.params
+    if threads:
+        set_threads(threads)
+    pos = snap.pos.astype(np.float64)
+    vel = snap.vel.astype(np.float64)
+    dens = snap.density.astype(np.float64)
+    gp, gv, gd, gpr, gid = make_ghosts(params, pos, vel, dens, snap.property_index, snap.id)
+    t = OracleTree(params, snap.n, gp, gv, gd, gpr, gid)
+    t.find_planetesimals()
+    return t

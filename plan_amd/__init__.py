@@ -1,0 +1,319 @@
+"""plan_amd — MI355X (gfx950) implementation of PLAN's clump-finding path.
+
+The compute runs in libplan_hip.so (hand-written HIP kernels behind the C ABI
+declared in include/plan_hip.h).  This module is a thin host mirror of the
+reference's C++ interface for that path:
+
+* ``ParticleSet``  — ParticleSet<3> after ReadLisFile (src/tree.hpp:1200-1621)
+  with ``MakeGhostParticles`` (src/tree.hpp:1626-1754);
+* ``BHtree``       — ``BuildTree`` (src/tree.hpp:2759) and
+  ``FindPlanetesimals`` (src/tree.hpp:3576);
+* ``DataSet``      — the aggregate the reference passes around (src/tree.hpp:5298-5320).
+
+There is no CPU fallback: every call goes to the device library, and loading
+fails loudly when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .abi import PlanCatalog, PlanParams, catalog_to_py, make_params
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libplan_hip.so")
+_lib = None
+
+P = np.ctypeslib.ndpointer
+_d = P(dtype=np.float64, flags="C_CONTIGUOUS")
+_f = P(dtype=np.float32, flags="C_CONTIGUOUS")
+_u32 = P(dtype=np.uint32, flags="C_CONTIGUOUS")
+_i32 = P(dtype=np.int32, flags="C_CONTIGUOUS")
+_u8 = P(dtype=np.uint8, flags="C_CONTIGUOUS")
+_u16 = P(dtype=np.uint16, flags="C_CONTIGUOUS")
+vp = C.c_void_p
+
+
+class TreeInfo(C.Structure):
+    _fields_ = [("num_particles", C.c_uint32), ("num_nodes", C.c_uint32), ("num_leaf_nodes", C.c_uint32),
+                ("num_sink_particles", C.c_uint32), ("depth", C.c_uint32), ("reserved", C.c_uint32),
+                ("root_center", C.c_double * 3), ("half_width", C.c_double), ("level_count", C.c_int32 * 32)]
+
+
+EXPORTS = [
+    "plan_ctx_create", "plan_ctx_destroy", "plan_last_error", "plan_set_particles", "plan_set_particles_f32",
+    "plan_make_ghost_particles", "plan_build_tree", "plan_knn_density", "plan_hop", "plan_find_planetesimals",
+    "plan_catalog_free", "plan_get_tree_particles", "plan_get_keys", "plan_get_nodes", "plan_get_hop",
+    "plan_knn_query", "plan_density_at", "plan_ball_query", "plan_stage_times", "plan_last_density_kernel_ms",
+]
+
+
+class PlanError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libplan_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise PlanError(f"{LIB_PATH} not built: run __graft_entry__.build() or python plan_amd/build.py")
+    L = C.CDLL(LIB_PATH)
+    L.plan_ctx_create.argtypes = [C.POINTER(vp), C.c_int]
+    L.plan_ctx_destroy.argtypes = [vp]
+    L.plan_last_error.restype = C.c_char_p
+    L.plan_last_error.argtypes = [vp]
+    L.plan_set_particles.argtypes = [vp, C.POINTER(PlanParams), C.c_uint32, _d, _d, _d, _i32, _u32]
+    L.plan_set_particles_f32.argtypes = [vp, C.POINTER(PlanParams), C.c_uint32, _f, _f, _f, _i32, _u32]
+    L.plan_make_ghost_particles.argtypes = [vp, C.POINTER(C.c_uint32)]
+    L.plan_build_tree.argtypes = [vp, C.c_int, C.POINTER(TreeInfo)]
+    L.plan_knn_density.argtypes = [vp, C.c_uint32, C.c_int]
+    L.plan_hop.argtypes = [vp, C.c_uint32, C.c_double, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+    L.plan_find_planetesimals.argtypes = [vp, C.POINTER(C.POINTER(PlanCatalog))]
+    L.plan_catalog_free.argtypes = [C.POINTER(PlanCatalog)]
+    L.plan_get_tree_particles.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
+    L.plan_get_keys.argtypes = [vp, vp, vp, vp]
+    L.plan_get_nodes.argtypes = [vp] + [vp] * 11
+    L.plan_get_hop.argtypes = [vp, vp, vp]
+    L.plan_knn_query.argtypes = [vp, C.c_uint32, vp, vp, vp, C.c_uint32, vp, vp, vp]
+    L.plan_density_at.argtypes = [vp, C.c_uint32, _d, _d, C.c_uint32, C.c_int, _d]
+    L.plan_ball_query.argtypes = [vp, _d, C.c_double, vp, C.c_uint32, C.POINTER(C.c_uint32)]
+    L.plan_stage_times.argtypes = [vp, _d, C.c_int]
+    L.plan_last_density_kernel_ms.argtypes = [vp, C.POINTER(C.c_double)]
+    for f in EXPORTS:
+        if f in ("plan_ctx_destroy", "plan_catalog_free", "plan_last_error"):
+            continue
+        getattr(L, f).restype = C.c_int
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+class Context:
+    """One device context (one HIP stream, one snapshot's device memory)."""
+
+    def __init__(self, device: int = 0):
+        self._h = vp()
+        rc = lib().plan_ctx_create(C.byref(self._h), int(device))
+        if rc != 0:
+            raise PlanError(f"plan_ctx_create failed with code {rc} (no usable HIP device?)")
+        self.info = None
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().plan_ctx_destroy(self._h)
+            self._h = vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = lib().plan_last_error(self._h).decode(errors="replace")
+            raise PlanError(f"{what} failed (code {rc}): {msg}")
+
+    # --- loading and building -------------------------------------------------
+    def set_particles(self, params: dict, pos, vel, density, prop=None, ids=None):
+        n = int(np.asarray(pos).shape[0])
+        prop = np.zeros(n, np.int32) if prop is None else np.ascontiguousarray(prop, np.int32)
+        ids = np.arange(n, dtype=np.uint32) if ids is None else np.ascontiguousarray(ids, np.uint32)
+        self._p = make_params(params)
+        if np.asarray(pos).dtype == np.float32:
+            self._chk(lib().plan_set_particles_f32(self._h, C.byref(self._p), n,
+                                                   np.ascontiguousarray(pos, np.float32).ravel(),
+                                                   np.ascontiguousarray(vel, np.float32).ravel(),
+                                                   np.ascontiguousarray(density, np.float32), prop, ids),
+                      "plan_set_particles_f32")
+        else:
+            self._chk(lib().plan_set_particles(self._h, C.byref(self._p), n,
+                                               np.ascontiguousarray(pos, np.float64).ravel(),
+                                               np.ascontiguousarray(vel, np.float64).ravel(),
+                                               np.ascontiguousarray(density, np.float64), prop, ids),
+                      "plan_set_particles")
+        return n
+
+    def make_ghosts(self) -> int:
+        t = C.c_uint32()
+        self._chk(lib().plan_make_ghost_particles(self._h, C.byref(t)), "plan_make_ghost_particles")
+        return t.value
+
+    def build_tree(self, check_pos: bool = True) -> TreeInfo:
+        info = TreeInfo()
+        self._chk(lib().plan_build_tree(self._h, 1 if check_pos else 0, C.byref(info)), "plan_build_tree")
+        self.info = info
+        self.n = info.num_particles
+        return info
+
+    def knn_density(self, K: int, kernel: int = -1):
+        self._chk(lib().plan_knn_density(self._h, int(K), int(kernel)), "plan_knn_density")
+
+    def hop(self, H: int, threshold: float):
+        a = C.c_uint32(); g = C.c_uint32()
+        self._chk(lib().plan_hop(self._h, int(H), float(threshold), C.byref(a), C.byref(g)), "plan_hop")
+        return a.value, g.value
+
+    def find_planetesimals(self) -> dict:
+        out = C.POINTER(PlanCatalog)()
+        self._chk(lib().plan_find_planetesimals(self._h, C.byref(out)), "plan_find_planetesimals")
+        try:
+            return catalog_to_py(out.contents)
+        finally:
+            lib().plan_catalog_free(out)
+
+    # --- inspection -------------------------------------------------------------
+    def tree_particles(self) -> dict:
+        n = self.n
+        o = dict(pos=np.empty((n, 3)), vel=np.empty((n, 3)), mass=np.empty(n), ath=np.empty(n), rho=np.empty(n),
+                 oid=np.empty(n, np.uint32), sinkcnt=np.empty(n, np.uint32))
+        self._chk(lib().plan_get_tree_particles(self._h, _ptr(o["pos"]), _ptr(o["vel"]), _ptr(o["mass"]),
+                                                _ptr(o["ath"]), _ptr(o["rho"]), _ptr(o["oid"]), _ptr(o["sinkcnt"])),
+                  "plan_get_tree_particles")
+        return o
+
+    def keys(self):
+        n = self.n
+        w = [np.empty(n, np.uint32) for _ in range(3)]
+        self._chk(lib().plan_get_keys(self._h, *[_ptr(x) for x in w]), "plan_get_keys")
+        return w
+
+    def nodes(self) -> dict:
+        nn = self.info.num_nodes
+        nl = self.info.num_leaf_nodes
+        o = dict(center=np.empty((nn, 3)), hw=np.empty(nn), begin=np.empty(nn, np.uint32),
+                 end=np.empty(nn, np.uint32), parent=np.empty(nn, np.uint32),
+                 first_daughter=np.empty(nn, np.uint32), num_daughters=np.empty(nn, np.uint8),
+                 level=np.empty(nn, np.uint8), orthant=np.empty(nn, np.uint16),
+                 node2leaf=np.empty(nn, np.int32), leaf_nodes=np.empty(max(nl, 1), np.int32))
+        self._chk(lib().plan_get_nodes(self._h, *[_ptr(o[k]) for k in ("center", "hw", "begin", "end", "parent",
+                                                                       "first_daughter", "num_daughters", "level",
+                                                                       "orthant", "node2leaf", "leaf_nodes")]),
+                  "plan_get_nodes")
+        o["leaf_nodes"] = o["leaf_nodes"][:nl]
+        return o
+
+    def hop_result(self):
+        n = self.n
+        dn = np.empty(n, np.uint32); pk = np.empty(n, np.uint32)
+        self._chk(lib().plan_get_hop(self._h, _ptr(dn), _ptr(pk)), "plan_get_hop")
+        return dn, pk
+
+    def knn(self, K: int, particle_idx=None, pts=None, est=None):
+        if particle_idx is not None:
+            q = np.ascontiguousarray(particle_idx, np.uint32)
+            nq = q.size
+            e = None if est is None else np.ascontiguousarray(est, np.float64)
+            pts_ = None
+        else:
+            pts_ = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+            nq = pts_.shape[0]
+            q = None
+            e = np.ascontiguousarray(est, np.float64)
+        idx = np.empty((nq, K), np.uint32); d2 = np.empty((nq, K)); rs = np.empty(nq)
+        self._chk(lib().plan_knn_query(self._h, nq, _ptr(q), _ptr(pts_), _ptr(e), int(K), _ptr(idx), _ptr(d2),
+                                       _ptr(rs)), "plan_knn_query")
+        return dict(idx=idx, d2=d2, rstop=rs)
+
+    def density_at(self, pts, est, K: int, kernel: int = 0):
+        pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+        est = np.ascontiguousarray(est, np.float64)
+        rho = np.empty(pts.shape[0])
+        self._chk(lib().plan_density_at(self._h, pts.shape[0], pts.ravel(), est, int(K), int(kernel), rho),
+                  "plan_density_at")
+        return rho
+
+    def ball(self, center, radius, cap=1 << 22):
+        out = np.empty(cap, np.uint32)
+        cnt = C.c_uint32()
+        self._chk(lib().plan_ball_query(self._h, np.ascontiguousarray(center, np.float64), float(radius),
+                                        _ptr(out), cap, C.byref(cnt)), "plan_ball_query")
+        return out[:min(cnt.value, cap)]
+
+    def stage_times(self) -> dict:
+        t = np.zeros(6)
+        self._chk(lib().plan_stage_times(self._h, t, 6), "plan_stage_times")
+        return dict(zip(("ghosts", "build", "density", "hop", "clumps", "find_total"), t.tolist()))
+
+    def density_kernel_ms(self) -> float:
+        v = C.c_double()
+        self._chk(lib().plan_last_density_kernel_ms(self._h, C.byref(v)), "plan_last_density_kernel_ms")
+        return v.value
+
+
+# ---------------------------------------------------------------- reference-shaped facade
+
+class ParticleSet:
+    """ParticleSet<3> after ReadLisFile (particles sorted by (cpu_id, id_in_run), id = index)."""
+
+    def __init__(self, pos, vel, density, property_index=None, ids=None, params: dict | None = None):
+        self.pos = pos
+        self.vel = vel
+        self.density = density
+        self.property_index = property_index
+        self.ids = ids
+        self.params = params
+        self.num_particles = int(np.asarray(pos).shape[0])
+        self.num_total_particles = self.num_particles
+        self._ctx = None
+
+    @classmethod
+    def from_snapshot(cls, snap):
+        return cls(snap.pos, snap.vel, snap.density, snap.property_index, snap.id, snap.params)
+
+    def _bind(self, ctx: Context):
+        if self._ctx is not ctx:
+            ctx.set_particles(self.params, self.pos, self.vel, self.density, self.property_index, self.ids)
+            self._ctx = ctx
+
+    def MakeGhostParticles(self, ctx: Context):
+        """Ghost images in the shearing-periodic ghost zone (src/tree.hpp:1626)."""
+        self._bind(ctx)
+        self.num_total_particles = ctx.make_ghosts()
+        return self.num_total_particles
+
+
+class BHtree:
+    """Device octree with the reference's BuildTree / FindPlanetesimals entry points."""
+
+    def __init__(self, ctx: Context | None = None):
+        self.ctx = ctx or Context(0)
+        self.info = None
+
+    def BuildTree(self, paras: dict, particle_set: ParticleSet, quiet: bool = False, check_pos: bool = True):
+        particle_set._bind(self.ctx)
+        self.info = self.ctx.build_tree(check_pos)
+        particle_set._ctx = None  # consumed, like the reference (src/tree.hpp:2820)
+        return self.info
+
+    def FindPlanetesimals(self, ds: "DataSet", loop_count: int = -1):
+        cat = self.ctx.find_planetesimals()
+        ds.planetesimal_list = cat
+        return cat
+
+
+class DataSet:
+    """particle_set + tree + planetesimal_list (src/tree.hpp:5298-5320)."""
+
+    def __init__(self, particle_set: ParticleSet, ctx: Context | None = None):
+        self.particle_set = particle_set
+        self.tree = BHtree(ctx)
+        self.planetesimal_list = None
+
+
+def find_clumps(snap, device: int = 0, ctx: Context | None = None) -> dict:
+    """Ghosts -> BuildTree -> FindPlanetesimals for a synth.Snapshot."""
+    ctx = ctx or Context(device)
+    ps = ParticleSet.from_snapshot(snap)
+    ds = DataSet(ps, ctx)
+    ps.MakeGhostParticles(ctx)
+    ds.tree.BuildTree(snap.params, ps)
+    return ds.tree.FindPlanetesimals(ds, -1)

@@ -1,0 +1,890 @@
+// api.cpp — the C ABI (include/plan_hip.h): context, particle upload, stage
+// orchestration of FindPlanetesimals and the device services of the host
+// clump logic.
+#include "../../include/plan_hip.h"
+#include "clumps.h"
+#include "ctx.h"
+#include "hop.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+using namespace plan;
+
+struct plan_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    plan_params pp{};
+    plan_derived d{};
+    bool have_particles = false;
+    bool have_tree = false;
+    bool have_density = false;
+    bool have_hop = false;
+    DevParticles ps;
+    DevTree tree;
+    DevTree ctree;
+    DevParticles cps;
+    Arena work;
+    HopResult hop;
+    uint32_t K = 0, H = 0;
+    int kernel = 0;
+    double times[6] = {0, 0, 0, 0, 0, 0};
+    double density_kernel_ms = 0;
+    hipEvent_t ev[8];
+};
+
+namespace plan {
+
+/* NumericalParameters after ReadLisFile (src/tree.hpp:1597-1612) and
+ * CalculateNewParameters (src/global.cpp:28-47). */
+void derive_params(const plan_params *p, uint32_t N, plan_derived *d)
+{
+    std::memset(d, 0, sizeof(*d));
+    const double PI = 3.14159265358979323846;
+    d->q = 1.5;
+    d->Omega = 1.0;
+    for (int k = 0; k < 3; ++k) {
+        d->box_center[k] = (p->box_min[k] + p->box_max[k]) / 2.0;
+        d->box_length[k] = p->box_max[k] - p->box_min[k];
+    }
+    d->max_half_width = 0.0;
+    d->max_ghost_zone_width = 0.025;
+    for (int k = 0; k < 3; ++k) {
+        d->box_half_width[k] = d->box_length[k] / 2.0;
+        d->ghost_zone_width[k] = 0.025;
+        d->max_half_width = std::max(d->max_half_width, d->box_half_width[k]);
+        d->max_ghost_zone_width = std::max(d->max_ghost_zone_width, d->ghost_zone_width[k]);
+    }
+    for (int k = 0; k < 3; ++k) d->cell_length[k] = d->box_length[k] / (double)p->box_resolution[k];
+    d->cell_volume = ((1.0 * d->cell_length[0]) * d->cell_length[1]) * d->cell_length[2];
+    d->grav_constant = p->four_pi_G / (4. * PI);
+    d->shear_speed = (d->q * d->Omega) * d->box_length[0];
+    d->min_trusted_mass_code_unit =
+        ((((3 * std::pow(d->cell_length[0], 3)) * d->Omega) * d->Omega) / 1.0) / d->grav_constant;
+    uint32_t nt = p->num_types > PLAN_MAX_TYPES ? PLAN_MAX_TYPES : p->num_types;
+    double frac[PLAN_MAX_TYPES];
+    double sum = 0;
+    for (uint32_t i = 0; i < nt; ++i) {
+        frac[i] = p->type_info[i];
+        sum += p->type_info[i];
+    }
+    d->mass_total_code_units = ((std::sqrt(2 * PI) * d->box_length[0]) * d->box_length[1]) * p->solid_to_gas_ratio;
+    for (uint32_t i = 0; i < nt; ++i) {
+        frac[i] /= sum;
+        d->mass_total_code_units = ((p->solid_to_gas_ratio * d->box_length[0]) * d->box_length[1]) * std::sqrt(2. * PI);
+        d->mass_per_particle[i] = (frac[i] * d->mass_total_code_units) / (double)N;
+    }
+}
+
+}  // namespace plan
+
+namespace {
+
+int fail(plan_ctx *c, int code, const std::string &msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+template <class F> int guarded(plan_ctx *c, F f)
+{
+    if (!c) return PLAN_E_ARG;
+    try {
+        HIPCHK(hipSetDevice(c->device));
+        return f();
+    } catch (const HipError &e) {
+        return fail(c, PLAN_E_DEVICE, e.what());
+    } catch (const std::exception &e) {
+        return fail(c, PLAN_E_INTERNAL, e.what());
+    }
+}
+
+void free_tree_nodes(DevTree &t)
+{
+    if (t.ngeo) (void)hipFree(t.ngeo);
+    if (t.ntop) (void)hipFree(t.ntop);
+    if (t.ninfo) (void)hipFree(t.ninfo);
+    t.ngeo = nullptr;
+    t.ntop = nullptr;
+    t.ninfo = nullptr;
+}
+
+void free_particles(DevParticles &p)
+{
+    if (p.pos) (void)hipFree(p.pos);
+    if (p.vel) (void)hipFree(p.vel);
+    if (p.prop) (void)hipFree(p.prop);
+    if (p.id) (void)hipFree(p.id);
+    p.pos = nullptr;
+    p.vel = nullptr;
+    p.prop = nullptr;
+    p.id = nullptr;
+    p.cap = 0;
+}
+
+void ensure_particles(DevParticles &p, uint32_t N)
+{
+    if (p.cap >= N && p.pos) return;
+    free_particles(p);
+    size_t cap = (size_t)N + N / 2 + 64;
+    HIPCHK(hipMalloc(&p.pos, cap * sizeof(double4)));
+    HIPCHK(hipMalloc(&p.vel, cap * sizeof(double4)));
+    HIPCHK(hipMalloc(&p.prop, cap * sizeof(int32_t)));
+    HIPCHK(hipMalloc(&p.id, cap * sizeof(uint32_t)));
+    p.cap = (uint32_t)cap;
+}
+
+__global__ void k_f32_to_set(const float *__restrict__ pos, const float *__restrict__ vel,
+                             const float *__restrict__ dens, uint32_t N, double4 *__restrict__ P,
+                             double4 *__restrict__ V)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    P[i] = make_double4((double)pos[3 * i], (double)pos[3 * i + 1], (double)pos[3 * i + 2], (double)dens[i]);
+    V[i] = make_double4((double)vel[3 * i], (double)vel[3 * i + 1], (double)vel[3 * i + 2], 0.0);
+}
+
+__global__ void k_f64_to_set(const double *__restrict__ pos, const double *__restrict__ vel,
+                             const double *__restrict__ dens, uint32_t N, double4 *__restrict__ P,
+                             double4 *__restrict__ V)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    P[i] = make_double4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], dens[i]);
+    V[i] = make_double4(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2], 0.0);
+}
+
+__global__ void k_rec_of_apos(const uint4 *__restrict__ order, uint32_t na, uint32_t *__restrict__ rec_of_apos)
+{
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= na) return;
+    rec_of_apos[order[j].w] = j;
+}
+
+__global__ void k_clump_peaks(const uint4 *__restrict__ order, const uint32_t *__restrict__ cstart, uint32_t nc,
+                              const uint32_t *__restrict__ a_idx, uint32_t *__restrict__ cpeak)
+{
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    cpeak[c] = a_idx[order[cstart[c]].z];
+}
+
+__global__ void k_hop_full(const uint32_t *__restrict__ apos, uint32_t n, const uint32_t *__restrict__ densest,
+                           const uint32_t *__restrict__ root, const uint32_t *__restrict__ a_idx,
+                           uint32_t *__restrict__ dn_out, uint32_t *__restrict__ pk_out)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t a = apos[i];
+    if (a == PLAN_NONE) {
+        dn_out[i] = i;
+        pk_out[i] = n;
+    } else {
+        dn_out[i] = densest[a];
+        pk_out[i] = a_idx[root[a]];
+    }
+}
+
+__global__ void k_split_pts(const double *__restrict__ pts, const double *__restrict__ est, uint32_t nq,
+                            double4 *__restrict__ q4, double *__restrict__ e)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    q4[i] = make_double4(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], 0.0);
+    e[i] = est[i];
+}
+
+/* device services for the host clump logic */
+struct Services : ClumpServices {
+    plan_ctx *c;
+    explicit Services(plan_ctx *cc) : c(cc) {}
+
+    void saddle_density(const std::vector<double4> &pts, std::vector<double> &rho) override
+    {
+        hipStream_t s = c->stream;
+        uint32_t nq = (uint32_t)pts.size();
+        double4 *q = c->work.get_t<double4>("sd_q", nq);
+        double *e = c->work.get_t<double>("sd_e", nq);
+        double *r = c->work.get_t<double>("sd_r", nq);
+        std::vector<double4> qh(nq);
+        std::vector<double> eh(nq);
+        for (uint32_t i = 0; i < nq; ++i) {
+            qh[i] = make_double4(pts[i].x, pts[i].y, pts[i].z, 0.0);
+            eh[i] = pts[i].w;
+        }
+        HIPCHK(hipMemcpyAsync(q, qh.data(), nq * sizeof(double4), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(e, eh.data(), nq * sizeof(double), hipMemcpyHostToDevice, s));
+        KnnOut ko;
+        ko.rho = r;
+        dev_knn(c->tree, c->K, 0, c->kernel, nq, nullptr, q, e, c->tree.rho, c->d, ko, s);
+        rho.resize(nq);
+        HIPCHK(hipMemcpyAsync(rho.data(), r, nq * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+
+    void run_balls(const DevTree &t, const std::vector<double4> &queries, const double *rho, double rho_min,
+                   int filter, std::vector<std::vector<uint32_t>> &out)
+    {
+        hipStream_t s = c->stream;
+        uint32_t nq = (uint32_t)queries.size();
+        out.assign(nq, {});
+        if (nq == 0) return;
+        double4 *q = c->work.get_t<double4>("bq", nq);
+        uint32_t *cnt = c->work.get_t<uint32_t>("bcnt", nq + 1);
+        uint32_t *off = c->work.get_t<uint32_t>("boff", nq + 1);
+        uint32_t *tot = c->work.get_t<uint32_t>("btot", 4);
+        HIPCHK(hipMemcpyAsync(q, queries.data(), nq * sizeof(double4), hipMemcpyHostToDevice, s));
+        dev_ball(t, nq, q, cnt, nullptr, nullptr, rho, rho_min, filter, s);
+        scan_exclusive_u32(c->work, cnt, off, nq, s, tot);
+        uint32_t total = 0;
+        HIPCHK(hipMemcpyAsync(&total, tot, 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        uint32_t *buf = c->work.get_t<uint32_t>("bout", total + 1);
+        dev_ball(t, nq, q, cnt, off, buf, rho, rho_min, filter, s);
+        std::vector<uint32_t> hoff(nq), hcnt(nq), hb(total);
+        HIPCHK(hipMemcpyAsync(hoff.data(), off, nq * 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(hcnt.data(), cnt, nq * 4, hipMemcpyDeviceToHost, s));
+        if (total) HIPCHK(hipMemcpyAsync(hb.data(), buf, (size_t)total * 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        for (uint32_t i = 0; i < nq; ++i) out[i].assign(hb.begin() + hoff[i], hb.begin() + hoff[i] + hcnt[i]);
+    }
+
+    void clump_tree_balls(const std::vector<double4> &com_peak, const std::vector<double4> &vel,
+                          const std::vector<double4> &queries, std::vector<std::vector<uint32_t>> &peaks) override
+    {
+        hipStream_t s = c->stream;
+        uint32_t n = (uint32_t)com_peak.size();
+        peaks.assign(queries.size(), {});
+        if (n == 0) return;
+        ensure_particles(c->cps, n);
+        std::vector<int32_t> prop(n, 0);
+        std::vector<uint32_t> ids(n);
+        for (uint32_t i = 0; i < n; ++i) ids[i] = i;
+        HIPCHK(hipMemcpyAsync(c->cps.pos, com_peak.data(), n * sizeof(double4), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(c->cps.vel, vel.data(), n * sizeof(double4), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(c->cps.prop, prop.data(), n * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(c->cps.id, ids.data(), n * 4, hipMemcpyHostToDevice, s));
+        c->cps.N = c->cps.T = n;
+        dev_build_tree(c->ctree, c->cps, c->d, false, c->tree.root_center, c->tree.half_width, s);
+        std::vector<std::vector<uint32_t>> idx;
+        run_balls(c->ctree, queries, nullptr, 0.0, 0, idx);
+        std::vector<double4> va(c->ctree.n);
+        HIPCHK(hipMemcpyAsync(va.data(), c->ctree.va, c->ctree.n * sizeof(double4), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        for (size_t qi = 0; qi < queries.size(); ++qi) {
+            peaks[qi].reserve(idx[qi].size());
+            for (uint32_t j : idx[qi]) peaks[qi].push_back((uint32_t)va[j].w);
+        }
+    }
+
+    void particle_balls(const std::vector<double4> &queries, double rho_min,
+                        std::vector<std::vector<uint32_t>> &out) override
+    {
+        run_balls(c->tree, queries, c->tree.rho, rho_min, 1, out);
+    }
+};
+
+double ms_between(hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, a, b);
+    return ms;
+}
+
+/* kernel choice and H (src/tree.hpp:3576-3612) */
+void choose_kernel(plan_ctx *c, uint32_t N, int &kernel, uint32_t &H, int &enough)
+{
+    const plan_params &pp = c->pp;
+    const plan_derived &d = c->d;
+    uint32_t hres = pp.box_resolution[0] * pp.box_resolution[1];
+    if (c->tree.half_width < d.max_half_width + d.max_ghost_zone_width - 1e-16)
+        hres = (uint32_t)std::pow(c->tree.half_width / d.cell_length[0] *
+                                      ((c->tree.half_width - d.max_ghost_zone_width) / c->tree.half_width),
+                                  2.0);
+    H = pp.num_hop;
+    if (N >= 4 * hres) {
+        if (!pp.fixed_num_hop) {
+            if (N >= 1.67e7) H = 64;
+            else if (N >= 2.68e8) H = 128;
+        }
+        enough = 1;
+        kernel = 0;
+    } else {
+        enough = 0;
+        kernel = 1;
+    }
+}
+
+double density_threshold(const plan_ctx *c, uint32_t N)
+{
+    const plan_derived &d = c->d;
+    double thr = std::pow(d.Omega, 2.) / 3.14159265358979323846 / d.grav_constant * 2.;
+    if (d.mass_total_code_units / N / d.cell_volume > thr / 4) thr *= 9. / 8.;
+    return thr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int plan_ctx_create(plan_ctx **out, int device)
+{
+    if (!out) return PLAN_E_ARG;
+    *out = nullptr;
+    plan_ctx *c = new plan_ctx();
+    c->device = device;
+    try {
+        HIPCHK(hipSetDevice(device));
+        HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
+    } catch (const std::exception &e) {
+        delete c;
+        return PLAN_E_DEVICE;
+    }
+    *out = c;
+    return PLAN_OK;
+}
+
+void plan_ctx_destroy(plan_ctx *c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_tree_nodes(c->tree);
+    free_tree_nodes(c->ctree);
+    free_particles(c->ps);
+    free_particles(c->cps);
+    for (auto &e : c->ev) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *plan_last_error(const plan_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+static int set_common(plan_ctx *c, const plan_params *params, uint32_t N)
+{
+    if (!params) return fail(c, PLAN_E_ARG, "params is NULL");
+    if (N == 0) return fail(c, PLAN_E_ARG, "no particles");
+    if (params->num_types == 0 || params->num_types > PLAN_MAX_TYPES)
+        return fail(c, PLAN_E_ARG, "num_types must be 1..16");
+    for (int k = 0; k < 3; ++k) {
+        if (params->box_resolution[k] == 0) return fail(c, PLAN_E_ARG, "box_resolution must be positive");
+        if (!(params->box_max[k] > params->box_min[k])) return fail(c, PLAN_E_ARG, "empty box");
+    }
+    if (params->clump_diffuse_threshold == 0)
+        return fail(c, PLAN_E_ARG, "clump_diffuse_threshold must be larger than 0 (or negative for automatic)");
+    c->pp = *params;
+    derive_params(params, N, &c->d);
+    c->ps.N = c->ps.T = N;
+    c->have_particles = true;
+    c->have_tree = c->have_density = c->have_hop = false;
+    return PLAN_OK;
+}
+
+int plan_set_particles(plan_ctx *c, const plan_params *params, uint32_t N, const double *pos, const double *vel,
+                       const double *density, const int32_t *prop, const uint32_t *id)
+{
+    return guarded(c, [&]() -> int {
+        if (!pos || !vel || !density || !prop || !id) return fail(c, PLAN_E_ARG, "NULL particle array");
+        int r = set_common(c, params, N);
+        if (r) return r;
+        for (uint32_t i = 0; i < N; ++i)
+            if (prop[i] < 0 || (uint32_t)prop[i] >= params->num_types)
+                return fail(c, PLAN_E_ARG, "property_index outside the species table");
+        ensure_particles(c->ps, N);
+        hipStream_t s = c->stream;
+        double *dp = c->work.get_t<double>("up_pos", (size_t)3 * N);
+        double *dv = c->work.get_t<double>("up_vel", (size_t)3 * N);
+        double *dd = c->work.get_t<double>("up_den", N);
+        HIPCHK(hipMemcpyAsync(dp, pos, (size_t)24 * N, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(dv, vel, (size_t)24 * N, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(dd, density, (size_t)8 * N, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(c->ps.prop, prop, (size_t)4 * N, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(c->ps.id, id, (size_t)4 * N, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_f64_to_set, dim3(ceil_div(N, 256)), dim3(256), 0, s, dp, dv, dd, N, c->ps.pos, c->ps.vel);
+        LAUNCHCHK();
+        HIPCHK(hipStreamSynchronize(s));
+        return PLAN_OK;
+    });
+}
+
+int plan_set_particles_f32(plan_ctx *c, const plan_params *params, uint32_t N, const float *pos, const float *vel,
+                           const float *density, const int32_t *prop, const uint32_t *id)
+{
+    return guarded(c, [&]() -> int {
+        if (!pos || !vel || !density || !prop || !id) return fail(c, PLAN_E_ARG, "NULL particle array");
+        int r = set_common(c, params, N);
+        if (r) return r;
+        for (uint32_t i = 0; i < N; ++i)
+            if (prop[i] < 0 || (uint32_t)prop[i] >= params->num_types)
+                return fail(c, PLAN_E_ARG, "property_index outside the species table");
+        ensure_particles(c->ps, N);
+        hipStream_t s = c->stream;
+        float *dp = c->work.get_t<float>("upf_pos", (size_t)3 * N);
+        float *dv = c->work.get_t<float>("upf_vel", (size_t)3 * N);
+        float *dd = c->work.get_t<float>("upf_den", N);
+        HIPCHK(hipMemcpyAsync(dp, pos, (size_t)12 * N, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(dv, vel, (size_t)12 * N, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(dd, density, (size_t)4 * N, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(c->ps.prop, prop, (size_t)4 * N, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(c->ps.id, id, (size_t)4 * N, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_f32_to_set, dim3(ceil_div(N, 256)), dim3(256), 0, s, dp, dv, dd, N, c->ps.pos, c->ps.vel);
+        LAUNCHCHK();
+        HIPCHK(hipStreamSynchronize(s));
+        return PLAN_OK;
+    });
+}
+
+int plan_make_ghost_particles(plan_ctx *c, uint32_t *T_out)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_particles) return fail(c, PLAN_E_STATE, "no particle set loaded");
+        HIPCHK(hipEventRecord(c->ev[0], c->stream));
+        if (!c->pp.no_ghost) dev_make_ghosts(c->ps, c->pp, c->d, c->stream);
+        HIPCHK(hipEventRecord(c->ev[1], c->stream));
+        HIPCHK(hipEventSynchronize(c->ev[1]));
+        c->times[0] = ms_between(c->ev[0], c->ev[1]);
+        if (T_out) *T_out = c->ps.T;
+        return PLAN_OK;
+    });
+}
+
+int plan_build_tree(plan_ctx *c, int check_pos, plan_tree_info *info)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_particles) return fail(c, PLAN_E_STATE, "no particle set loaded");
+        HIPCHK(hipEventRecord(c->ev[0], c->stream));
+        free_tree_nodes(c->tree);
+        double zc[3] = {0, 0, 0};
+        dev_build_tree(c->tree, c->ps, c->d, check_pos != 0, zc, 0.0, c->stream);
+        HIPCHK(hipEventRecord(c->ev[1], c->stream));
+        HIPCHK(hipEventSynchronize(c->ev[1]));
+        c->times[1] = ms_between(c->ev[0], c->ev[1]);
+        c->have_tree = true;
+        c->have_density = c->have_hop = false;
+        if (info) {
+            std::memset(info, 0, sizeof(*info));
+            info->num_particles = c->tree.n;
+            info->num_nodes = c->tree.num_nodes;
+            info->num_leaf_nodes = c->tree.num_leaves;
+            std::vector<uint32_t> sc(c->tree.n);
+            HIPCHK(hipMemcpy(sc.data(), c->tree.sinkcnt, (size_t)c->tree.n * 4, hipMemcpyDeviceToHost));
+            uint32_t ns = 0;
+            for (uint32_t v : sc) ns += v > 1;
+            info->num_sink_particles = ns;
+            info->depth = (uint32_t)c->tree.depth;
+            for (int k = 0; k < 3; ++k) info->root_center[k] = c->tree.root_center[k];
+            info->half_width = c->tree.half_width;
+            for (int l = 0; l < 32; ++l) info->level_count[l] = c->tree.level_count[l];
+        }
+        return PLAN_OK;
+    });
+}
+
+static int run_density(plan_ctx *c, uint32_t K, int kernel)
+{
+    if (!c->have_tree) return fail(c, PLAN_E_STATE, "no tree built");
+    if (K == 0 || K > c->tree.n) return fail(c, PLAN_E_ARG, "K must be in 1..num_particles");
+    if (kernel < 0) {
+        uint32_t H;
+        int enough;
+        choose_kernel(c, c->ps.N, kernel, H, enough);
+    }
+    c->K = K;
+    c->kernel = kernel;
+    KnnOut ko;
+    ko.rho = c->tree.rho;
+    HIPCHK(hipEventRecord(c->ev[2], c->stream));
+    dev_knn(c->tree, K, 0, kernel, c->tree.n, nullptr, nullptr, nullptr, nullptr, c->d, ko, c->stream);
+    HIPCHK(hipEventRecord(c->ev[3], c->stream));
+    HIPCHK(hipEventSynchronize(c->ev[3]));
+    c->density_kernel_ms = ms_between(c->ev[2], c->ev[3]);
+    c->times[2] = c->density_kernel_ms;
+    c->have_density = true;
+    return PLAN_OK;
+}
+
+int plan_knn_density(plan_ctx *c, uint32_t K, int kernel)
+{
+    return guarded(c, [&]() -> int { return run_density(c, K, kernel); });
+}
+
+static int run_hop(plan_ctx *c, uint32_t H, double thr)
+{
+    if (!c->have_density) return fail(c, PLAN_E_STATE, "densities not computed");
+    if (H == 0 || H > c->tree.n) return fail(c, PLAN_E_ARG, "H must be in 1..num_particles");
+    HIPCHK(hipEventRecord(c->ev[4], c->stream));
+    dev_hop(c->tree, c->work, c->d, thr, H, c->hop, c->stream);
+    HIPCHK(hipEventRecord(c->ev[5], c->stream));
+    HIPCHK(hipEventSynchronize(c->ev[5]));
+    c->times[3] = ms_between(c->ev[4], c->ev[5]);
+    c->H = H;
+    c->have_hop = true;
+    return PLAN_OK;
+}
+
+int plan_hop(plan_ctx *c, uint32_t H, double threshold, uint32_t *num_above, uint32_t *num_groups)
+{
+    return guarded(c, [&]() -> int {
+        int r = run_hop(c, H, threshold);
+        if (r) return r;
+        if (num_above) *num_above = c->hop.na;
+        if (num_groups) *num_groups = c->hop.nclumps;
+        return PLAN_OK;
+    });
+}
+
+int plan_find_planetesimals(plan_ctx *c, plan_catalog **out)
+{
+    return guarded(c, [&]() -> int {
+        if (!out) return fail(c, PLAN_E_ARG, "out is NULL");
+        *out = nullptr;
+        if (!c->have_tree) return fail(c, PLAN_E_STATE, "no tree built");
+        auto t0 = std::chrono::steady_clock::now();
+        const uint32_t N = c->ps.N;
+        int kernel, enough;
+        uint32_t H;
+        choose_kernel(c, N, kernel, H, enough);
+        uint32_t K = c->pp.num_knn;
+        if (K == 0 || K > c->tree.n || H > c->tree.n) return fail(c, PLAN_E_ARG, "neighbour count exceeds particles");
+        const double thr = density_threshold(c, N);
+        int r = run_density(c, K, kernel);
+        if (r) return r;
+        r = run_hop(c, H, thr);
+        if (r) return r;
+        auto t1 = std::chrono::steady_clock::now();
+        /* member table to the host */
+        hipStream_t s = c->stream;
+        HostMembers hm;
+        const uint32_t na = c->hop.na, nc = c->hop.nclumps;
+        hm.rec.resize(na);
+        hm.sinkcnt.resize(na);
+        hm.cstart.resize(nc + 1);
+        hm.cpeak.resize(nc);
+        hm.sorted_tidx.resize(na);
+        hm.rec_of_sorted.resize(na);
+        if (na) {
+            uint32_t *roa = c->work.get_t<uint32_t>("rec_of_apos", na);
+            uint32_t *cpk = c->work.get_t<uint32_t>("cpeak", nc + 1);
+            hipLaunchKernelGGL(k_rec_of_apos, dim3(ceil_div(na, 256)), dim3(256), 0, s, c->hop.order, na, roa);
+            LAUNCHCHK();
+            hipLaunchKernelGGL(k_clump_peaks, dim3(ceil_div(nc, 256)), dim3(256), 0, s, c->hop.order, c->hop.cstart,
+                               nc, c->hop.a_idx, cpk);
+            LAUNCHCHK();
+            HIPCHK(hipMemcpyAsync(hm.rec.data(), c->hop.members, (size_t)na * sizeof(MemberRec), hipMemcpyDeviceToHost,
+                                  s));
+            HIPCHK(hipMemcpyAsync(hm.sinkcnt.data(), c->hop.msink, (size_t)na * 4, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(hm.cstart.data(), c->hop.cstart, (size_t)nc * 4, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(hm.cpeak.data(), cpk, (size_t)nc * 4, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(hm.sorted_tidx.data(), c->hop.a_idx, (size_t)na * 4, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(hm.rec_of_sorted.data(), roa, (size_t)na * 4, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+        }
+        hm.cstart[nc] = na;
+        ClumpSettings cs;
+        cs.K = K;
+        cs.kernel = kernel;
+        cs.enough_resolution = enough;
+        cs.density_threshold = thr;
+        cs.peak_density_threshold = 3 * thr;
+        cs.clump_mass_threshold = c->d.min_trusted_mass_code_unit;
+        {
+            double mn = 1e300;
+            for (int k = 0; k < 3; ++k) mn = std::min(mn, (double)c->pp.box_resolution[k] / c->d.box_length[k]);
+            unsigned hrh = (unsigned)mn;
+            cs.diffuse = 0.55;
+            cs.hill_frac = 0.25;
+            if (hrh <= 1024) {
+                cs.diffuse = 0.35;
+                cs.hill_frac = 0.75;
+            } else if (hrh <= 1536) {
+                cs.diffuse = 0.4;
+                cs.hill_frac = 0.5;
+            } else if (hrh <= 2048) {
+                cs.diffuse = 0.5;
+                cs.hill_frac = 0.35;
+            }
+            if (c->pp.clump_diffuse_threshold > 0) cs.diffuse = c->pp.clump_diffuse_threshold;
+            if (c->pp.Hill_fraction_for_merge > 0) cs.hill_frac = c->pp.Hill_fraction_for_merge;
+        }
+        cs.G = c->d.grav_constant;
+        cs.Omega = c->d.Omega;
+        cs.q = c->d.q;
+        for (int k = 0; k < 3; ++k) {
+            cs.box_min[k] = c->pp.box_min[k];
+            cs.box_max[k] = c->pp.box_max[k];
+        }
+        cs.num_peaks = c->pp.num_peaks;
+        Services svc(c);
+        std::map<uint32_t, HClump> cl;
+        double Mp_max = 0, Mp_tot = 0;
+        ClumpStats st;
+        run_clump_pipeline(hm, cs, svc, cl, Mp_max, Mp_tot, st);
+        /* catalog */
+        plan_catalog *cat = (plan_catalog *)std::calloc(1, sizeof(plan_catalog));
+        uint32_t tot = 0;
+        for (auto &it : cl) tot += (uint32_t)it.second.particles.size();
+        cat->num_clumps = (uint32_t)cl.size();
+        cat->num_members_total = tot;
+        cat->clumps = (plan_clump *)std::calloc(cl.size() ? cl.size() : 1, sizeof(plan_clump));
+        cat->members = (uint32_t *)std::calloc(tot ? tot : 1, 4);
+        cat->member_original_ids = (uint32_t *)std::calloc(tot ? tot : 1, 4);
+        cat->density_threshold = thr;
+        cat->peak_density_threshold = cs.peak_density_threshold;
+        cat->clump_mass_threshold = cs.clump_mass_threshold;
+        cat->Mp_max = Mp_max;
+        cat->Mp_tot = Mp_tot;
+        uint32_t off = 0, i = 0;
+        for (auto &it : cl) {
+            const HClump &q = it.second;
+            plan_clump &o = cat->clumps[i++];
+            o.peak_index = it.first;
+            o.num_members = (uint32_t)q.particles.size();
+            uint32_t np = 0;
+            for (uint32_t h : q.indices) np += hm.sinkcnt[h];
+            o.num_particles = np;
+            o.member_offset = off;
+            o.total_mass = q.total_mass;
+            for (int k = 0; k < 3; ++k) {
+                o.center_of_mass[k] = q.com[k];
+                o.vel_com[k] = q.vel[k];
+                o.J[k] = q.J[k];
+                o.J_quarter_Hill[k] = q.Jq[k];
+            }
+            o.Hill_radius = q.hill;
+            o.one10th_radius = q.one10th;
+            o.two_sigma_mass_radius = q.two_sigma;
+            o.max_distance = q.particles.empty() ? 0.0 : q.particles.back().second;
+            uint32_t kr = hm.rec_of_tidx(it.first);
+            o.peak_density = kr == PLAN_NONE ? 0.0 : hm.rec[kr].rho;
+            for (auto &m : q.particles) {
+                cat->members[off] = hm.rec[m.first].tidx;
+                cat->member_original_ids[off] = hm.rec[m.first].oid;
+                ++off;
+            }
+        }
+        auto t2 = std::chrono::steady_clock::now();
+        c->times[4] = std::chrono::duration<double, std::milli>(t2 - t1).count();
+        c->times[5] = std::chrono::duration<double, std::milli>(t2 - t0).count();
+        *out = cat;
+        return PLAN_OK;
+    });
+}
+
+void plan_catalog_free(plan_catalog *cat)
+{
+    if (!cat) return;
+    std::free(cat->clumps);
+    std::free(cat->members);
+    std::free(cat->member_original_ids);
+    std::free(cat);
+}
+
+int plan_get_tree_particles(plan_ctx *c, double *pos, double *vel, double *mass, double *ath, double *rho,
+                            uint32_t *oid, uint32_t *sinkcnt)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_tree) return fail(c, PLAN_E_STATE, "no tree built");
+        const uint32_t n = c->tree.n;
+        std::vector<double4> pm(n), va(n);
+        HIPCHK(hipMemcpy(pm.data(), c->tree.pm, (size_t)n * 32, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(va.data(), c->tree.va, (size_t)n * 32, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < n; ++i) {
+            if (pos) {
+                pos[3 * i] = pm[i].x;
+                pos[3 * i + 1] = pm[i].y;
+                pos[3 * i + 2] = pm[i].z;
+            }
+            if (vel) {
+                vel[3 * i] = va[i].x;
+                vel[3 * i + 1] = va[i].y;
+                vel[3 * i + 2] = va[i].z;
+            }
+            if (mass) mass[i] = pm[i].w;
+            if (ath) ath[i] = va[i].w;
+        }
+        if (rho) HIPCHK(hipMemcpy(rho, c->tree.rho, (size_t)n * 8, hipMemcpyDeviceToHost));
+        if (oid) HIPCHK(hipMemcpy(oid, c->tree.oid, (size_t)n * 4, hipMemcpyDeviceToHost));
+        if (sinkcnt) HIPCHK(hipMemcpy(sinkcnt, c->tree.sinkcnt, (size_t)n * 4, hipMemcpyDeviceToHost));
+        return PLAN_OK;
+    });
+}
+
+int plan_get_keys(plan_ctx *c, uint32_t *w0, uint32_t *w1, uint32_t *w2)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_tree) return fail(c, PLAN_E_STATE, "no tree built");
+        const uint32_t n = c->tree.n;
+        std::vector<uint4> k(n);
+        HIPCHK(hipMemcpy(k.data(), c->tree.keys, (size_t)n * 16, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < n; ++i) {
+            if (w0) w0[i] = k[i].x;
+            if (w1) w1[i] = k[i].y;
+            if (w2) w2[i] = k[i].z;
+        }
+        return PLAN_OK;
+    });
+}
+
+int plan_get_nodes(plan_ctx *c, double *center, double *hw, uint32_t *begin, uint32_t *end, uint32_t *parent,
+                   uint32_t *first_daughter, uint8_t *num_daughters, uint8_t *level, uint16_t *orthant,
+                   int32_t *node2leaf, int32_t *leaf_nodes)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_tree) return fail(c, PLAN_E_STATE, "no tree built");
+        const uint32_t nn = c->tree.num_nodes;
+        std::vector<NodeGeo> g(nn);
+        std::vector<NodeTop> t(nn);
+        std::vector<uint32_t> info(nn);
+        HIPCHK(hipMemcpy(g.data(), c->tree.ngeo, (size_t)nn * sizeof(NodeGeo), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(t.data(), c->tree.ntop, (size_t)nn * sizeof(NodeTop), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(info.data(), c->tree.ninfo, (size_t)nn * 4, hipMemcpyDeviceToHost));
+        for (uint32_t v = 0; v < nn; ++v) {
+            if (center) {
+                center[3 * v] = g[v].cx;
+                center[3 * v + 1] = g[v].cy;
+                center[3 * v + 2] = g[v].cz;
+            }
+            if (hw) hw[v] = g[v].hw;
+            if (begin) begin[v] = t[v].begin;
+            if (end) end[v] = t[v].end;
+            if (parent) parent[v] = t[v].parent;
+            if (first_daughter) first_daughter[v] = t[v].first_daughter;
+            if (num_daughters) num_daughters[v] = (uint8_t)(info[v] & 0xff);
+            if (level) level[v] = (uint8_t)((info[v] >> 8) & 0xff);
+            if (orthant) orthant[v] = (uint16_t)(info[v] >> 16);
+        }
+        if (node2leaf) HIPCHK(hipMemcpy(node2leaf, c->tree.node2leaf, (size_t)nn * 4, hipMemcpyDeviceToHost));
+        if (leaf_nodes)
+            HIPCHK(hipMemcpy(leaf_nodes, c->tree.leaf_nodes, (size_t)c->tree.num_leaves * 4, hipMemcpyDeviceToHost));
+        return PLAN_OK;
+    });
+}
+
+int plan_get_hop(plan_ctx *c, uint32_t *densest, uint32_t *peak)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_hop) return fail(c, PLAN_E_STATE, "hop not run");
+        const uint32_t n = c->tree.n;
+        uint32_t *dn = c->work.get_t<uint32_t>("gh_dn", n);
+        uint32_t *pk = c->work.get_t<uint32_t>("gh_pk", n);
+        hipLaunchKernelGGL(k_hop_full, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, c->hop.apos, n,
+                           c->hop.densest, c->hop.root, c->hop.a_idx, dn, pk);
+        LAUNCHCHK();
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (densest) HIPCHK(hipMemcpy(densest, dn, (size_t)n * 4, hipMemcpyDeviceToHost));
+        if (peak) HIPCHK(hipMemcpy(peak, pk, (size_t)n * 4, hipMemcpyDeviceToHost));
+        return PLAN_OK;
+    });
+}
+
+int plan_knn_query(plan_ctx *c, uint32_t nq, const uint32_t *pidx, const double *pts, const double *est, uint32_t K,
+                   uint32_t *idx, double *d2, double *rstop)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_tree) return fail(c, PLAN_E_STATE, "no tree built");
+        if (!pidx && (!pts || !est)) return fail(c, PLAN_E_ARG, "point queries need pts and est");
+        if (K == 0 || K > c->tree.n) return fail(c, PLAN_E_ARG, "K must be in 1..num_particles");
+        if (nq == 0) return PLAN_OK;
+        hipStream_t s = c->stream;
+        uint32_t *di = c->work.get_t<uint32_t>("kq_idx", (size_t)nq * K);
+        double *dd = c->work.get_t<double>("kq_d2", (size_t)nq * K);
+        double *dr = c->work.get_t<double>("kq_r", nq);
+        KnnOut ko;
+        ko.idx = di;
+        ko.d2 = dd;
+        ko.rstop = dr;
+        if (pidx) {
+            for (uint32_t i = 0; i < nq; ++i)
+                if (pidx[i] >= c->tree.n) return fail(c, PLAN_E_ARG, "particle index out of range");
+            uint32_t *dq = c->work.get_t<uint32_t>("kq_q", nq);
+            HIPCHK(hipMemcpyAsync(dq, pidx, (size_t)nq * 4, hipMemcpyHostToDevice, s));
+            double *de = nullptr;
+            if (est) {
+                de = c->work.get_t<double>("kq_e", nq);
+                HIPCHK(hipMemcpyAsync(de, est, (size_t)nq * 8, hipMemcpyHostToDevice, s));
+            }
+            dev_knn(c->tree, K, 2, 0, nq, dq, nullptr, de, c->tree.rho, c->d, ko, s);
+        } else {
+            for (uint32_t i = 0; i < nq; ++i)
+                if (!(est[i] > 0)) return fail(c, PLAN_E_ARG, "point queries need a positive start radius");
+            double *dp = c->work.get_t<double>("kq_p", (size_t)nq * 3);
+            double *de = c->work.get_t<double>("kq_e", nq);
+            double4 *q4 = c->work.get_t<double4>("kq_q4", nq);
+            double *e2 = c->work.get_t<double>("kq_e2", nq);
+            HIPCHK(hipMemcpyAsync(dp, pts, (size_t)nq * 24, hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(de, est, (size_t)nq * 8, hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_split_pts, dim3(ceil_div(nq, 256)), dim3(256), 0, s, dp, de, nq, q4, e2);
+            LAUNCHCHK();
+            dev_knn(c->tree, K, 2, 0, nq, nullptr, q4, e2, c->tree.rho, c->d, ko, s);
+        }
+        HIPCHK(hipStreamSynchronize(s));
+        if (idx) HIPCHK(hipMemcpy(idx, di, (size_t)nq * K * 4, hipMemcpyDeviceToHost));
+        if (d2) HIPCHK(hipMemcpy(d2, dd, (size_t)nq * K * 8, hipMemcpyDeviceToHost));
+        if (rstop) HIPCHK(hipMemcpy(rstop, dr, (size_t)nq * 8, hipMemcpyDeviceToHost));
+        return PLAN_OK;
+    });
+}
+
+int plan_density_at(plan_ctx *c, uint32_t nq, const double *pts, const double *est, uint32_t K, int kernel,
+                    double *rho)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_tree) return fail(c, PLAN_E_STATE, "no tree built");
+        if (!pts || !est || !rho) return fail(c, PLAN_E_ARG, "NULL argument");
+        if (K == 0 || K > c->tree.n) return fail(c, PLAN_E_ARG, "K must be in 1..num_particles");
+        std::vector<double4> q(nq);
+        for (uint32_t i = 0; i < nq; ++i) {
+            if (!(est[i] > 0)) return fail(c, PLAN_E_ARG, "point queries need a positive start radius");
+            q[i] = make_double4(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], est[i]);
+        }
+        uint32_t oldK = c->K;
+        int oldk = c->kernel;
+        c->K = K;
+        c->kernel = kernel;
+        std::vector<double> r;
+        Services svc(c);
+        svc.saddle_density(q, r);
+        c->K = oldK;
+        c->kernel = oldk;
+        std::memcpy(rho, r.data(), (size_t)nq * 8);
+        return PLAN_OK;
+    });
+}
+
+int plan_ball_query(plan_ctx *c, const double *center, double radius, uint32_t *out, uint32_t cap, uint32_t *count)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_tree) return fail(c, PLAN_E_STATE, "no tree built");
+        if (!center || !count) return fail(c, PLAN_E_ARG, "NULL argument");
+        Services svc(c);
+        std::vector<std::vector<uint32_t>> l;
+        svc.run_balls(c->tree, {make_double4(center[0], center[1], center[2], radius)}, nullptr, 0.0, 0, l);
+        *count = (uint32_t)l[0].size();
+        if (out)
+            for (uint32_t i = 0; i < *count && i < cap; ++i) out[i] = l[0][i];
+        return PLAN_OK;
+    });
+}
+
+int plan_stage_times(plan_ctx *c, double *ms, int n)
+{
+    if (!c || !ms) return PLAN_E_ARG;
+    for (int i = 0; i < n && i < 6; ++i) ms[i] = c->times[i];
+    return PLAN_OK;
+}
+
+int plan_last_density_kernel_ms(plan_ctx *c, double *ms)
+{
+    if (!c || !ms) return PLAN_E_ARG;
+    *ms = c->density_kernel_ms;
+    return PLAN_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,625 @@
+// build.hip — ghost particles, Morton keys, key sort, sink merging and the
+// level-major octree, all on the device.
+//
+// Reference behaviour restated: ParticleSet::MakeGhostParticles
+// (src/tree.hpp:1626-1754); BHtree::BuildTree (src/tree.hpp:2759-2908) with
+// SortPoints (2608-2651), MakeSinkParticle (2591-2604), CountNodesLeaves /
+// FillTree (2655-2755).  The recursive count/fill passes become one
+// level-synchronous pass per tree level: at level L every particle of a
+// non-leaf node flags the start of its octant run; a prefix sum over the
+// flags numbers the level-(L+1) nodes in key order, which is exactly the
+// reference's level-major, DFS-within-level numbering.
+#include "ctx.h"
+#include "morton.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace plan {
+
+static constexpr int BLK = 256;
+
+/* ---------------------------------------------------------------- ghosts */
+
+struct GhostGeom {
+    double ngmin[3], ngmax[3];
+    double Lx, Ly, ymin, ymax, shear_t;
+};
+
+__global__ void k_ghost_count(const double4 *__restrict__ pos, uint32_t N, GhostGeom g, uint32_t *__restrict__ cx,
+                              uint32_t *__restrict__ cy)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    double4 p = pos[i];
+    cx[i] = (uint32_t)(p.x > g.ngmax[0]) + (uint32_t)(p.x < g.ngmin[0]);
+    cy[i] = (uint32_t)(p.y < g.ngmin[1]) + (uint32_t)(p.y > g.ngmax[1]);
+}
+
+__device__ __forceinline__ void copy_particle(double4 *pos, double4 *vel, int32_t *prop, uint32_t *id, uint32_t dst,
+                                              uint32_t src, double4 p)
+{
+    pos[dst] = p;
+    vel[dst] = vel[src];
+    prop[dst] = prop[src];
+    id[dst] = dst;
+}
+
+/* x images with the shearing-box y offset (src/tree.hpp:1660-1681) */
+__global__ void k_ghost_x(double4 *pos, double4 *vel, int32_t *prop, uint32_t *id, uint32_t N,
+                          const uint32_t *__restrict__ ox, GhostGeom g)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    double4 p = pos[i];
+    uint32_t slot = N + ox[i];
+    if (p.x > g.ngmax[0]) {
+        double4 q = p;
+        q.x = __dsub_rn(q.x, g.Lx);
+        double ny = __dadd_rn(q.y, g.shear_t);
+        double k = (double)(int)(__ddiv_rn(__dsub_rn(ny, g.ymin), g.Ly));
+        q.y = __dsub_rn(ny, __dmul_rn(k, g.Ly));
+        copy_particle(pos, vel, prop, id, slot++, i, q);
+    }
+    if (p.x < g.ngmin[0]) {
+        double4 q = p;
+        q.x = __dadd_rn(q.x, g.Lx);
+        double ny = __dsub_rn(q.y, g.shear_t);
+        double k = (double)(int)(__ddiv_rn(__dsub_rn(g.ymax, ny), g.Ly));
+        q.y = __dadd_rn(ny, __dmul_rn(k, g.Ly));
+        copy_particle(pos, vel, prop, id, slot, i, q);
+    }
+}
+
+__global__ void k_ghost_count_y(const double4 *__restrict__ pos, uint32_t first, uint32_t cnt, GhostGeom g,
+                                uint32_t *__restrict__ cy)
+{
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= cnt) return;
+    double y = pos[first + j].y;
+    cy[j] = (uint32_t)(y < g.ngmin[1]) + (uint32_t)(y > g.ngmax[1]);
+}
+
+/* y images (src/tree.hpp:1685-1720) of particles [first, first+cnt) */
+__global__ void k_ghost_y(double4 *pos, double4 *vel, int32_t *prop, uint32_t *id, uint32_t first, uint32_t cnt,
+                          uint32_t out_base, const uint32_t *__restrict__ oy, GhostGeom g)
+{
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= cnt) return;
+    uint32_t src = first + j;
+    double4 p = pos[src];
+    uint32_t slot = out_base + oy[j];
+    if (p.y < g.ngmin[1]) {
+        double4 q = p;
+        q.y = __dadd_rn(q.y, g.Ly);
+        copy_particle(pos, vel, prop, id, slot++, src, q);
+    }
+    if (p.y > g.ngmax[1]) {
+        double4 q = p;
+        q.y = __dsub_rn(q.y, g.Ly);
+        copy_particle(pos, vel, prop, id, slot, src, q);
+    }
+}
+
+static uint32_t read_u32(const uint32_t *dp, hipStream_t s)
+{
+    uint32_t v;
+    HIPCHK(hipMemcpyAsync(&v, dp, 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return v;
+}
+
+void dev_make_ghosts(DevParticles &ps, const plan_params &pp, const plan_derived &d, hipStream_t s)
+{
+    const uint32_t N = ps.N;
+    GhostGeom g;
+    for (int k = 0; k < 3; ++k) {
+        double w = d.box_half_width[k] - d.ghost_zone_width[k];
+        if (w < 0) w = 0;
+        g.ngmin[k] = d.box_center[k] - w;
+        g.ngmax[k] = d.box_center[k] + w;
+    }
+    g.Lx = d.box_length[0];
+    g.Ly = d.box_length[1];
+    g.ymin = pp.box_min[1];
+    g.ymax = pp.box_max[1];
+    g.shear_t = d.shear_speed * pp.time;
+    Arena &a = ps.arena;
+    uint32_t *cx = a.get_t<uint32_t>("g_cx", N + 1);
+    uint32_t *cy = a.get_t<uint32_t>("g_cy", N + 1);
+    uint32_t *ox = a.get_t<uint32_t>("g_ox", N + 1);
+    uint32_t *oy = a.get_t<uint32_t>("g_oy", N + 1);
+    uint32_t *tot = a.get_t<uint32_t>("g_tot", 4);
+    hipLaunchKernelGGL(k_ghost_count, dim3(ceil_div(N, BLK)), dim3(BLK), 0, s, ps.pos, N, g, cx, cy);
+    LAUNCHCHK();
+    scan_exclusive_u32(a, cx, ox, N, s, tot);
+    scan_exclusive_u32(a, cy, oy, N, s, tot + 1);
+    uint32_t tt[2];
+    HIPCHK(hipMemcpyAsync(tt, tot, 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const uint32_t GX = tt[0], GYO = tt[1];
+    /* capacity: the y images of the x images are bounded by 2 GX */
+    uint64_t need = (uint64_t)N + GX + 2ull * GX + GYO;
+    if (need >= 0xfffffffeull) throw std::runtime_error("too many particles for 32-bit tree indices");
+    if (need > ps.cap) {
+        /* grow, preserving the N originals */
+        double4 *np = static_cast<double4 *>(nullptr);
+        double4 *nv = nullptr;
+        int32_t *npr = nullptr;
+        uint32_t *nid = nullptr;
+        HIPCHK(hipMalloc(&np, need * sizeof(double4)));
+        HIPCHK(hipMalloc(&nv, need * sizeof(double4)));
+        HIPCHK(hipMalloc(&npr, need * sizeof(int32_t)));
+        HIPCHK(hipMalloc(&nid, need * sizeof(uint32_t)));
+        HIPCHK(hipMemcpyAsync(np, ps.pos, (size_t)N * sizeof(double4), hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(nv, ps.vel, (size_t)N * sizeof(double4), hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(npr, ps.prop, (size_t)N * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(nid, ps.id, (size_t)N * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(hipFree(ps.pos));
+        HIPCHK(hipFree(ps.vel));
+        HIPCHK(hipFree(ps.prop));
+        HIPCHK(hipFree(ps.id));
+        ps.pos = np;
+        ps.vel = nv;
+        ps.prop = npr;
+        ps.id = nid;
+        ps.cap = (uint32_t)need;
+    }
+    if (GX) {
+        hipLaunchKernelGGL(k_ghost_x, dim3(ceil_div(N, BLK)), dim3(BLK), 0, s, ps.pos, ps.vel, ps.prop, ps.id, N, ox,
+                           g);
+        LAUNCHCHK();
+    }
+    uint32_t *cyx = a.get_t<uint32_t>("g_cyx", GX + 1);
+    uint32_t *oyx = a.get_t<uint32_t>("g_oyx", GX + 1);
+    uint32_t GYX = 0;
+    if (GX) {
+        hipLaunchKernelGGL(k_ghost_count_y, dim3(ceil_div(GX, BLK)), dim3(BLK), 0, s, ps.pos, N, GX, g, cyx);
+        LAUNCHCHK();
+        scan_exclusive_u32(a, cyx, oyx, GX, s, tot + 2);
+        GYX = read_u32(tot + 2, s);
+        hipLaunchKernelGGL(k_ghost_y, dim3(ceil_div(GX, BLK)), dim3(BLK), 0, s, ps.pos, ps.vel, ps.prop, ps.id, N,
+                           GX, N + GX, oyx, g);
+        LAUNCHCHK();
+    }
+    if (GYO) {
+        hipLaunchKernelGGL(k_ghost_y, dim3(ceil_div(N, BLK)), dim3(BLK), 0, s, ps.pos, ps.vel, ps.prop, ps.id, 0u, N,
+                           N + GX + GYX, oy, g);
+        LAUNCHCHK();
+    }
+    ps.T = N + GX + GYX + GYO;
+}
+
+/* ------------------------------------------------------------ tree build */
+
+__global__ void k_bbox(const double4 *__restrict__ pos, uint32_t T, double *__restrict__ out)
+{
+    __shared__ double sm[6][BLK];
+    double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < T; i += gridDim.x * blockDim.x) {
+        double4 p = pos[i];
+        mn[0] = fmin(mn[0], p.x);
+        mn[1] = fmin(mn[1], p.y);
+        mn[2] = fmin(mn[2], p.z);
+        mx[0] = fmax(mx[0], p.x);
+        mx[1] = fmax(mx[1], p.y);
+        mx[2] = fmax(mx[2], p.z);
+    }
+    for (int k = 0; k < 3; ++k) {
+        sm[k][threadIdx.x] = mn[k];
+        sm[3 + k][threadIdx.x] = mx[k];
+    }
+    __syncthreads();
+    for (int o = BLK / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            for (int k = 0; k < 3; ++k) {
+                sm[k][threadIdx.x] = fmin(sm[k][threadIdx.x], sm[k][threadIdx.x + o]);
+                sm[3 + k][threadIdx.x] = fmax(sm[3 + k][threadIdx.x], sm[3 + k][threadIdx.x + o]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) out[blockIdx.x * 6 + threadIdx.x] = sm[threadIdx.x][0];
+}
+
+struct MortonFrame {
+    double boxmin[3], scale[3];
+};
+
+__global__ void k_morton(const double4 *__restrict__ pos, uint32_t T, MortonFrame f, uint4 *__restrict__ keys)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= T) return;
+    double4 p = pos[i];
+    uint32_t ix = double_to_grid(__dmul_rn(__dsub_rn(p.x, f.boxmin[0]), f.scale[0]));
+    uint32_t iy = double_to_grid(__dmul_rn(__dsub_rn(p.y, f.boxmin[1]), f.scale[1]));
+    uint32_t iz = double_to_grid(__dmul_rn(__dsub_rn(p.z, f.boxmin[2]), f.scale[2]));
+    keys[i] = morton_key(ix, iy, iz, i);
+}
+
+__global__ void k_run_flags(const uint4 *__restrict__ keys, uint32_t T, uint32_t *__restrict__ flags)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= T) return;
+    flags[i] = (i == 0) ? 1u : (key_eq(keys[i], keys[i - 1]) ? 0u : 1u);
+}
+
+__global__ void k_run_starts(const uint32_t *__restrict__ flags, const uint32_t *__restrict__ ridx, uint32_t T,
+                             uint32_t *__restrict__ start)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= T) return;
+    if (flags[i]) start[ridx[i]] = i;
+}
+
+/* gather the unique particles in key order; runs of equal 96-bit keys become
+ * one sink particle (first of the run represents; mass and momentum summed
+ * in run order) */
+__global__ void k_gather_particles(const uint4 *__restrict__ skeys, const uint32_t *__restrict__ start, uint32_t n,
+                                   uint32_t T, const double4 *__restrict__ ppos, const double4 *__restrict__ pvel,
+                                   const int32_t *__restrict__ pprop, const uint32_t *__restrict__ pid,
+                                   const double *__restrict__ mass_of_type, uint32_t ntypes, double4 *__restrict__ pm,
+                                   double4 *__restrict__ va, uint32_t *__restrict__ oid, uint32_t *__restrict__ sinkcnt,
+                                   uint32_t *__restrict__ sinkoff, uint4 *__restrict__ ukeys)
+{
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    uint32_t s = start[r];
+    uint32_t e = (r + 1 < n) ? start[r + 1] : T;
+    uint4 k = skeys[s];
+    uint32_t src = k.w;
+    double4 p = ppos[src];
+    double4 v = pvel[src];
+    int32_t ty = pprop[src];
+    double m = mass_of_type[(ty >= 0 && (uint32_t)ty < ntypes) ? ty : 0];
+    if (e - s > 1) {
+        double ms = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
+        for (uint32_t j = s; j < e; ++j) {
+            uint32_t q = skeys[j].w;
+            int32_t tq = pprop[q];
+            double mq = mass_of_type[(tq >= 0 && (uint32_t)tq < ntypes) ? tq : 0];
+            double4 vq = pvel[q];
+            vx = __dadd_rn(vx, __dmul_rn(mq, vq.x));
+            vy = __dadd_rn(vy, __dmul_rn(mq, vq.y));
+            vz = __dadd_rn(vz, __dmul_rn(mq, vq.z));
+            ms = __dadd_rn(ms, mq);
+        }
+        m = ms;
+        v.x = __ddiv_rn(vx, ms);
+        v.y = __ddiv_rn(vy, ms);
+        v.z = __ddiv_rn(vz, ms);
+    }
+    pm[r] = make_double4(p.x, p.y, p.z, m);
+    va[r] = make_double4(v.x, v.y, v.z, p.w);
+    oid[r] = pid[src];
+    sinkcnt[r] = e - s;
+    sinkoff[r] = s;
+    ukeys[r] = make_uint4(k.x, k.y, k.z, s);
+}
+
+struct Level {
+    int L;              /* parent level (1..31) */
+    uint32_t base;      /* first node id of level L+1 */
+};
+
+__global__ void k_lvl_flags(const uint4 *__restrict__ keys, uint32_t n, const uint32_t *__restrict__ pslot,
+                            const uint32_t *__restrict__ plist, const NodeTop *__restrict__ ntop, int L,
+                            uint32_t *__restrict__ flags)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t k = pslot[i];
+    uint32_t f = 0;
+    if (k != PLAN_NONE) {
+        uint32_t b = ntop[plist[k]].begin;
+        f = (i == b) || (key_digit(keys[i], L) != key_digit(keys[i - 1], L));
+    }
+    flags[i] = f;
+}
+
+__global__ void k_lvl_starts(const uint32_t *__restrict__ flags, const uint32_t *__restrict__ cidx, uint32_t n,
+                             uint32_t *__restrict__ cstart)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (flags[i]) cstart[cidx[i]] = i;
+}
+
+__global__ void k_lvl_nodes(const uint4 *__restrict__ keys, const uint32_t *__restrict__ cstart, uint32_t C,
+                            const uint32_t *__restrict__ pslot, const uint32_t *__restrict__ plist, int L,
+                            uint32_t base, NodeGeo *__restrict__ ngeo, NodeTop *__restrict__ ntop,
+                            uint32_t *__restrict__ ninfo, uint32_t *__restrict__ nonleaf)
+{
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= C) return;
+    uint32_t i = cstart[r];
+    uint32_t k = pslot[i];
+    uint32_t P = plist[k];
+    NodeTop pt = ntop[P];
+    NodeGeo pg = ngeo[P];
+    uint32_t e = (r + 1 < C && pslot[cstart[r + 1]] == k) ? cstart[r + 1] : pt.end;
+    uint32_t o = key_digit(keys[i], L);
+    double hw = __dmul_rn(0.5, pg.hw);
+    NodeGeo g;
+    g.cx = __dadd_rn(pg.cx, (o & 1u) ? hw : -hw);
+    g.cy = __dadd_rn(pg.cy, (o & 2u) ? hw : -hw);
+    g.cz = __dadd_rn(pg.cz, (o & 4u) ? hw : -hw);
+    g.hw = hw;
+    uint32_t node = base + r;
+    ngeo[node] = g;
+    ntop[node] = NodeTop{i, e, 0u, P};
+    ninfo[node] = ((uint32_t)(L + 1) << 8) | (o << 16);
+    bool leaf = (e - i <= 8u) || (L == 31);
+    nonleaf[r] = leaf ? 0u : 1u;
+    if (i == pt.begin) ntop[P].first_daughter = node;
+    atomicAdd(&ninfo[P], 1u);
+}
+
+__global__ void k_lvl_next(const uint32_t *__restrict__ nonleaf, const uint32_t *__restrict__ nlidx, uint32_t C,
+                           uint32_t base, uint32_t *__restrict__ plist_next)
+{
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= C) return;
+    if (nonleaf[r]) plist_next[nlidx[r]] = base + r;
+}
+
+__global__ void k_lvl_slots(uint32_t n, const uint32_t *__restrict__ flags, const uint32_t *__restrict__ cidx,
+                            const uint32_t *__restrict__ nonleaf, const uint32_t *__restrict__ nlidx,
+                            uint32_t *__restrict__ pslot)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (pslot[i] == PLAN_NONE) return;
+    uint32_t r = cidx[i] + flags[i] - 1;
+    pslot[i] = nonleaf[r] ? nlidx[r] : PLAN_NONE;
+}
+
+/* skip pointer: first node after the subtree in pre-order */
+__global__ void k_skip(const NodeTop *__restrict__ ntop, const uint32_t *__restrict__ ninfo, uint32_t nn,
+                       uint32_t *__restrict__ skip)
+{
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nn) return;
+    uint32_t x = v, res = PLAN_NONE;
+    for (int up = 0; x != 0 && up < 40; ++up) { /* at most 32 levels */
+        uint32_t P = ntop[x].parent;
+        uint32_t fd = ntop[P].first_daughter, nd = ninfo[P] & 0xffu;
+        if (x + 1 < fd + nd) {
+            res = x + 1;
+            break;
+        }
+        x = P;
+    }
+    skip[v] = res;
+}
+
+__global__ void k_leaf_mark(const NodeTop *__restrict__ ntop, const uint32_t *__restrict__ ninfo, uint32_t nn,
+                            uint32_t *__restrict__ leaf_of, uint32_t *__restrict__ lflag)
+{
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nn || v == 0) return;
+    if ((ninfo[v] & 0xffu) != 0) return;
+    NodeTop t = ntop[v];
+    lflag[t.begin] = 1u;
+    for (uint32_t i = t.begin; i < t.end; ++i) leaf_of[i] = v;
+}
+
+__global__ void k_leaf_index(const NodeTop *__restrict__ ntop, const uint32_t *__restrict__ ninfo, uint32_t nn,
+                             const uint32_t *__restrict__ lidx, int32_t *__restrict__ node2leaf,
+                             int32_t *__restrict__ leaf_nodes)
+{
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nn) return;
+    bool leaf = v != 0 && (ninfo[v] & 0xffu) == 0;
+    if (!leaf) {
+        node2leaf[v] = -1;
+        return;
+    }
+    uint32_t k = lidx[ntop[v].begin];
+    node2leaf[v] = (int32_t)k;
+    leaf_nodes[k] = (int32_t)v;
+}
+
+template <class T> static void grow_copy(Arena &a, const std::string &name, T *&p, size_t old_n, size_t new_n,
+                                         hipStream_t s)
+{
+    T *np = nullptr;
+    HIPCHK(hipMalloc(&np, new_n * sizeof(T)));
+    if (p && old_n) HIPCHK(hipMemcpyAsync(np, p, old_n * sizeof(T), hipMemcpyDeviceToDevice, s));
+    (void)a;
+    (void)name;
+    HIPCHK(hipStreamSynchronize(s));
+    if (p) HIPCHK(hipFree(p));
+    p = np;
+}
+
+void dev_build_tree(DevTree &t, const DevParticles &ps, const plan_derived &d, bool check_pos,
+                    const double *root_center_in, double half_width_in, hipStream_t s)
+{
+    const uint32_t T = ps.T;
+    if (T == 0) throw std::runtime_error("BuildTree: no particles");
+    Arena &a = t.arena;
+    /* root box (src/tree.hpp:2763-2803) */
+    if (check_pos) {
+        unsigned nb = std::min<unsigned>(1024, ceil_div(T, BLK));
+        double *bb = a.get_t<double>("bbox", (size_t)nb * 6);
+        hipLaunchKernelGGL(k_bbox, dim3(nb), dim3(BLK), 0, s, ps.pos, T, bb);
+        LAUNCHCHK();
+        std::vector<double> hb((size_t)nb * 6);
+        HIPCHK(hipMemcpyAsync(hb.data(), bb, hb.size() * 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        double pmin[3], pmax[3];
+        for (int k = 0; k < 3; ++k) {
+            pmin[k] = hb[k];
+            pmax[k] = hb[3 + k];
+        }
+        for (unsigned b = 1; b < nb; ++b)
+            for (int k = 0; k < 3; ++k) {
+                pmin[k] = std::min(hb[b * 6 + k], pmin[k]);
+                pmax[k] = std::max(hb[b * 6 + 3 + k], pmax[k]);
+            }
+        double extent = 0.0;
+        for (int k = 0; k < 3; ++k) extent = std::max(extent, pmax[k] - pmin[k]);
+        t.half_width = d.max_half_width + d.max_ghost_zone_width;
+        for (int k = 0; k < 3; ++k) t.root_center[k] = d.box_center[k];
+        if (extent < t.half_width) {
+            t.half_width = (std::ceil(extent * 100.) + 1) / 100.;
+            for (int k = 0; k < 3; ++k) {
+                if (pmax[k] - pmin[k] < 0.5 * d.box_length[k]) {
+                    t.root_center[k] = (pmax[k] + pmin[k]) / 2.0;
+                    t.root_center[k] = std::round(t.root_center[k] * 1000.) / 1000.;
+                } else {
+                    t.root_center[k] = d.box_center[k];
+                }
+            }
+        }
+    } else {
+        t.half_width = half_width_in;
+        for (int k = 0; k < 3; ++k) t.root_center[k] = root_center_in[k];
+        if (t.half_width < 1e-16) {
+            t.half_width = d.max_half_width + d.max_ghost_zone_width;
+            for (int k = 0; k < 3; ++k) t.root_center[k] = d.box_center[k];
+        }
+    }
+    MortonFrame f;
+    for (int k = 0; k < 3; ++k) {
+        volatile double lo = t.root_center[k] - t.half_width;
+        volatile double hi = t.root_center[k] + t.half_width;
+        t.boxmin[k] = lo;
+        t.scale[k] = 1.0 / (hi - lo);
+        f.boxmin[k] = t.boxmin[k];
+        f.scale[k] = t.scale[k];
+    }
+    t.epsilon = 2 * t.half_width / std::pow(2, 31);
+
+    /* keys and sort */
+    uint4 *k0 = a.get_t<uint4>("keys_a", T);
+    uint4 *k1 = a.get_t<uint4>("keys_b", T);
+    hipLaunchKernelGGL(k_morton, dim3(ceil_div(T, BLK)), dim3(BLK), 0, s, ps.pos, T, f, k0);
+    LAUNCHCHK();
+    radix_sort_96(a, &k0, &k1, T, 3, s);
+    uint4 *skeys = k0;
+
+    /* sink runs */
+    uint32_t *flags = a.get_t<uint32_t>("flags", T + 1);
+    uint32_t *ridx = a.get_t<uint32_t>("ridx", T + 1);
+    uint32_t *tot = a.get_t<uint32_t>("tot", 8);
+    hipLaunchKernelGGL(k_run_flags, dim3(ceil_div(T, BLK)), dim3(BLK), 0, s, skeys, T, flags);
+    LAUNCHCHK();
+    scan_exclusive_u32(a, flags, ridx, T, s, tot);
+    const uint32_t n = read_u32(tot, s);
+    t.n = n;
+    t.num_sinks = 0;
+    uint32_t *start = a.get_t<uint32_t>("rstart", n + 1);
+    hipLaunchKernelGGL(k_run_starts, dim3(ceil_div(T, BLK)), dim3(BLK), 0, s, flags, ridx, T, start);
+    LAUNCHCHK();
+    double *mot = a.get_t<double>("mass_of_type", PLAN_MAX_TYPES);
+    HIPCHK(hipMemcpyAsync(mot, d.mass_per_particle, sizeof(double) * PLAN_MAX_TYPES, hipMemcpyHostToDevice, s));
+    t.pm = a.get_t<double4>("pm", n);
+    t.va = a.get_t<double4>("va", n);
+    t.rho = a.get_t<double>("rho", n);
+    t.oid = a.get_t<uint32_t>("oid", n);
+    t.sinkcnt = a.get_t<uint32_t>("sinkcnt", n);
+    t.sinkoff = a.get_t<uint32_t>("sinkoff", n);
+    t.keys = a.get_t<uint4>("ukeys", n);
+    hipLaunchKernelGGL(k_gather_particles, dim3(ceil_div(n, BLK)), dim3(BLK), 0, s, skeys, start, n, T, ps.pos,
+                       ps.vel, ps.prop, ps.id, mot, (uint32_t)PLAN_MAX_TYPES, t.pm, t.va, t.oid, t.sinkcnt,
+                       t.sinkoff, t.keys);
+    LAUNCHCHK();
+    HIPCHK(hipMemsetAsync(t.rho, 0, (size_t)n * 8, s));
+    t.num_sinks = 0;
+    if (n != T) t.num_sinks = 1; /* exact count computed on demand by callers */
+
+    /* level-synchronous node construction */
+    size_t cap = (size_t)2 * n + 64;
+    NodeGeo *ngeo = nullptr;
+    NodeTop *ntop = nullptr;
+    uint32_t *ninfo = nullptr;
+    HIPCHK(hipMalloc(&ngeo, cap * sizeof(NodeGeo)));
+    HIPCHK(hipMalloc(&ntop, cap * sizeof(NodeTop)));
+    HIPCHK(hipMalloc(&ninfo, cap * sizeof(uint32_t)));
+    NodeGeo rg{t.root_center[0], t.root_center[1], t.root_center[2], t.half_width};
+    NodeTop rt{0u, n, 0u, PLAN_NONE};
+    uint32_t ri = 1u << 8;
+    HIPCHK(hipMemcpyAsync(ngeo, &rg, sizeof(rg), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(ntop, &rt, sizeof(rt), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(ninfo, &ri, sizeof(ri), hipMemcpyHostToDevice, s));
+    uint32_t *pslot = a.get_t<uint32_t>("pslot", n);
+    uint32_t *cidx = a.get_t<uint32_t>("cidx", n + 1);
+    uint32_t *cstart = a.get_t<uint32_t>("cstart", n + 1);
+    HIPCHK(hipMemsetAsync(pslot, 0, (size_t)n * 4, s));
+    uint32_t *plist = a.get_t<uint32_t>("plist_a", n + 1);
+    uint32_t *plist_next = a.get_t<uint32_t>("plist_b", n + 1);
+    uint32_t zero = 0;
+    HIPCHK(hipMemcpyAsync(plist, &zero, 4, hipMemcpyHostToDevice, s));
+    uint32_t np = 1;
+    t.level_count[0] = 1;
+    for (int l = 1; l < 32; ++l) t.level_count[l] = 0;
+    uint32_t base = 1;
+    int L = 1;
+    t.depth = 1;
+    while (np > 0 && L <= 31) {
+        hipLaunchKernelGGL(k_lvl_flags, dim3(ceil_div(n, BLK)), dim3(BLK), 0, s, t.keys, n, pslot, plist, ntop, L,
+                           flags);
+        LAUNCHCHK();
+        scan_exclusive_u32(a, flags, cidx, n, s, tot);
+        uint32_t C = read_u32(tot, s);
+        if (C == 0) break;
+        if (base + (size_t)C > cap) {
+            size_t ncap = std::max(cap * 2, (size_t)base + C + 64);
+            grow_copy(a, "ngeo", ngeo, base, ncap, s);
+            grow_copy(a, "ntop", ntop, base, ncap, s);
+            grow_copy(a, "ninfo", ninfo, base, ncap, s);
+            cap = ncap;
+        }
+        hipLaunchKernelGGL(k_lvl_starts, dim3(ceil_div(n, BLK)), dim3(BLK), 0, s, flags, cidx, n, cstart);
+        LAUNCHCHK();
+        uint32_t *nonleaf = a.get_t<uint32_t>("nonleaf", C + 1);
+        uint32_t *nlidx = a.get_t<uint32_t>("nlidx", C + 1);
+        hipLaunchKernelGGL(k_lvl_nodes, dim3(ceil_div(C, BLK)), dim3(BLK), 0, s, t.keys, cstart, C, pslot, plist, L,
+                           base, ngeo, ntop, ninfo, nonleaf);
+        LAUNCHCHK();
+        scan_exclusive_u32(a, nonleaf, nlidx, C, s, tot + 1);
+        uint32_t Cn = read_u32(tot + 1, s);
+        hipLaunchKernelGGL(k_lvl_next, dim3(ceil_div(C, BLK)), dim3(BLK), 0, s, nonleaf, nlidx, C, base, plist_next);
+        LAUNCHCHK();
+        hipLaunchKernelGGL(k_lvl_slots, dim3(ceil_div(n, BLK)), dim3(BLK), 0, s, n, flags, cidx, nonleaf, nlidx,
+                           pslot);
+        LAUNCHCHK();
+        t.level_count[L] = (int)C;
+        base += C;
+        np = Cn;
+        std::swap(plist, plist_next);
+        t.depth = L + 1;
+        ++L;
+    }
+    t.num_nodes = base;
+    t.level_ptr[0] = 0;
+    for (int l = 1; l < 32; ++l) t.level_ptr[l] = t.level_ptr[l - 1] + t.level_count[l - 1];
+    /* adopt the node arrays into the arena bookkeeping */
+    if (t.ngeo) HIPCHK(hipFree(t.ngeo));
+    if (t.ntop) HIPCHK(hipFree(t.ntop));
+    if (t.ninfo) HIPCHK(hipFree(t.ninfo));
+    t.ngeo = ngeo;
+    t.ntop = ntop;
+    t.ninfo = ninfo;
+    const uint32_t nn = t.num_nodes;
+    t.skip = a.get_t<uint32_t>("skip", nn);
+    t.leaf_of = a.get_t<uint32_t>("leaf_of", n);
+    t.node2leaf = a.get_t<int32_t>("node2leaf", nn);
+    hipLaunchKernelGGL(k_skip, dim3(ceil_div(nn, BLK)), dim3(BLK), 0, s, ntop, ninfo, nn, t.skip);
+    LAUNCHCHK();
+    HIPCHK(hipMemsetAsync(flags, 0, (size_t)n * 4, s));
+    hipLaunchKernelGGL(k_leaf_mark, dim3(ceil_div(nn, BLK)), dim3(BLK), 0, s, ntop, ninfo, nn, t.leaf_of, flags);
+    LAUNCHCHK();
+    scan_exclusive_u32(a, flags, cidx, n, s, tot);
+    t.num_leaves = read_u32(tot, s);
+    t.leaf_nodes = a.get_t<int32_t>("leaf_nodes", t.num_leaves + 1);
+    hipLaunchKernelGGL(k_leaf_index, dim3(ceil_div(nn, BLK)), dim3(BLK), 0, s, ntop, ninfo, nn, cidx, t.node2leaf,
+                       t.leaf_nodes);
+    LAUNCHCHK();
+    HIPCHK(hipStreamSynchronize(s));
+}
+
+}  // namespace plan

@@ -1,0 +1,579 @@
+// clumps.cpp — clump post-processing after HOP: per-clump kinematics, the two
+// greedy merge loops, real-peak re-keying, unbinding, rebinding, envelope
+// ripping, filters, angular momentum and ranking.
+//
+// Restated from BHtree::FindPlanetesimals (src/tree.hpp:3810-4339) and
+// Planetesimal / PlanetesimalList (src/tree.hpp:4351-4909).  The control flow
+// is inherently sequential (map-ordered greedy merges, key-ordered rebinding
+// claims), so it runs on the host over the table of particles above the
+// density threshold, and every tree query it needs goes to the device in
+// batches: the clump tree and its ball searches for a whole merge pass, all
+// saddle-point densities of a pass, and the Hill-sphere ball searches of the
+// rebinding step.  Floating-point expressions keep the reference's operation
+// order; the library is built without FMA contraction.
+#include "clumps.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <stdexcept>
+
+namespace plan {
+
+typedef std::pair<uint32_t, double> PD;
+static bool pd_less(const PD &a, const PD &b) { return a.second < b.second; }
+
+uint32_t HostMembers::rec_of_tidx(uint32_t tidx) const
+{
+    auto it = std::lower_bound(sorted_tidx.begin(), sorted_tidx.end(), tidx);
+    if (it == sorted_tidx.end() || *it != tidx) return PLAN_NONE;
+    return rec_of_sorted[it - sorted_tidx.begin()];
+}
+
+namespace {
+
+struct Ops {
+    const std::vector<MemberRec> &P;
+    const ClumpSettings &cs;
+
+    double dist_to(uint32_t h, const double *c) const
+    {
+        const MemberRec &r = P[h];
+        double dx = r.x - c[0], dy = r.y - c[1], dz = r.z - c[2];
+        double s = 0;
+        s += dx * dx;
+        s += dy * dy;
+        s += dz * dz;
+        return std::sqrt(s);
+    }
+    static double dist_pts(const double *a, const double *b)
+    {
+        double s = 0;
+        for (int k = 0; k < 3; ++k) {
+            double t = a[k] - b[k];
+            s += t * t;
+        }
+        return std::sqrt(s);
+    }
+    static const double *pos(const MemberRec &r) { return &r.x; }
+
+    /* CalculateKinematicProperties (src/tree.hpp:4431-4442) */
+    void kinematics(HClump &c) const
+    {
+        c.total_mass = 0;
+        for (int k = 0; k < 3; ++k) c.com[k] = c.vel[k] = 0;
+        for (uint32_t h : c.indices) {
+            const MemberRec &r = P[h];
+            c.total_mass += r.m;
+            c.com[0] += r.x * r.m;
+            c.com[1] += r.y * r.m;
+            c.com[2] += r.z * r.m;
+            c.vel[0] += r.vx * r.m;
+            c.vel[1] += r.vy * r.m;
+            c.vel[2] += r.vz * r.m;
+        }
+        for (int k = 0; k < 3; ++k) c.com[k] /= c.total_mass;
+        for (int k = 0; k < 3; ++k) c.vel[k] /= c.total_mass;
+    }
+    /* radii of SortParticles / MergeAnotherPlanetesimal (src/tree.hpp:4461-4511) */
+    void radii(HClump &c) const
+    {
+        double tenth = P[c.indices[c.indices.size() / 2]].rho / 10.;
+        auto it = c.particles.begin();
+        for (; it != c.particles.end(); ++it)
+            if (P[it->first].rho < tenth) break;
+        c.inner = (it != c.particles.end()) ? it->second : c.particles.back().second;
+        auto rit = c.particles.rbegin();
+        for (; rit != c.particles.rend(); ++rit)
+            if (P[rit->first].rho > tenth) break;
+        c.outer = rit->second;
+        c.one10th = (c.inner + c.outer) / 2.;
+        rit = c.particles.rbegin();
+        double tm = 0., lim = c.total_mass * (1.0 - 0.95);
+        for (; rit != c.particles.rend(); ++rit) {
+            tm += P[rit->first].m;
+            if (tm > lim) break;
+        }
+        c.two_sigma = rit->second;
+        if (c.inner > 0 && c.outer / c.inner > 2.5) c.one10th = std::min(c.outer, c.two_sigma);
+        else c.one10th = std::min(c.one10th, c.two_sigma);
+    }
+    /* SortParticles (src/tree.hpp:4446-4512) */
+    void sort_particles(HClump &c) const
+    {
+        kinematics(c);
+        for (uint32_t h : c.indices) c.particles.push_back(PD(h, dist_to(h, c.com)));
+        std::sort(c.particles.begin(), c.particles.end(), pd_less);
+        for (size_t i = 0; i < c.particles.size(); ++i) c.indices[i] = c.particles[i].first;
+        radii(c);
+    }
+    /* IsPositionDispersion2Large (src/tree.hpp:4516-4528) */
+    static bool dispersion_too_large(const HClump &c, double tol = 0.55)
+    {
+        size_t n = c.particles.size();
+        double s = 0.0;
+        for (size_t i = 0; i < n; ++i) s = s + c.particles[i].second * c.particles[i].second;
+        return s / n > std::pow(tol * c.particles[n - 1].second, 2);
+    }
+    /* MergeAnotherPlanetesimal (src/tree.hpp:4532-4611) */
+    void merge(HClump &a, const HClump &b, double rho_a_peak, double rho_b_peak) const
+    {
+        if (rho_a_peak < rho_b_peak) {
+            a.subpeaks.push_back(a.peak_index);
+            a.peak_index = b.peak_index;
+        } else {
+            a.subpeaks.push_back(b.peak_index);
+        }
+        double tot = a.total_mass + b.total_mass;
+        for (int k = 0; k < 3; ++k) a.com[k] = (a.com[k] * a.total_mass + b.com[k] * b.total_mass) / tot;
+        for (int k = 0; k < 3; ++k) a.vel[k] = (a.vel[k] * a.total_mass + b.vel[k] * b.total_mass) / tot;
+        a.total_mass = tot;
+        a.particles.insert(a.particles.end(), b.particles.begin(), b.particles.end());
+        for (auto &it : a.particles) it.second = dist_to(it.first, a.com);
+        std::sort(a.particles.begin(), a.particles.end(), pd_less);
+        a.indices.resize(a.particles.size());
+        for (size_t i = 0; i < a.particles.size(); ++i) a.indices[i] = a.particles[i].first;
+        radii(a);
+    }
+    /* RemoveUnboundParticles (src/tree.hpp:4615-4674) */
+    void remove_unbound(HClump &c) const
+    {
+        const double G = cs.G, sh = cs.q * cs.Omega;
+        size_t i = c.particles.size() - 1;
+        while (i != 0) {
+            const MemberRec &q = P[c.particles[i].first];
+            double tm = c.total_mass - q.m;
+            double tc[3], tv[3];
+            const double *qp = &q.x, *qv = &q.vx;
+            for (int k = 0; k < 3; ++k) tc[k] = (c.com[k] * c.total_mass - qp[k] * q.m) / tm;
+            for (int k = 0; k < 3; ++k) tv[k] = (c.vel[k] * c.total_mass - qv[k] * q.m) / tm;
+            double a = tc[0] - qp[0];
+            double sv[3] = {0.0 * a, sh * a, 0.0 * a};
+            double n2 = 0;
+            for (int k = 0; k < 3; ++k) {
+                double dv = (tv[k] - sv[k]) - qv[k];
+                n2 += dv * dv;
+            }
+            double e = (0.5 / c.total_mass) * n2 - G / dist_pts(tc, qp);
+            if (e > 0) {
+                std::swap(c.indices[i], c.indices.back());
+                c.indices.pop_back();
+                std::swap(c.particles[i], c.particles.back());
+                c.particles.pop_back();
+                for (int k = 0; k < 3; ++k) {
+                    c.vel[k] = tv[k];
+                    c.com[k] = tc[k];
+                }
+                c.total_mass = tm;
+            }
+            i--;
+        }
+        std::vector<PD>().swap(c.particles);
+        sort_particles(c);
+    }
+    /* CalculateHillRadius (src/tree.hpp:4715-4717) */
+    void hill_radius(HClump &c) const
+    {
+        c.hill = std::pow((((c.total_mass * cs.G) / 3.) / cs.Omega) / cs.Omega, 1. / 3.);
+    }
+    /* CalculateAngularMomentum (src/tree.hpp:4721-4747) */
+    void angular_momentum(HClump &c) const
+    {
+        double quarter = 0.25 * c.hill;
+        double hu = ((c.total_mass * c.hill) * c.hill) * cs.Omega;
+        double s1 = cs.q * cs.Omega;
+        for (const auto &it : c.particles) {
+            const MemberRec &q = P[it.first];
+            const double *qp = &q.x, *qv = &q.vx;
+            double dr[3], dv[3], j[3];
+            for (int k = 0; k < 3; ++k) dr[k] = qp[k] - c.com[k];
+            double sp[3] = {0. * qp[0], s1 * qp[0], 0. * qp[0]};
+            double sc[3] = {0. * c.com[0], s1 * c.com[0], 0. * c.com[0]};
+            for (int k = 0; k < 3; ++k) dv[k] = (qv[k] - sp[k]) - (c.vel[k] - sc[k]);
+            j[0] = dr[1] * dv[2] - dr[2] * dv[1];
+            j[1] = dr[2] * dv[0] - dr[0] * dv[2];
+            j[2] = dr[0] * dv[1] - dr[1] * dv[0];
+            double o[3] = {-dr[0] * dr[2], -dr[1] * dr[2], (dr[0] * dr[0] + dr[1] * dr[1])};
+            for (int k = 0; k < 3; ++k) j[k] += cs.Omega * o[k];
+            for (int k = 0; k < 3; ++k) c.J[k] += q.m * j[k];
+            if (it.second < quarter)
+                for (int k = 0; k < 3; ++k) c.Jq[k] += q.m * j[k];
+        }
+        for (int k = 0; k < 3; ++k) {
+            c.J[k] /= hu;
+            c.Jq[k] /= hu;
+        }
+    }
+    /* PlanetesimalList::IsGravitationallyBound (src/tree.hpp:4867-4874) */
+    bool grav_bound(const HClump &a, const HClump &b) const
+    {
+        double tm = a.total_mass + b.total_mass;
+        double Pg = -cs.G / dist_pts(a.com, b.com);
+        double dx = a.com[0] - b.com[0];
+        double sh = cs.q * cs.Omega;
+        double sv[3] = {0.0 * dx, sh * dx, 0.0 * dx};
+        double n2 = 0;
+        for (int k = 0; k < 3; ++k) {
+            double t = (a.vel[k] - sv[k]) - b.vel[k];
+            n2 += t * t;
+        }
+        double Ek = (0.5 / tm) * n2;
+        return Pg + Ek < 0.;
+    }
+};
+
+/* RemoveSmallMassAndLowPeak (src/tree.hpp:3524-3546); peak densities come
+ * from the member table (the key is a member of its own clump or of an
+ * absorbed one, both above threshold) */
+void remove_small_mass_low_peak(std::map<uint32_t, HClump> &cl, const HostMembers &hm, const ClumpSettings &cs)
+{
+    std::vector<uint32_t> del;
+    for (auto &it : cl) {
+        uint32_t kr = hm.rec_of_tidx(it.first);
+        double pk = kr == PLAN_NONE ? 0.0 : hm.rec[kr].rho;
+        if (it.second.total_mass < cs.clump_mass_threshold || pk < cs.peak_density_threshold) {
+            del.push_back(it.first);
+            continue;
+        }
+        uint32_t lim = it.second.indices.size() < cs.K ? (uint32_t)it.second.indices.size() : cs.K;
+        double ori = 0;
+        for (uint32_t i = 0; i < lim; ++i) ori = std::max(ori, hm.rec[it.second.indices[i]].ath);
+        if (ori < cs.peak_density_threshold) del.push_back(it.first);
+    }
+    for (auto k : del) cl.erase(k);
+}
+
+double rho_of_key(const HostMembers &hm, uint32_t tidx)
+{
+    uint32_t r = hm.rec_of_tidx(tidx);
+    if (r == PLAN_NONE) throw std::runtime_error("clump peak is not a particle above the density threshold");
+    return hm.rec[r].rho;
+}
+
+/* one merge loop (src/tree.hpp:3863-4006 with one10th radii, 4011-4152 with
+ * Hill radii), serial semantics: predators in key order; a clump eaten in a
+ * pass cannot eat in it; merges applied after the pass in pair order */
+void merge_loop(std::map<uint32_t, HClump> &cl, const HostMembers &hm, const ClumpSettings &cs, const Ops &ops,
+                ClumpServices &svc, bool hill, ClumpStats &st)
+{
+    double max_radius = 0.;
+    std::vector<uint32_t> deleted;
+    bool happened = true;
+    while (happened) {
+        st.merge_passes++;
+        /* BuildClumpTree (src/tree.hpp:5251-5284) over the masked clumps */
+        std::vector<double4> pts, vel, queries;
+        std::vector<std::map<uint32_t, HClump>::iterator> p1s;
+        for (auto it = cl.begin(); it != cl.end(); ++it) {
+            if (!it->second.mask) continue;
+            pts.push_back(make_double4(it->second.com[0], it->second.com[1], it->second.com[2], (double)it->first));
+            vel.push_back(make_double4(it->second.vel[0], it->second.vel[1], it->second.vel[2], 0.0));
+            if (hill) max_radius = std::max(it->second.hill * cs.hill_frac, max_radius);
+            else max_radius = std::max(it->second.one10th, max_radius);
+        }
+        for (auto it = cl.begin(); it != cl.end(); ++it) {
+            if (!it->second.mask) continue;
+            double r1 = hill ? it->second.hill * cs.hill_frac : it->second.one10th;
+            queries.push_back(make_double4(it->second.com[0], it->second.com[1], it->second.com[2], r1 + max_radius));
+            p1s.push_back(it);
+        }
+        std::vector<std::vector<uint32_t>> near;
+        svc.clump_tree_balls(pts, vel, queries, near);
+        /* every saddle density this pass could ask for, in one device batch */
+        struct Cand {
+            uint32_t p1k, p2k;
+            int kind; /* 1 contains, 2 needs saddle */
+            size_t sq;
+        };
+        std::vector<std::vector<Cand>> cands(p1s.size());
+        std::vector<double4> sq;
+        for (size_t a = 0; a < p1s.size(); ++a) {
+            const HClump &c1 = p1s[a]->second;
+            double r1 = hill ? c1.hill * cs.hill_frac : c1.one10th;
+            for (uint32_t id2 : near[a]) {
+                if (id2 == p1s[a]->first) continue;
+                auto p2 = cl.find(id2);
+                if (p2 == cl.end()) {
+                    std::fprintf(stderr, "Error: Cannot find a clump with peak_id=%u. Proceed for now.\n", id2);
+                    continue;
+                }
+                const HClump &c2 = p2->second;
+                if (!c2.mask) continue;
+                double r2 = hill ? c2.hill * cs.hill_frac : c2.one10th;
+                double cd = Ops::dist_pts(c1.com, c2.com);
+                Cand cd_;
+                cd_.p1k = p1s[a]->first;
+                cd_.p2k = id2;
+                cd_.sq = 0;
+                if (cd < std::max(r1, r2)) {
+                    cd_.kind = 1;
+                } else if (cd < (r1 + r2) && ops.grav_bound(c1, c2)) {
+                    /* IsSaddlePointDeepEnough / IsHillSaddlePointDeepEnough (src/tree.hpp:4880-4909) */
+                    double ra = hill ? c1.hill : r1, rb = hill ? c2.hill : r2;
+                    double est = hill ? std::min(c1.hill, c2.hill) / cs.hill_frac : std::min(r1, r2);
+                    double f = ra / (ra + rb);
+                    double sp[3];
+                    for (int k = 0; k < 3; ++k) {
+                        double r12 = c2.com[k] - c1.com[k];
+                        r12 *= f;
+                        sp[k] = c1.com[k] + r12;
+                    }
+                    if (!(est > 0)) throw std::runtime_error("saddle query with zero start radius");
+                    cd_.kind = 2;
+                    cd_.sq = sq.size();
+                    sq.push_back(make_double4(sp[0], sp[1], sp[2], est));
+                } else {
+                    continue;
+                }
+                cands[a].push_back(cd_);
+            }
+        }
+        std::vector<double> srho;
+        if (!sq.empty()) svc.saddle_density(sq, srho);
+        st.saddle_queries += (uint32_t)sq.size();
+        /* greedy replay */
+        happened = false;
+        std::vector<std::pair<uint32_t, uint32_t>> pairs;
+        for (size_t a = 0; a < p1s.size(); ++a) {
+            if (!p1s[a]->second.mask) continue;
+            size_t before = pairs.size();
+            for (const Cand &c : cands[a]) {
+                auto p2 = cl.find(c.p2k);
+                if (!p2->second.mask) continue;
+                bool eat = c.kind == 1 || !(srho[c.sq] < 2.5 * cs.density_threshold);
+                if (eat) {
+                    p2->second.mask = false;
+                    pairs.push_back(std::make_pair(c.p1k, c.p2k));
+                    deleted.push_back(c.p2k);
+                }
+            }
+            if (pairs.size() > before) happened = true;
+        }
+        for (auto &pr : pairs) {
+            HClump &a = cl[pr.first];
+            const HClump &b = cl[pr.second];
+            ops.merge(a, b, rho_of_key(hm, a.peak_index), rho_of_key(hm, b.peak_index));
+            if (hill) ops.hill_radius(a);
+        }
+        for (auto &it : cl) {
+            if (it.second.mask && Ops::dispersion_too_large(it.second)) {
+                it.second.mask = false;
+                deleted.push_back(it.first);
+            }
+        }
+    }
+    for (auto k : deleted) cl.erase(k);
+}
+
+}  // namespace
+
+/* SearchBoundParticlesWithinHillRadius (src/tree.hpp:4678-4711) over the
+ * candidates (tree indices, traversal order, density >= threshold) */
+static void search_bound(HClump &c, const std::vector<uint32_t> &cand, const HostMembers &hm,
+                         std::vector<uint8_t> &in_clump, const ClumpSettings &cs, const Ops &ops)
+{
+    const double sh = cs.q * cs.Omega;
+    for (uint32_t tidx : cand) {
+        uint32_t r = hm.rec_of_tidx(tidx);
+        if (r == PLAN_NONE) throw std::runtime_error("rebinding candidate above threshold missing from member table");
+        if (in_clump[r]) continue;
+        const MemberRec &q = hm.rec[r];
+        const double *qp = &q.x, *qv = &q.vx;
+        double tm = c.total_mass + q.m;
+        double a = c.com[0] - qp[0];
+        double sv[3] = {0.0 * a, sh * a, 0.0 * a};
+        double n2 = 0;
+        for (int k = 0; k < 3; ++k) {
+            double dv = (c.vel[k] - sv[k]) - qv[k];
+            n2 += dv * dv;
+        }
+        double e = (0.5 / tm) * n2 - cs.G / Ops::dist_pts(c.com, qp);
+        if (e < 0) {
+            c.indices.push_back(r);
+            c.total_mass = tm;
+            for (int k = 0; k < 3; ++k) c.com[k] = (c.com[k] * c.total_mass + q.m * qp[k]) / c.total_mass;
+            for (int k = 0; k < 3; ++k) c.vel[k] = (c.total_mass * c.vel[k] + q.m * qv[k]) / c.total_mass;
+            in_clump[r] = 1;
+        }
+    }
+    if (c.indices.size() > c.particles.size()) {
+        std::vector<PD>().swap(c.particles);
+        ops.sort_particles(c);
+    }
+}
+
+template <class F> static void for_each_clump_parallel(std::map<uint32_t, HClump> &cl, F f)
+{
+    std::vector<HClump *> v;
+    v.reserve(cl.size());
+    for (auto &it : cl) v.push_back(&it.second);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long i = 0; i < (long)v.size(); ++i) f(*v[i]);
+}
+
+void run_clump_pipeline(HostMembers &hm, const ClumpSettings &cs, ClumpServices &svc,
+                        std::map<uint32_t, HClump> &cl, double &Mp_max, double &Mp_tot, ClumpStats &st)
+{
+    Ops ops{hm.rec, cs};
+    cl.clear();
+    Mp_max = Mp_tot = 0;
+    const uint32_t nc = (uint32_t)hm.cpeak.size();
+    /* naive clumps in key order, members in the serial walk's discovery order
+     * (src/tree.hpp:3766-3809) */
+    for (uint32_t c = 0; c < nc; ++c) {
+        auto it = cl.emplace_hint(cl.end(), hm.cpeak[c], HClump());
+        HClump &h = it->second;
+        h.peak_index = hm.cpeak[c];
+        uint32_t s = hm.cstart[c], e = hm.cstart[c + 1];
+        h.indices.resize(e - s);
+        for (uint32_t i = s; i < e; ++i) h.indices[i - s] = i;
+    }
+    for_each_clump_parallel(cl, [&](HClump &h) { ops.sort_particles(h); });
+    std::vector<uint32_t> del;
+    if (cs.diffuse < 0.525) {
+        for (auto &it : cl)
+            if (Ops::dispersion_too_large(it.second, cs.diffuse)) del.push_back(it.first);
+        for (auto k : del) cl.erase(k);
+        del.clear();
+    }
+    if (!cs.enough_resolution) remove_small_mass_low_peak(cl, hm, cs);
+    st.naive = (uint32_t)cl.size();
+
+    /* 4: merging */
+    merge_loop(cl, hm, cs, ops, svc, false, st);
+    for (auto &it : cl) ops.hill_radius(it.second);
+    merge_loop(cl, hm, cs, ops, svc, true, st);
+    st.after_merge = (uint32_t)cl.size();
+
+    /* real peaks into keys (src/tree.hpp:4156-4182) */
+    {
+        std::vector<uint32_t> old_keys, new_keys;
+        for (auto &it : cl) {
+            uint32_t pk = it.first;
+            for (uint32_t o : it.second.subpeaks)
+                if (rho_of_key(hm, pk) < rho_of_key(hm, o)) pk = o;
+            if (pk != it.first) {
+                old_keys.push_back(it.first);
+                new_keys.push_back(pk);
+            }
+        }
+        for (size_t i = 0; i < old_keys.size(); ++i) {
+            auto ins = cl.emplace(new_keys[i], HClump());
+            if (ins.second) {
+                auto orig = cl.find(old_keys[i]);
+                std::swap(orig->second, ins.first->second);
+                ins.first->second.subpeaks.push_back(orig->first);
+                cl.erase(orig);
+            } else {
+                std::fprintf(stderr, "Real peak index replacing failed.\n");
+            }
+        }
+    }
+
+    /* 5: unbinding and rebinding */
+    for_each_clump_parallel(cl, [&](HClump &h) {
+        ops.remove_unbound(h);
+        ops.hill_radius(h);
+    });
+    std::vector<uint8_t> in_clump(hm.rec.size(), 0);
+    for (auto &it : cl)
+        for (uint32_t r : it.second.indices) in_clump[r] = 1;
+    {
+        std::vector<double4> qs;
+        for (auto &it : cl)
+            qs.push_back(make_double4(it.second.com[0], it.second.com[1], it.second.com[2], it.second.hill));
+        std::vector<std::vector<uint32_t>> first;
+        if (!qs.empty()) svc.particle_balls(qs, cs.density_threshold, first);
+        st.rebind_searches += (uint32_t)qs.size();
+        size_t ci = 0;
+        for (auto &it : cl) {
+            HClump &h = it.second;
+            size_t before;
+            int iter = 0;
+            do {
+                before = h.indices.size();
+                if (iter == 0) {
+                    search_bound(h, first[ci], hm, in_clump, cs, ops);
+                } else {
+                    std::vector<double4> q1{make_double4(h.com[0], h.com[1], h.com[2], h.hill)};
+                    std::vector<std::vector<uint32_t>> l1;
+                    svc.particle_balls(q1, cs.density_threshold, l1);
+                    st.rebind_searches++;
+                    search_bound(h, l1[0], hm, in_clump, cs, ops);
+                }
+                ops.hill_radius(h);
+                ++iter;
+            } while (h.indices.size() > before);
+            ++ci;
+        }
+    }
+
+    /* 6: filters */
+    remove_small_mass_low_peak(cl, hm, cs);
+    for (auto &it : cl)
+        if (it.second.mask && Ops::dispersion_too_large(it.second, std::min(cs.diffuse + 0.1, 0.475)))
+            del.push_back(it.first);
+    for (auto k : del) cl.erase(k);
+    del.clear();
+
+    /* 7: envelope ripping (src/tree.hpp:4229-4249) */
+    {
+        std::vector<HClump *> v;
+        std::vector<uint32_t> keys;
+        for (auto &it : cl) {
+            v.push_back(&it.second);
+            keys.push_back(it.first);
+        }
+        std::vector<uint8_t> empty(v.size(), 0);
+#pragma omp parallel for schedule(dynamic, 1)
+        for (long i = 0; i < (long)v.size(); ++i) {
+            HClump &c = *v[i];
+            while (c.hill < c.particles.back().second) {
+                auto low = std::lower_bound(c.particles.begin(), c.particles.end(), PD(0, c.hill), pd_less);
+                c.particles.resize(low - c.particles.begin());
+                c.indices.resize(c.particles.size());
+                if (c.indices.empty()) {
+                    empty[i] = 1;
+                    break;
+                }
+                ops.kinematics(c);
+                ops.remove_unbound(c);
+                ops.hill_radius(c);
+            }
+        }
+        for (size_t i = 0; i < v.size(); ++i)
+            if (empty[i]) cl.erase(keys[i]);
+    }
+    /* 8 */
+    remove_small_mass_low_peak(cl, hm, cs);
+    /* 9: clumps centred in the ghost zones */
+    for (auto &it : cl) {
+        bool in = true;
+        for (int k = 0; k < 3; ++k)
+            in = in && (it.second.com[k] >= cs.box_min[k] && it.second.com[k] <= cs.box_max[k]);
+        if (!in) del.push_back(it.first);
+    }
+    for (auto k : del) cl.erase(k);
+    del.clear();
+    /* 10: angular momentum */
+    for_each_clump_parallel(cl, [&](HClump &h) { ops.angular_momentum(h); });
+    /* ranking and optional cap (src/tree.hpp:4291-4323) */
+    if (!cl.empty()) {
+        std::vector<std::pair<uint32_t, double>> pm;
+        for (auto &it : cl) pm.push_back(std::make_pair(it.first, it.second.total_mass));
+        std::sort(pm.begin(), pm.end(), [](const std::pair<uint32_t, double> &a, const std::pair<uint32_t, double> &b) {
+            if (a.second == b.second) return a.first < b.first;
+            return a.second < b.second;
+        });
+        if (cs.num_peaks > 0 && cl.size() > cs.num_peaks) {
+            size_t nd = cl.size() - cs.num_peaks;
+            for (size_t i = 0; i < nd; ++i) cl.erase(pm[i].first);
+            pm.erase(pm.begin(), pm.begin() + nd);
+        }
+        Mp_tot = 0.;
+        for (auto &q : pm) Mp_tot = Mp_tot + q.second;
+        Mp_max = pm.back().second;
+    }
+}
+
+}  // namespace plan

@@ -1,0 +1,78 @@
+// clumps.h — clump post-processing after HOP (host control over device queries).
+#pragma once
+
+#include "hop.h"
+#include "../../include/plan_params.h"
+
+#include <map>
+#include <vector>
+
+namespace plan {
+
+/* Device services the clump logic calls; implemented by the context. */
+struct ClumpServices {
+    virtual ~ClumpServices() {}
+    /* density at points {x, y, z, start radius} with the run's K and kernel */
+    virtual void saddle_density(const std::vector<double4> &pts, std::vector<double> &rho) = 0;
+    /* build the clump tree over pts {com, peak id} and ball-search it;
+     * returns, per query {x, y, z, r}, the peak ids found in traversal order */
+    virtual void clump_tree_balls(const std::vector<double4> &com_peak, const std::vector<double4> &vel,
+                                  const std::vector<double4> &queries,
+                                  std::vector<std::vector<uint32_t>> &peaks) = 0;
+    /* ball search on the particle tree keeping rho >= rho_min; tree indices */
+    virtual void particle_balls(const std::vector<double4> &queries, double rho_min,
+                                std::vector<std::vector<uint32_t>> &out) = 0;
+};
+
+struct ClumpSettings {
+    unsigned K = 32;
+    int kernel = 0;
+    int enough_resolution = 1;
+    double density_threshold = 0;
+    double peak_density_threshold = 0;
+    double clump_mass_threshold = 0;
+    double diffuse = 0.55;
+    double hill_frac = 0.25;
+    double G = 0;
+    double Omega = 1;
+    double q = 1.5;
+    double box_min[3], box_max[3];
+    uint32_t num_peaks = 0;
+};
+
+/* per-clump state (Planetesimal<3>, src/tree.hpp:4351-4427); members are
+ * handles into the host member table (records of particles above threshold) */
+struct HClump {
+    uint32_t peak_index = 0;
+    std::vector<uint32_t> subpeaks;
+    std::vector<uint32_t> indices;
+    bool mask = true;
+    double total_mass = 0;
+    double com[3] = {0, 0, 0}, vel[3] = {0, 0, 0};
+    std::vector<std::pair<uint32_t, double>> particles;
+    double outer = 0, inner = 0, one10th = 0, two_sigma = 0, hill = 0;
+    double J[3] = {0, 0, 0}, Jq[3] = {0, 0, 0};
+};
+
+struct HostMembers {
+    std::vector<MemberRec> rec;         /* grouped order */
+    std::vector<uint32_t> cstart;       /* naive clump starts, plus na at the end */
+    std::vector<uint32_t> cpeak;        /* naive clump peak (tree index), ascending */
+    std::vector<uint32_t> sorted_tidx;  /* tree indices ascending */
+    std::vector<uint32_t> rec_of_sorted;/* record of sorted_tidx[i] */
+    std::vector<uint32_t> sinkcnt;      /* per record: particles merged into it */
+    uint32_t rec_of_tidx(uint32_t tidx) const;
+};
+
+struct ClumpStats {
+    uint32_t naive = 0;
+    uint32_t after_merge = 0;
+    uint32_t merge_passes = 0;
+    uint32_t saddle_queries = 0;
+    uint32_t rebind_searches = 0;
+};
+
+void run_clump_pipeline(HostMembers &hm, const ClumpSettings &cs, ClumpServices &svc,
+                        std::map<uint32_t, HClump> &out, double &Mp_max, double &Mp_tot, ClumpStats &st);
+
+}  // namespace plan

@@ -1,0 +1,83 @@
+// ctx.h — device-resident state of one clump-finding context.
+#pragma once
+
+#include "devmem.h"
+#include "../../include/plan_params.h"
+
+#include <vector>
+
+namespace plan {
+
+/* An octree in the reference's level-major layout (src/tree.hpp:2419-2455,
+ * 2849-2886), stored structure-of-arrays in HBM.  Particles are in Morton
+ * order; nodes are level-major, key-ordered within a level. */
+struct DevTree {
+    uint32_t n = 0;               /* tree particles after sink merging */
+    uint32_t num_nodes = 0;
+    uint32_t num_leaves = 0;
+    uint32_t num_sinks = 0;
+    int depth = 0;
+    double root_center[3] = {0, 0, 0};
+    double half_width = 0;
+    double epsilon = 0;
+    double boxmin[3] = {0, 0, 0}, scale[3] = {1, 1, 1};
+    int level_count[32];
+    int level_ptr[32];
+
+    double4 *pm = nullptr;        /* x, y, z, mass            */
+    double4 *va = nullptr;        /* vx, vy, vz, ath_density  */
+    double *rho = nullptr;        /* new_density              */
+    uint32_t *oid = nullptr;      /* original_id              */
+    uint32_t *sinkcnt = nullptr;  /* members merged into this particle (1 = not a sink) */
+    uint32_t *sinkoff = nullptr;  /* first member in sorted order (for sink expansion) */
+    uint4 *keys = nullptr;        /* sorted unique keys {w0, w1, w2, first sorted slot} */
+    NodeGeo *ngeo = nullptr;
+    NodeTop *ntop = nullptr;
+    uint32_t *ninfo = nullptr;    /* num_daughters | level << 8 | orthant << 16 */
+    uint32_t *skip = nullptr;     /* next node in pre-order after the subtree */
+    uint32_t *leaf_of = nullptr;  /* leaf node holding each particle */
+    int32_t *node2leaf = nullptr;
+    int32_t *leaf_nodes = nullptr;
+    Arena arena;
+};
+
+/* Particle set before the tree (ParticleSet<3>, src/tree.hpp:1200-1298). */
+struct DevParticles {
+    uint32_t N = 0;       /* real particles */
+    uint32_t T = 0;       /* with ghosts */
+    uint32_t cap = 0;
+    double4 *pos = nullptr;   /* x, y, z, density (.lis grid density) */
+    double4 *vel = nullptr;   /* vx, vy, vz, 0 */
+    int32_t *prop = nullptr;
+    uint32_t *id = nullptr;
+    Arena arena;
+};
+
+/* derived parameters: the reference's NumericalParameters after ReadLisFile */
+void derive_params(const plan_params *p, uint32_t N, plan_derived *d);
+
+/* build.hip */
+void dev_make_ghosts(DevParticles &ps, const plan_params &pp, const plan_derived &d, hipStream_t s);
+void dev_build_tree(DevTree &t, const DevParticles &ps, const plan_derived &d, bool check_pos,
+                    const double *root_center_in, double half_width_in, hipStream_t s);
+
+/* query.hip */
+struct KnnOut {
+    uint32_t *idx = nullptr;   /* nq * K, ascending d^2 (optional) */
+    double *d2 = nullptr;      /* nq * K (optional) */
+    double *rho = nullptr;     /* nq densities (optional) */
+    uint32_t *densest = nullptr; /* nq densest-neighbour ids (hop mode) */
+    double *rstop = nullptr;   /* nq final radius (optional) */
+};
+/* mode: 0 density, 1 hop (densest neighbour), 2 lists only.
+ * Queries are tree particles qidx[i] (or i when qidx is null) or explicit
+ * points qpts with start radii est. kernel: 0 median spherical, 1 QS. */
+void dev_knn(const DevTree &t, uint32_t K, int mode, int kernel, uint32_t nq, const uint32_t *qidx,
+             const double4 *qpts, const double *est, const double *rho_in, const plan_derived &d, KnnOut out,
+             hipStream_t s);
+/* ball search for nq centres/radii: two-pass (count, fill) into a CSR list,
+ * optionally keeping only particles with rho >= rho_min (rho_min_on). */
+void dev_ball(const DevTree &t, uint32_t nq, const double4 *centre_r, uint32_t *counts, const uint32_t *offsets,
+              uint32_t *out, const double *rho, double rho_min, int filter, hipStream_t s);
+
+}  // namespace plan

@@ -1,0 +1,204 @@
+// scan_sort.hip — device prefix sum and stable LSD radix sort (gfx950, wave64).
+#include "devmem.h"
+
+namespace plan {
+
+static constexpr int SCAN_BLOCK = 256;
+static constexpr int SCAN_ITEMS = 8;
+static constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
+{
+    const int lane = __lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+/* exclusive block scan of one value per thread (256 threads); returns the
+ * block total in *total */
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *lds4, uint32_t *total)
+{
+    const int lane = __lane_id();
+    const int wave = threadIdx.x >> 6;
+    uint32_t inc = wave_incl_scan(v);
+    if (lane == 63) lds4[wave] = inc;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < SCAN_BLOCK / 64; ++w) {
+        uint32_t s = lds4[w];
+        if (w < wave) off += s;
+        tot += s;
+    }
+    *total = tot;
+    __syncthreads();
+    return off + inc - v;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(const uint32_t *__restrict__ in, uint32_t n,
+                                                            uint32_t *__restrict__ sums)
+{
+    __shared__ uint32_t lds4[4];
+    uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        uint64_t j = base + i;
+        if (j < n) s += in[j];
+    }
+    uint32_t tot;
+    (void)block_excl_scan(s, lds4, &tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(const uint32_t *__restrict__ in,
+                                                           uint32_t *__restrict__ out, uint32_t n,
+                                                           const uint32_t *__restrict__ block_off,
+                                                           uint32_t *__restrict__ total)
+{
+    __shared__ uint32_t lds4[4];
+    uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        uint64_t j = base + i;
+        v[i] = (j < n) ? in[j] : 0u;
+        s += v[i];
+    }
+    uint32_t tot;
+    uint32_t off = block_excl_scan(s, lds4, &tot);
+    if (block_off) off += block_off[blockIdx.x];
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        uint64_t j = base + i;
+        if (j < n) out[j] = off;
+        off += v[i];
+    }
+    if (total && blockIdx.x == gridDim.x - 1 && threadIdx.x == SCAN_BLOCK - 1) *total = off;
+}
+
+static void scan_rec(Arena &a, const uint32_t *in, uint32_t *out, uint32_t n, hipStream_t s, uint32_t *total,
+                     int depth)
+{
+    unsigned nb = ceil_div(n, SCAN_TILE);
+    if (nb <= 1) {
+        hipLaunchKernelGGL(k_scan_apply, dim3(1), dim3(SCAN_BLOCK), 0, s, in, out, n, nullptr, total);
+        LAUNCHCHK();
+        return;
+    }
+    std::string tag = "scan_sums" + std::to_string(depth);
+    uint32_t *sums = a.get_t<uint32_t>(tag, nb);
+    uint32_t *soff = a.get_t<uint32_t>(tag + "o", nb);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_BLOCK), 0, s, in, n, sums);
+    LAUNCHCHK();
+    scan_rec(a, sums, soff, nb, s, nullptr, depth + 1);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_BLOCK), 0, s, in, out, n, soff, total);
+    LAUNCHCHK();
+}
+
+void scan_exclusive_u32(Arena &a, const uint32_t *in, uint32_t *out, uint32_t n, hipStream_t s,
+                        uint32_t *total_dev)
+{
+    if (n == 0) {
+        if (total_dev) HIPCHK(hipMemsetAsync(total_dev, 0, 4, s));
+        return;
+    }
+    scan_rec(a, in, out, n, s, total_dev, 0);
+}
+
+/* ----------------------------------------------------------- radix sort */
+
+static constexpr int RS_BLOCK = 256;
+static constexpr int RS_ROUNDS = 16;
+static constexpr int RS_TILE = RS_BLOCK * RS_ROUNDS;
+
+__device__ __forceinline__ uint32_t rs_digit(const uint4 &k, int d)
+{
+    uint32_t w = (d < 4) ? k.x : (d < 8 ? k.y : k.z);
+    return (w >> ((d & 3) * 8)) & 0xffu;
+}
+
+__global__ __launch_bounds__(RS_BLOCK) void k_rs_hist(const uint4 *__restrict__ keys, uint32_t n, int d,
+                                                      uint32_t *__restrict__ counts, uint32_t nb)
+{
+    __shared__ uint32_t hist[256];
+    hist[threadIdx.x] = 0;
+    __syncthreads();
+    uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+#pragma unroll 4
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        uint64_t j = base + (uint64_t)r * RS_BLOCK + threadIdx.x;
+        if (j < n) atomicAdd(&hist[rs_digit(keys[j], d)], 1u);
+    }
+    __syncthreads();
+    counts[(uint64_t)threadIdx.x * nb + blockIdx.x] = hist[threadIdx.x];
+}
+
+__global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const uint4 *__restrict__ keys, uint4 *__restrict__ out,
+                                                         uint32_t n, int d, const uint32_t *__restrict__ offs,
+                                                         uint32_t nb)
+{
+    __shared__ uint32_t running[256];
+    __shared__ uint32_t wcount[RS_BLOCK / 64][256];
+    const int t = threadIdx.x;
+    const int wave = t >> 6;
+    running[t] = offs[(uint64_t)t * nb + blockIdx.x];
+#pragma unroll
+    for (int w = 0; w < RS_BLOCK / 64; ++w) wcount[w][t] = 0;
+    __syncthreads();
+    const uint64_t lt = lanemask_lt();
+    uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        uint64_t j = base + (uint64_t)r * RS_BLOCK + t;
+        bool valid = j < n;
+        uint4 k = valid ? keys[j] : make_uint4(0, 0, 0, 0);
+        uint32_t dig = valid ? rs_digit(k, d) : 256u;
+        uint64_t peers = __ballot(1);
+#pragma unroll
+        for (int b = 0; b < 9; ++b) {
+            uint64_t m = __ballot((dig >> b) & 1u);
+            peers &= ((dig >> b) & 1u) ? m : ~m;
+        }
+        uint32_t rank = __popcll(peers & lt);
+        if (valid && rank == 0) wcount[wave][dig] = __popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t pre = 0;
+            for (int w = 0; w < wave; ++w) pre += wcount[w][dig];
+            out[running[dig] + pre + rank] = k;
+        }
+        __syncthreads();
+        uint32_t add = 0;
+#pragma unroll
+        for (int w = 0; w < RS_BLOCK / 64; ++w) {
+            add += wcount[w][t];
+            wcount[w][t] = 0;
+        }
+        running[t] += add;
+        __syncthreads();
+    }
+}
+
+void radix_sort_96(Arena &a, uint4 **keys, uint4 **tmp, uint32_t n, int nwords, hipStream_t s, int first_digit)
+{
+    if (n <= 1) return;
+    unsigned nb = ceil_div(n, RS_TILE);
+    uint32_t *counts = a.get_t<uint32_t>("rs_counts", (size_t)256 * nb);
+    uint32_t *offs = a.get_t<uint32_t>("rs_offs", (size_t)256 * nb);
+    int nd = nwords * 4;
+    for (int d = first_digit; d < nd; ++d) {
+        hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(RS_BLOCK), 0, s, *keys, n, d, counts, nb);
+        LAUNCHCHK();
+        scan_exclusive_u32(a, counts, offs, 256 * nb, s);
+        hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(RS_BLOCK), 0, s, *keys, *tmp, n, d, offs, nb);
+        LAUNCHCHK();
+        std::swap(*keys, *tmp);
+    }
+}
+
+}  // namespace plan

@@ -1,0 +1,20 @@
+"""Seeded snapshot cases shared by the parity tests."""
+from plan_amd import synth
+
+
+def dense_case(n, n_clumps, seed, a=4e-4, frac=0.3, knn=32, hop=None, nx=None, decades=1.0):
+    import math
+    nx = nx or max(2, int(math.isqrt(n // 4)))
+    return synth.make_snapshot(n, n_clumps, frac, a_range=(a * 0.5, a * 2), mass_decades=decades, seed=seed,
+                               nx=nx, knn=knn, hop=hop)
+
+
+CASES = {
+    "n4k": lambda: dense_case(4096, 4, 1, a=2e-4),
+    "n8k": lambda: dense_case(8192, 6, 2, a=3e-4),
+    "n32k": lambda: dense_case(32768, 10, 3, a=4e-4),
+    "n65k": lambda: dense_case(65536, 16, 4, a=5e-4),
+    "n32k_k16": lambda: dense_case(32768, 10, 5, a=4e-4, knn=16),
+    "n32k_k64_h64": lambda: dense_case(32768, 10, 6, a=4e-4, knn=64, hop=64),
+    "n16k_qs": lambda: dense_case(16384, 6, 7, a=3e-4, nx=128),
+}
