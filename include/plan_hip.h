@@ -118,11 +118,25 @@ int plan_density_at(plan_ctx *ctx, uint32_t nq, const double *pts, const double 
  * the count in *count (writes at most cap). */
 int plan_ball_query(plan_ctx *ctx, const double *center, double radius, uint32_t *out, uint32_t cap,
                     uint32_t *count);
+/* Keep a device copy of the loaded N-particle set / restore it (so a caller
+ * can rerun ghosts + BuildTree + FindPlanetesimals on resident input). */
+int plan_keep_input(plan_ctx *ctx);
+int plan_restore_input(plan_ctx *ctx);
+/* device bytes held by the context */
+int plan_device_memory(plan_ctx *ctx, uint64_t *bytes);
 /* wall-clock milliseconds of the last pipeline stages: ghosts, build, density,
  * hop, clumps, total (hipEvent timing on the context stream; clumps = host). */
 int plan_stage_times(plan_ctx *ctx, double *ms, int n);
-/* device milliseconds of the last density KNN launch (single kernel) */
+/* counters of the last FindPlanetesimals: particles above threshold, naive
+ * groups, clumps after HOP filters, after merging, merge passes, pair
+ * candidates, saddle queries, rebinding searches, member-table copy ms, 0,
+ * then host ms of: sort, merge A, merge B, re-key, unbind, rebind, filters,
+ * rip, J, ranking */
+int plan_clump_stats(plan_ctx *ctx, double *values, int n);
+/* device milliseconds of the last density pass's gather kernel (the K-nearest
+ * selection and density, one launch) and of its radius-schedule kernel */
 int plan_last_density_kernel_ms(plan_ctx *ctx, double *ms);
+int plan_last_density_radius_ms(plan_ctx *ctx, double *ms);
 
 #ifdef __cplusplus
 }

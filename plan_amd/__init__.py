@@ -47,6 +47,8 @@ EXPORTS = [
     "plan_make_ghost_particles", "plan_build_tree", "plan_knn_density", "plan_hop", "plan_find_planetesimals",
     "plan_catalog_free", "plan_get_tree_particles", "plan_get_keys", "plan_get_nodes", "plan_get_hop",
     "plan_knn_query", "plan_density_at", "plan_ball_query", "plan_stage_times", "plan_last_density_kernel_ms",
+    "plan_keep_input", "plan_restore_input", "plan_device_memory", "plan_clump_stats",
+    "plan_last_density_radius_ms",
 ]
 
 
@@ -83,6 +85,11 @@ def lib():
     L.plan_ball_query.argtypes = [vp, _d, C.c_double, vp, C.c_uint32, C.POINTER(C.c_uint32)]
     L.plan_stage_times.argtypes = [vp, _d, C.c_int]
     L.plan_last_density_kernel_ms.argtypes = [vp, C.POINTER(C.c_double)]
+    L.plan_keep_input.argtypes = [vp]
+    L.plan_restore_input.argtypes = [vp]
+    L.plan_device_memory.argtypes = [vp, C.POINTER(C.c_uint64)]
+    L.plan_clump_stats.argtypes = [vp, _d, C.c_int]
+    L.plan_last_density_radius_ms.argtypes = [vp, C.POINTER(C.c_double)]
     for f in EXPORTS:
         if f in ("plan_ctx_destroy", "plan_catalog_free", "plan_last_error"):
             continue
@@ -243,9 +250,34 @@ class Context:
         self._chk(lib().plan_stage_times(self._h, t, 6), "plan_stage_times")
         return dict(zip(("ghosts", "build", "density", "hop", "clumps", "find_total"), t.tolist()))
 
+    def clump_stats(self) -> dict:
+        v = np.zeros(24)
+        self._chk(lib().plan_clump_stats(self._h, v, 24), "plan_clump_stats")
+        names = ["num_above", "hop_groups", "naive", "after_merge", "merge_passes", "pair_candidates",
+                 "saddle_queries", "rebind_searches", "d2h_ms", "rebind_reruns", "ms_sort", "ms_mergeA", "ms_mergeB",
+                 "ms_rekey", "ms_unbind", "ms_rebind", "ms_filters", "ms_rip", "ms_J", "ms_rank",
+                 "rebind_iters", "rebind_max_iters", "rebind_max_n", "rebind_max_ms"]
+        return {k: float(x) for k, x in zip(names, v) if k != "_"}
+
+    def keep_input(self):
+        self._chk(lib().plan_keep_input(self._h), "plan_keep_input")
+
+    def restore_input(self):
+        self._chk(lib().plan_restore_input(self._h), "plan_restore_input")
+
+    def device_memory(self) -> int:
+        v = C.c_uint64()
+        self._chk(lib().plan_device_memory(self._h, C.byref(v)), "plan_device_memory")
+        return v.value
+
     def density_kernel_ms(self) -> float:
         v = C.c_double()
         self._chk(lib().plan_last_density_kernel_ms(self._h, C.byref(v)), "plan_last_density_kernel_ms")
+        return v.value
+
+    def density_radius_ms(self) -> float:
+        v = C.c_double()
+        self._chk(lib().plan_last_density_radius_ms(self._h, C.byref(v)), "plan_last_density_radius_ms")
         return v.value
 
 

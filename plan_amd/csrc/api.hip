@@ -27,6 +27,7 @@ struct plan_ctx {
     bool have_density = false;
     bool have_hop = false;
     DevParticles ps;
+    DevParticles saved;      /* optional pristine copy of the N input particles */
     DevTree tree;
     DevTree ctree;
     DevParticles cps;
@@ -35,7 +36,11 @@ struct plan_ctx {
     uint32_t K = 0, H = 0;
     int kernel = 0;
     double times[6] = {0, 0, 0, 0, 0, 0};
+    ClumpStats cstats;
+    uint32_t hop_na = 0, hop_groups = 0;
+    double d2h_ms = 0;
     double density_kernel_ms = 0;
+    double density_radius_ms = 0;
     hipEvent_t ev[8];
 };
 
@@ -360,6 +365,7 @@ void plan_ctx_destroy(plan_ctx *c)
     free_tree_nodes(c->ctree);
     free_particles(c->ps);
     free_particles(c->cps);
+    free_particles(c->saved);
     for (auto &e : c->ev) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -500,12 +506,14 @@ static int run_density(plan_ctx *c, uint32_t K, int kernel)
     c->kernel = kernel;
     KnnOut ko;
     ko.rho = c->tree.rho;
+    ko.ev = &c->ev[5];
     HIPCHK(hipEventRecord(c->ev[2], c->stream));
     dev_knn(c->tree, K, 0, kernel, c->tree.n, nullptr, nullptr, nullptr, nullptr, c->d, ko, c->stream);
     HIPCHK(hipEventRecord(c->ev[3], c->stream));
     HIPCHK(hipEventSynchronize(c->ev[3]));
-    c->density_kernel_ms = ms_between(c->ev[2], c->ev[3]);
-    c->times[2] = c->density_kernel_ms;
+    c->density_kernel_ms = ms_between(c->ev[6], c->ev[7]);
+    c->density_radius_ms = ms_between(c->ev[5], c->ev[6]);
+    c->times[2] = ms_between(c->ev[2], c->ev[3]);
     c->have_density = true;
     return PLAN_OK;
 }
@@ -519,11 +527,11 @@ static int run_hop(plan_ctx *c, uint32_t H, double thr)
 {
     if (!c->have_density) return fail(c, PLAN_E_STATE, "densities not computed");
     if (H == 0 || H > c->tree.n) return fail(c, PLAN_E_ARG, "H must be in 1..num_particles");
-    HIPCHK(hipEventRecord(c->ev[4], c->stream));
+    HIPCHK(hipEventRecord(c->ev[0], c->stream));
     dev_hop(c->tree, c->work, c->d, thr, H, c->hop, c->stream);
-    HIPCHK(hipEventRecord(c->ev[5], c->stream));
-    HIPCHK(hipEventSynchronize(c->ev[5]));
-    c->times[3] = ms_between(c->ev[4], c->ev[5]);
+    HIPCHK(hipEventRecord(c->ev[4], c->stream));
+    HIPCHK(hipEventSynchronize(c->ev[4]));
+    c->times[3] = ms_between(c->ev[0], c->ev[4]);
     c->H = H;
     c->have_hop = true;
     return PLAN_OK;
@@ -621,11 +629,16 @@ int plan_find_planetesimals(plan_ctx *c, plan_catalog **out)
             cs.box_max[k] = c->pp.box_max[k];
         }
         cs.num_peaks = c->pp.num_peaks;
+        cs.ball_margin_abs = 2.5 * c->tree.epsilon;
         Services svc(c);
         std::map<uint32_t, HClump> cl;
         double Mp_max = 0, Mp_tot = 0;
         ClumpStats st;
+        c->d2h_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
         run_clump_pipeline(hm, cs, svc, cl, Mp_max, Mp_tot, st);
+        c->cstats = st;
+        c->hop_na = na;
+        c->hop_groups = nc;
         /* catalog */
         plan_catalog *cat = (plan_catalog *)std::calloc(1, sizeof(plan_catalog));
         uint32_t tot = 0;
@@ -873,6 +886,49 @@ int plan_ball_query(plan_ctx *c, const double *center, double radius, uint32_t *
     });
 }
 
+int plan_keep_input(plan_ctx *c)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_particles || c->ps.T != c->ps.N) return fail(c, PLAN_E_STATE, "load particles before ghosts");
+        const uint32_t N = c->ps.N;
+        ensure_particles(c->saved, N);
+        hipStream_t s = c->stream;
+        HIPCHK(hipMemcpyAsync(c->saved.pos, c->ps.pos, (size_t)N * 32, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(c->saved.vel, c->ps.vel, (size_t)N * 32, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(c->saved.prop, c->ps.prop, (size_t)N * 4, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(c->saved.id, c->ps.id, (size_t)N * 4, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+        c->saved.N = c->saved.T = N;
+        return PLAN_OK;
+    });
+}
+
+int plan_restore_input(plan_ctx *c)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->saved.pos || c->saved.N == 0) return fail(c, PLAN_E_STATE, "no kept input");
+        const uint32_t N = c->saved.N;
+        ensure_particles(c->ps, N);
+        hipStream_t s = c->stream;
+        HIPCHK(hipMemcpyAsync(c->ps.pos, c->saved.pos, (size_t)N * 32, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(c->ps.vel, c->saved.vel, (size_t)N * 32, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(c->ps.prop, c->saved.prop, (size_t)N * 4, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(c->ps.id, c->saved.id, (size_t)N * 4, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+        c->ps.N = c->ps.T = N;
+        c->have_particles = true;
+        c->have_tree = c->have_density = c->have_hop = false;
+        return PLAN_OK;
+    });
+}
+
+int plan_device_memory(plan_ctx *c, uint64_t *bytes)
+{
+    if (!c || !bytes) return PLAN_E_ARG;
+    *bytes = c->ps.arena.bytes() + c->tree.arena.bytes() + c->work.bytes() + c->ctree.arena.bytes() + (uint64_t)c->ps.cap * 72 + (uint64_t)c->saved.cap * 72 + (uint64_t)c->tree.num_nodes * 56;
+    return PLAN_OK;
+}
+
 int plan_stage_times(plan_ctx *c, double *ms, int n)
 {
     if (!c || !ms) return PLAN_E_ARG;
@@ -880,10 +936,33 @@ int plan_stage_times(plan_ctx *c, double *ms, int n)
     return PLAN_OK;
 }
 
+int plan_clump_stats(plan_ctx *c, double *v, int n)
+{
+    if (!c || !v) return PLAN_E_ARG;
+    double a[24] = {(double)c->hop_na, (double)c->hop_groups, (double)c->cstats.naive, (double)c->cstats.after_merge,
+                    (double)c->cstats.merge_passes, (double)c->cstats.pair_candidates,
+                    (double)c->cstats.saddle_queries, (double)c->cstats.rebind_searches, c->d2h_ms,
+                    (double)c->cstats.rebind_reruns};
+    for (int k = 0; k < 10; ++k) a[10 + k] = c->cstats.ms[k];
+    a[20] = c->cstats.rebind_iters;
+    a[21] = c->cstats.rebind_max_iters;
+    a[22] = c->cstats.rebind_max_n;
+    a[23] = c->cstats.rebind_max_ms;
+    for (int i = 0; i < n && i < 24; ++i) v[i] = a[i];
+    return PLAN_OK;
+}
+
 int plan_last_density_kernel_ms(plan_ctx *c, double *ms)
 {
     if (!c || !ms) return PLAN_E_ARG;
-    *ms = c->density_kernel_ms;
+    ms[0] = c->density_kernel_ms;
+    return PLAN_OK;
+}
+
+int plan_last_density_radius_ms(plan_ctx *c, double *ms)
+{
+    if (!c || !ms) return PLAN_E_ARG;
+    ms[0] = c->density_radius_ms;
     return PLAN_OK;
 }
 

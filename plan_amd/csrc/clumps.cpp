@@ -14,9 +14,11 @@
 #include "clumps.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <stdexcept>
+#include <unordered_set>
 
 namespace plan {
 
@@ -99,13 +101,51 @@ struct Ops {
         else c.one10th = std::min(c.one10th, c.two_sigma);
     }
     /* SortParticles (src/tree.hpp:4446-4512) */
-    void sort_particles(HClump &c) const
+    void sort_particles(HClump &c, bool nearly_sorted = false) const
     {
         kinematics(c);
         for (uint32_t h : c.indices) c.particles.push_back(PD(h, dist_to(h, c.com)));
-        std::sort(c.particles.begin(), c.particles.end(), pd_less);
+        sort_by_distance(c.particles, nearly_sorted);
         for (size_t i = 0; i < c.particles.size(); ++i) c.indices[i] = c.particles[i].first;
         radii(c);
+    }
+    /* Sort by distance.  Without exact distance ties the sorted order is
+     * unique, so an insertion sort of an almost sorted list (members re-sorted
+     * around a slightly moved centre) gives std::sort's result in O(n); on a
+     * tie, or when the list is far from sorted, std::sort runs on the original
+     * order so libstdc++'s tie order is kept exactly. */
+    static void sort_by_distance(std::vector<PD> &v, bool nearly_sorted)
+    {
+        if (!nearly_sorted || v.size() < 32) {
+            std::sort(v.begin(), v.end(), pd_less);
+            return;
+        }
+        std::vector<PD> orig(v);
+        size_t budget = 16 * v.size() + 1024, moves = 0;
+        bool ok = true;
+        for (size_t i = 1; i < v.size() && ok; ++i) {
+            PD x = v[i];
+            size_t j = i;
+            while (j > 0 && x.second < v[j - 1].second) {
+                v[j] = v[j - 1];
+                --j;
+                if (++moves > budget) {
+                    ok = false;
+                    break;
+                }
+            }
+            v[j] = x;
+        }
+        if (ok)
+            for (size_t i = 1; i < v.size(); ++i)
+                if (v[i].second == v[i - 1].second) {
+                    ok = false;
+                    break;
+                }
+        if (!ok) {
+            v.swap(orig);
+            std::sort(v.begin(), v.end(), pd_less);
+        }
     }
     /* IsPositionDispersion2Large (src/tree.hpp:4516-4528) */
     static bool dispersion_too_large(const HClump &c, double tol = 0.55)
@@ -169,7 +209,7 @@ struct Ops {
             i--;
         }
         std::vector<PD>().swap(c.particles);
-        sort_particles(c);
+        sort_particles(c, true);
     }
     /* CalculateHillRadius (src/tree.hpp:4715-4717) */
     void hill_radius(HClump &c) const
@@ -326,6 +366,7 @@ void merge_loop(std::map<uint32_t, HClump> &cl, const HostMembers &hm, const Clu
                     continue;
                 }
                 cands[a].push_back(cd_);
+                st.pair_candidates++;
             }
         }
         std::vector<double> srho;
@@ -367,38 +408,221 @@ void merge_loop(std::map<uint32_t, HClump> &cl, const HostMembers &hm, const Clu
 
 }  // namespace
 
-/* SearchBoundParticlesWithinHillRadius (src/tree.hpp:4678-4711) over the
- * candidates (tree indices, traversal order, density >= threshold) */
-static void search_bound(HClump &c, const std::vector<uint32_t> &cand, const HostMembers &hm,
-                         std::vector<uint8_t> &in_clump, const ClumpSettings &cs, const Ops &ops)
+/* State of one clump's rebinding: a superset of its Hill-sphere candidates
+ * (records above threshold, ascending tree index, fetched once from the
+ * device for a ball of radius R around c0), what it tested and claimed. */
+struct Rebind {
+    std::vector<uint32_t> sup;
+    std::vector<double> sd;   /* distance of each superset member from c0 */
+    double c0[3] = {0, 0, 0};
+    double R = 0;
+    std::vector<uint32_t> tested, claimed;
+    bool ok = true;
+    uint32_t exact_searches = 0;
+    uint32_t iters = 0;
+    double ms = 0;
+};
+
+static void to_records(const HostMembers &hm, const std::vector<uint32_t> &tidx, std::vector<uint32_t> &out)
+{
+    out.clear();
+    out.reserve(tidx.size());
+    for (uint32_t t : tidx) {
+        uint32_t r = hm.rec_of_tidx(t);
+        if (r == PLAN_NONE) throw std::runtime_error("rebinding candidate above threshold missing from member table");
+        out.push_back(r);
+    }
+}
+
+/* The do-while of SearchBoundParticlesWithinHillRadius + CalculateHillRadius
+ * for one clump (src/tree.hpp:4201-4208, 4678-4711).  Each iteration's ball
+ * (COM, R_Hill) is cut from the superset with the reference's own distance
+ * expression; a candidate within `margin` of the sphere, where the tree's
+ * box tests could disagree with the plain distance test, or a ball leaving
+ * the superset, sends the iteration to the exact device ball search (or, with
+ * no device, reports failure so the clump is redone in order). */
+static bool rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::vector<uint8_t> &base_flags,
+                       const std::unordered_set<uint32_t> *earlier, const ClumpSettings &cs, const Ops &ops,
+                       ClumpServices *svc)
 {
     const double sh = cs.q * cs.Omega;
-    for (uint32_t tidx : cand) {
-        uint32_t r = hm.rec_of_tidx(tidx);
-        if (r == PLAN_NONE) throw std::runtime_error("rebinding candidate above threshold missing from member table");
-        if (in_clump[r]) continue;
-        const MemberRec &q = hm.rec[r];
-        const double *qp = &q.x, *qv = &q.vx;
-        double tm = c.total_mass + q.m;
-        double a = c.com[0] - qp[0];
-        double sv[3] = {0.0 * a, sh * a, 0.0 * a};
-        double n2 = 0;
-        for (int k = 0; k < 3; ++k) {
-            double dv = (c.vel[k] - sv[k]) - qv[k];
-            n2 += dv * dv;
+    std::unordered_set<uint32_t> mine;
+    std::vector<uint32_t> cand;
+    size_t before;
+    auto t_start = std::chrono::steady_clock::now();
+    do {
+        rb.iters++;
+        before = c.indices.size();
+        const double r = c.hill, r2 = r * r;
+        const double margin = cs.ball_margin_abs + 1e-12 * r;
+        bool exact = Ops::dist_pts(c.com, rb.c0) + r + margin > rb.R;
+        cand.clear();
+        if (!exact) {
+            if (rb.sd.size() != rb.sup.size()) {
+                rb.sd.resize(rb.sup.size());
+                for (size_t k = 0; k < rb.sup.size(); ++k) rb.sd[k] = Ops::dist_pts(rb.c0, &hm.rec[rb.sup[k]].x);
+            }
+            /* a member farther than this from c0 is outside the ball by more
+             * than the margin (triangle inequality, generous rounding slack) */
+            const double bound = (Ops::dist_pts(c.com, rb.c0) + r + margin) * (1 + 1e-9) + 1e-300;
+            for (size_t k = 0; k < rb.sup.size(); ++k) {
+                if (rb.sd[k] > bound) continue;
+                const uint32_t h = rb.sup[k];
+                const MemberRec &q = hm.rec[h];
+                double s = 0;
+                double t0 = c.com[0] - q.x, t1 = c.com[1] - q.y, t2 = c.com[2] - q.z;
+                s += t0 * t0;
+                s += t1 * t1;
+                s += t2 * t2;
+                double d = std::sqrt(s);
+                if (std::fabs(d - r) <= margin) {
+                    exact = true;
+                    break;
+                }
+                if (s <= r2) cand.push_back(h);
+            }
         }
-        double e = (0.5 / tm) * n2 - cs.G / Ops::dist_pts(c.com, qp);
-        if (e < 0) {
-            c.indices.push_back(r);
-            c.total_mass = tm;
-            for (int k = 0; k < 3; ++k) c.com[k] = (c.com[k] * c.total_mass + q.m * qp[k]) / c.total_mass;
-            for (int k = 0; k < 3; ++k) c.vel[k] = (c.total_mass * c.vel[k] + q.m * qv[k]) / c.total_mass;
-            in_clump[r] = 1;
+        if (exact) {
+            if (!svc) return false;
+            std::vector<double4> qs{make_double4(c.com[0], c.com[1], c.com[2], r),
+                                    make_double4(c.com[0], c.com[1], c.com[2], 2.0 * r)};
+            std::vector<std::vector<uint32_t>> l;
+            svc->particle_balls(qs, cs.density_threshold, l);
+            rb.exact_searches++;
+            to_records(hm, l[0], cand);
+            to_records(hm, l[1], rb.sup);
+            rb.sd.clear();
+            for (int k = 0; k < 3; ++k) rb.c0[k] = c.com[k];
+            rb.R = 2.0 * r;
         }
+        for (uint32_t h : cand) {
+            if (base_flags[h] || mine.count(h) || (earlier && earlier->count(h))) continue;
+            rb.tested.push_back(h);
+            const MemberRec &q = hm.rec[h];
+            const double *qp = &q.x, *qv = &q.vx;
+            double tm = c.total_mass + q.m;
+            double a = c.com[0] - qp[0];
+            double sv[3] = {0.0 * a, sh * a, 0.0 * a};
+            double n2 = 0;
+            for (int k = 0; k < 3; ++k) {
+                double dv = (c.vel[k] - sv[k]) - qv[k];
+                n2 += dv * dv;
+            }
+            double e = (0.5 / tm) * n2 - cs.G / Ops::dist_pts(c.com, qp);
+            if (e < 0) {
+                c.indices.push_back(h);
+                c.total_mass = tm;
+                for (int k = 0; k < 3; ++k) c.com[k] = (c.com[k] * c.total_mass + q.m * qp[k]) / c.total_mass;
+                for (int k = 0; k < 3; ++k) c.vel[k] = (c.total_mass * c.vel[k] + q.m * qv[k]) / c.total_mass;
+                mine.insert(h);
+                rb.claimed.push_back(h);
+            }
+        }
+        if (c.indices.size() > c.particles.size()) {
+            std::vector<PD>().swap(c.particles);
+            ops.sort_particles(c, true);
+        }
+        ops.hill_radius(c);
+    } while (c.indices.size() > before);
+    rb.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    return true;
+}
+
+/* Rebinding of all clumps in key order (src/tree.hpp:4196-4208).  Clumps are
+ * first run independently in parallel against the flags set before
+ * rebinding; then, in key order, a clump whose tested particles include one
+ * claimed by an earlier clump (or that needed an exact device search) is
+ * rerun from its saved state with the earlier claims applied.  Accepted runs
+ * are exactly the serial ones. */
+static void rebind_all(std::map<uint32_t, HClump> &cl, const HostMembers &hm, const ClumpSettings &cs,
+                       const Ops &ops, ClumpServices &svc, std::vector<uint8_t> &in_clump, ClumpStats &st)
+{
+    std::vector<HClump *> v;
+    std::vector<double4> qs;
+    for (auto &it : cl) {
+        v.push_back(&it.second);
+        qs.push_back(make_double4(it.second.com[0], it.second.com[1], it.second.com[2], 2.0 * it.second.hill));
     }
-    if (c.indices.size() > c.particles.size()) {
-        std::vector<PD>().swap(c.particles);
-        ops.sort_particles(c);
+    if (v.empty()) return;
+    const size_t nc = v.size();
+    std::vector<std::vector<uint32_t>> sup;
+    svc.particle_balls(qs, cs.density_threshold, sup);
+    st.rebind_searches += (uint32_t)qs.size();
+    std::vector<Rebind> base(nc), rb(nc);
+    std::vector<HClump> saved(nc);
+    for (size_t i = 0; i < nc; ++i) {
+        to_records(hm, sup[i], base[i].sup);
+        for (int k = 0; k < 3; ++k) base[i].c0[k] = v[i]->com[k];
+        base[i].R = 2.0 * v[i]->hill;
+        saved[i] = *v[i];
+    }
+    /* Rounds of speculation.  Every open clump runs in parallel from its saved
+     * state against the claims of the clumps validated so far.  A sweep in
+     * key order then accepts a run when none of its claims collides with a
+     * claim accepted during the sweep or with a particle an earlier, still
+     * open clump might claim (its superset).  The first open clump's run is
+     * always exact, so each round accepts at least one clump. */
+    std::vector<char> open(nc, 1);
+    std::vector<uint32_t> added_stamp(hm.rec.size(), 0), blocked_stamp(hm.rec.size(), 0);
+    std::unordered_set<uint32_t> claimed;
+    size_t first_open = 0;
+    uint32_t round = 0;
+    while (first_open < nc) {
+        ++round;
+        std::vector<size_t> todo;
+        for (size_t i = first_open; i < nc; ++i)
+            if (open[i]) todo.push_back(i);
+#pragma omp parallel for schedule(dynamic, 1)
+        for (long t = 0; t < (long)todo.size(); ++t) {
+            size_t i = todo[t];
+            *v[i] = saved[i];
+            rb[i] = Rebind();
+            rb[i].sup = base[i].sup;
+            for (int k = 0; k < 3; ++k) rb[i].c0[k] = base[i].c0[k];
+            rb[i].R = base[i].R;
+            rb[i].ok = rebind_one(*v[i], rb[i], hm, in_clump, &claimed, cs, ops, nullptr);
+        }
+        if (round > 1) st.rebind_reruns += (uint32_t)todo.size();
+        bool earlier_open = false;
+        for (size_t i : todo) {
+            bool ok = rb[i].ok;
+            if (ok)
+                for (uint32_t h : rb[i].claimed)
+                    if (added_stamp[h] == round || blocked_stamp[h] == round) {
+                        ok = false;
+                        break;
+                    }
+            if (!ok && !earlier_open && !rb[i].ok) {
+                /* first open clump that needs the exact device search */
+                *v[i] = saved[i];
+                rb[i] = Rebind();
+                rb[i].sup = base[i].sup;
+                for (int k = 0; k < 3; ++k) rb[i].c0[k] = base[i].c0[k];
+                rb[i].R = base[i].R;
+                rebind_one(*v[i], rb[i], hm, in_clump, &claimed, cs, ops, &svc);
+                ok = true;
+            }
+            if (ok) {
+                open[i] = 0;
+                st.rebind_searches += rb[i].exact_searches;
+                st.rebind_iters += rb[i].iters;
+                if (rb[i].ms > st.rebind_max_ms) {
+                    st.rebind_max_ms = rb[i].ms;
+                    st.rebind_max_iters = rb[i].iters;
+                    st.rebind_max_n = (uint32_t)v[i]->indices.size();
+                }
+                for (uint32_t h : rb[i].claimed) {
+                    claimed.insert(h);
+                    added_stamp[h] = round;
+                    in_clump[h] = 1;
+                }
+            } else {
+                earlier_open = true;
+                for (uint32_t h : rb[i].sup) blocked_stamp[h] = round;
+                for (uint32_t h : rb[i].claimed) blocked_stamp[h] = round;
+            }
+        }
+        while (first_open < nc && !open[first_open]) ++first_open;
     }
 }
 
@@ -417,6 +641,12 @@ void run_clump_pipeline(HostMembers &hm, const ClumpSettings &cs, ClumpServices 
     Ops ops{hm.rec, cs};
     cl.clear();
     Mp_max = Mp_tot = 0;
+    auto tl = std::chrono::steady_clock::now();
+    auto lap = [&](int k) {
+        auto t = std::chrono::steady_clock::now();
+        st.ms[k] += std::chrono::duration<double, std::milli>(t - tl).count();
+        tl = t;
+    };
     const uint32_t nc = (uint32_t)hm.cpeak.size();
     /* naive clumps in key order, members in the serial walk's discovery order
      * (src/tree.hpp:3766-3809) */
@@ -438,11 +668,14 @@ void run_clump_pipeline(HostMembers &hm, const ClumpSettings &cs, ClumpServices 
     }
     if (!cs.enough_resolution) remove_small_mass_low_peak(cl, hm, cs);
     st.naive = (uint32_t)cl.size();
+    lap(0);
 
     /* 4: merging */
     merge_loop(cl, hm, cs, ops, svc, false, st);
+    lap(1);
     for (auto &it : cl) ops.hill_radius(it.second);
     merge_loop(cl, hm, cs, ops, svc, true, st);
+    lap(2);
     st.after_merge = (uint32_t)cl.size();
 
     /* real peaks into keys (src/tree.hpp:4156-4182) */
@@ -470,44 +703,19 @@ void run_clump_pipeline(HostMembers &hm, const ClumpSettings &cs, ClumpServices 
         }
     }
 
+    lap(3);
     /* 5: unbinding and rebinding */
     for_each_clump_parallel(cl, [&](HClump &h) {
         ops.remove_unbound(h);
         ops.hill_radius(h);
     });
+    lap(4);
     std::vector<uint8_t> in_clump(hm.rec.size(), 0);
     for (auto &it : cl)
         for (uint32_t r : it.second.indices) in_clump[r] = 1;
-    {
-        std::vector<double4> qs;
-        for (auto &it : cl)
-            qs.push_back(make_double4(it.second.com[0], it.second.com[1], it.second.com[2], it.second.hill));
-        std::vector<std::vector<uint32_t>> first;
-        if (!qs.empty()) svc.particle_balls(qs, cs.density_threshold, first);
-        st.rebind_searches += (uint32_t)qs.size();
-        size_t ci = 0;
-        for (auto &it : cl) {
-            HClump &h = it.second;
-            size_t before;
-            int iter = 0;
-            do {
-                before = h.indices.size();
-                if (iter == 0) {
-                    search_bound(h, first[ci], hm, in_clump, cs, ops);
-                } else {
-                    std::vector<double4> q1{make_double4(h.com[0], h.com[1], h.com[2], h.hill)};
-                    std::vector<std::vector<uint32_t>> l1;
-                    svc.particle_balls(q1, cs.density_threshold, l1);
-                    st.rebind_searches++;
-                    search_bound(h, l1[0], hm, in_clump, cs, ops);
-                }
-                ops.hill_radius(h);
-                ++iter;
-            } while (h.indices.size() > before);
-            ++ci;
-        }
-    }
+    rebind_all(cl, hm, cs, ops, svc, in_clump, st);
 
+    lap(5);
     /* 6: filters */
     remove_small_mass_low_peak(cl, hm, cs);
     for (auto &it : cl)
@@ -516,6 +724,7 @@ void run_clump_pipeline(HostMembers &hm, const ClumpSettings &cs, ClumpServices 
     for (auto k : del) cl.erase(k);
     del.clear();
 
+    lap(6);
     /* 7: envelope ripping (src/tree.hpp:4229-4249) */
     {
         std::vector<HClump *> v;
@@ -544,6 +753,7 @@ void run_clump_pipeline(HostMembers &hm, const ClumpSettings &cs, ClumpServices 
         for (size_t i = 0; i < v.size(); ++i)
             if (empty[i]) cl.erase(keys[i]);
     }
+    lap(7);
     /* 8 */
     remove_small_mass_low_peak(cl, hm, cs);
     /* 9: clumps centred in the ghost zones */
@@ -557,6 +767,7 @@ void run_clump_pipeline(HostMembers &hm, const ClumpSettings &cs, ClumpServices 
     del.clear();
     /* 10: angular momentum */
     for_each_clump_parallel(cl, [&](HClump &h) { ops.angular_momentum(h); });
+    lap(8);
     /* ranking and optional cap (src/tree.hpp:4291-4323) */
     if (!cl.empty()) {
         std::vector<std::pair<uint32_t, double>> pm;
@@ -574,6 +785,7 @@ void run_clump_pipeline(HostMembers &hm, const ClumpSettings &cs, ClumpServices 
         for (auto &q : pm) Mp_tot = Mp_tot + q.second;
         Mp_max = pm.back().second;
     }
+    lap(9);
 }
 
 }  // namespace plan

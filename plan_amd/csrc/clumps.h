@@ -38,6 +38,10 @@ struct ClumpSettings {
     double q = 1.5;
     double box_min[3], box_max[3];
     uint32_t num_peaks = 0;
+    /* distance band around a ball's surface inside which a plain distance
+     * test may disagree with the tree's box tests (particles sit up to the
+     * Morton quantisation outside their leaf box) */
+    double ball_margin_abs = 0;
 };
 
 /* per-clump state (Planetesimal<3>, src/tree.hpp:4351-4427); members are
@@ -70,6 +74,13 @@ struct ClumpStats {
     uint32_t merge_passes = 0;
     uint32_t saddle_queries = 0;
     uint32_t rebind_searches = 0;
+    uint32_t pair_candidates = 0;
+    uint32_t rebind_reruns = 0;
+    uint32_t rebind_iters = 0, rebind_max_iters = 0, rebind_max_n = 0;
+    double rebind_max_ms = 0;
+    /* host milliseconds: sort, merge A, merge B, re-key, unbind, rebind,
+     * filters, rip, J, ranking */
+    double ms[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 void run_clump_pipeline(HostMembers &hm, const ClumpSettings &cs, ClumpServices &svc,

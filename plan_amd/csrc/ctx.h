@@ -68,6 +68,7 @@ struct KnnOut {
     double *rho = nullptr;     /* nq densities (optional) */
     uint32_t *densest = nullptr; /* nq densest-neighbour ids (hop mode) */
     double *rstop = nullptr;   /* nq final radius (optional) */
+    hipEvent_t *ev = nullptr;  /* 3 events: before radius pass, between passes, after gather */
 };
 /* mode: 0 density, 1 hop (densest neighbour), 2 lists only.
  * Queries are tree particles qidx[i] (or i when qidx is null) or explicit
