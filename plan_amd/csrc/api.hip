@@ -12,6 +12,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <mutex>
 #include <vector>
 
 using namespace plan;
@@ -208,6 +209,7 @@ __global__ void k_split_pts(const double *__restrict__ pts, const double *__rest
 /* device services for the host clump logic */
 struct Services : ClumpServices {
     plan_ctx *c;
+    std::mutex mu;
     explicit Services(plan_ctx *cc) : c(cc) {}
 
     void saddle_density(const std::vector<double4> &pts, std::vector<double> &rho) override
@@ -288,9 +290,11 @@ struct Services : ClumpServices {
         }
     }
 
+    /* called from the parallel rebinding runs: one search on the device at a time */
     void particle_balls(const std::vector<double4> &queries, double rho_min,
                         std::vector<std::vector<uint32_t>> &out) override
     {
+        std::lock_guard<std::mutex> g(mu);
         run_balls(c->tree, queries, c->tree.rho, rho_min, 1, out);
     }
 };
@@ -939,7 +943,7 @@ int plan_stage_times(plan_ctx *c, double *ms, int n)
 int plan_clump_stats(plan_ctx *c, double *v, int n)
 {
     if (!c || !v) return PLAN_E_ARG;
-    double a[24] = {(double)c->hop_na, (double)c->hop_groups, (double)c->cstats.naive, (double)c->cstats.after_merge,
+    double a[30] = {(double)c->hop_na, (double)c->hop_groups, (double)c->cstats.naive, (double)c->cstats.after_merge,
                     (double)c->cstats.merge_passes, (double)c->cstats.pair_candidates,
                     (double)c->cstats.saddle_queries, (double)c->cstats.rebind_searches, c->d2h_ms,
                     (double)c->cstats.rebind_reruns};
@@ -948,7 +952,9 @@ int plan_clump_stats(plan_ctx *c, double *v, int n)
     a[21] = c->cstats.rebind_max_iters;
     a[22] = c->cstats.rebind_max_n;
     a[23] = c->cstats.rebind_max_ms;
-    for (int i = 0; i < n && i < 24; ++i) v[i] = a[i];
+    for (int k = 0; k < 3; ++k) a[24 + k] = c->cstats.mparts[k];
+    for (int k = 0; k < 3; ++k) a[27 + k] = c->cstats.rparts[k];
+    for (int i = 0; i < n && i < 30; ++i) v[i] = a[i];
     return PLAN_OK;
 }
 

@@ -147,6 +147,48 @@ struct Ops {
             std::sort(v.begin(), v.end(), pd_less);
         }
     }
+    /* std::sort of v = A ++ B by distance, where A (the first na entries) is
+     * nearly sorted: A by insertion, B by std::sort, then a linear merge.
+     * Without distance ties the sorted order is unique and equals std::sort's;
+     * on a tie (or a far-from-sorted A) std::sort runs on the original v. */
+    static void sort_union(std::vector<PD> &v, size_t na)
+    {
+        if (v.size() < 64 || na < 32) {
+            std::sort(v.begin(), v.end(), pd_less);
+            return;
+        }
+        std::vector<PD> a(v.begin(), v.begin() + na), b(v.begin() + na, v.end());
+        size_t budget = 16 * a.size() + 1024, moves = 0;
+        bool ok = true;
+        for (size_t i = 1; i < a.size() && ok; ++i) {
+            PD x = a[i];
+            size_t j = i;
+            while (j > 0 && x.second < a[j - 1].second) {
+                a[j] = a[j - 1];
+                --j;
+                if (++moves > budget) {
+                    ok = false;
+                    break;
+                }
+            }
+            a[j] = x;
+        }
+        if (ok) {
+            std::sort(b.begin(), b.end(), pd_less);
+            std::vector<PD> m(v.size());
+            std::merge(a.begin(), a.end(), b.begin(), b.end(), m.begin(), pd_less);
+            for (size_t i = 1; i < m.size(); ++i)
+                if (m[i].second == m[i - 1].second) {
+                    ok = false;
+                    break;
+                }
+            if (ok) {
+                v.swap(m);
+                return;
+            }
+        }
+        std::sort(v.begin(), v.end(), pd_less);
+    }
     /* IsPositionDispersion2Large (src/tree.hpp:4516-4528) */
     static bool dispersion_too_large(const HClump &c, double tol = 0.55)
     {
@@ -168,9 +210,10 @@ struct Ops {
         for (int k = 0; k < 3; ++k) a.com[k] = (a.com[k] * a.total_mass + b.com[k] * b.total_mass) / tot;
         for (int k = 0; k < 3; ++k) a.vel[k] = (a.vel[k] * a.total_mass + b.vel[k] * b.total_mass) / tot;
         a.total_mass = tot;
+        const size_t na = a.particles.size();
         a.particles.insert(a.particles.end(), b.particles.begin(), b.particles.end());
         for (auto &it : a.particles) it.second = dist_to(it.first, a.com);
-        std::sort(a.particles.begin(), a.particles.end(), pd_less);
+        sort_union(a.particles, na);
         a.indices.resize(a.particles.size());
         for (size_t i = 0; i < a.particles.size(); ++i) a.indices[i] = a.particles[i].first;
         radii(a);
@@ -318,7 +361,10 @@ void merge_loop(std::map<uint32_t, HClump> &cl, const HostMembers &hm, const Clu
             p1s.push_back(it);
         }
         std::vector<std::vector<uint32_t>> near;
+        auto t0 = std::chrono::steady_clock::now();
         svc.clump_tree_balls(pts, vel, queries, near);
+        auto t1 = std::chrono::steady_clock::now();
+        st.mparts[0] += std::chrono::duration<double, std::milli>(t1 - t0).count();
         /* every saddle density this pass could ask for, in one device batch */
         struct Cand {
             uint32_t p1k, p2k;
@@ -372,6 +418,8 @@ void merge_loop(std::map<uint32_t, HClump> &cl, const HostMembers &hm, const Clu
         std::vector<double> srho;
         if (!sq.empty()) svc.saddle_density(sq, srho);
         st.saddle_queries += (uint32_t)sq.size();
+        auto t2 = std::chrono::steady_clock::now();
+        st.mparts[1] += std::chrono::duration<double, std::milli>(t2 - t1).count();
         /* greedy replay */
         happened = false;
         std::vector<std::pair<uint32_t, uint32_t>> pairs;
@@ -396,6 +444,7 @@ void merge_loop(std::map<uint32_t, HClump> &cl, const HostMembers &hm, const Clu
             ops.merge(a, b, rho_of_key(hm, a.peak_index), rho_of_key(hm, b.peak_index));
             if (hill) ops.hill_radius(a);
         }
+        st.mparts[2] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t2).count();
         for (auto &it : cl) {
             if (it.second.mask && Ops::dispersion_too_large(it.second)) {
                 it.second.mask = false;
@@ -410,14 +459,12 @@ void merge_loop(std::map<uint32_t, HClump> &cl, const HostMembers &hm, const Clu
 
 /* State of one clump's rebinding: a superset of its Hill-sphere candidates
  * (records above threshold, ascending tree index, fetched once from the
- * device for a ball of radius R around c0), what it tested and claimed. */
+ * device for a ball of radius R around c0) and what it claimed. */
 struct Rebind {
     std::vector<uint32_t> sup;
-    std::vector<double> sd;   /* distance of each superset member from c0 */
     double c0[3] = {0, 0, 0};
     double R = 0;
-    std::vector<uint32_t> tested, claimed;
-    bool ok = true;
+    std::vector<uint32_t> claimed;
     uint32_t exact_searches = 0;
     uint32_t iters = 0;
     double ms = 0;
@@ -439,15 +486,29 @@ static void to_records(const HostMembers &hm, const std::vector<uint32_t> &tidx,
  * (COM, R_Hill) is cut from the superset with the reference's own distance
  * expression; a candidate within `margin` of the sphere, where the tree's
  * box tests could disagree with the plain distance test, or a ball leaving
- * the superset, sends the iteration to the exact device ball search (or, with
- * no device, reports failure so the clump is redone in order). */
-static bool rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::vector<uint8_t> &base_flags,
-                       const std::unordered_set<uint32_t> *earlier, const ClumpSettings &cs, const Ops &ops,
-                       ClumpServices *svc)
+ * the superset, sends the iteration to the exact device ball search, which
+ * also refreshes the superset. */
+static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::vector<uint8_t> &flags,
+                       const ClumpSettings &cs, const Ops &ops, ClumpServices &svc)
 {
     const double sh = cs.q * cs.Omega;
     std::unordered_set<uint32_t> mine;
-    std::vector<uint32_t> cand;
+    /* superset records that are neither flagged nor claimed by this run,
+     * ascending tree index, with their distance from c0 */
+    std::vector<uint32_t> live;
+    std::vector<double> lsd;
+    auto rebuild = [&]() {
+        live.clear();
+        lsd.clear();
+        for (uint32_t h : rb.sup)
+            if (!flags[h] && !mine.count(h)) {
+                live.push_back(h);
+                lsd.push_back(Ops::dist_pts(rb.c0, &hm.rec[h].x));
+            }
+    };
+    rebuild();
+    std::vector<uint32_t> cand, cpos;
+    std::vector<uint8_t> dead;
     size_t before;
     auto t_start = std::chrono::steady_clock::now();
     do {
@@ -457,17 +518,14 @@ static bool rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
         const double margin = cs.ball_margin_abs + 1e-12 * r;
         bool exact = Ops::dist_pts(c.com, rb.c0) + r + margin > rb.R;
         cand.clear();
+        cpos.clear();
         if (!exact) {
-            if (rb.sd.size() != rb.sup.size()) {
-                rb.sd.resize(rb.sup.size());
-                for (size_t k = 0; k < rb.sup.size(); ++k) rb.sd[k] = Ops::dist_pts(rb.c0, &hm.rec[rb.sup[k]].x);
-            }
             /* a member farther than this from c0 is outside the ball by more
              * than the margin (triangle inequality, generous rounding slack) */
             const double bound = (Ops::dist_pts(c.com, rb.c0) + r + margin) * (1 + 1e-9) + 1e-300;
-            for (size_t k = 0; k < rb.sup.size(); ++k) {
-                if (rb.sd[k] > bound) continue;
-                const uint32_t h = rb.sup[k];
+            for (size_t k = 0; k < live.size(); ++k) {
+                if (lsd[k] > bound) continue;
+                const uint32_t h = live[k];
                 const MemberRec &q = hm.rec[h];
                 double s = 0;
                 double t0 = c.com[0] - q.x, t1 = c.com[1] - q.y, t2 = c.com[2] - q.z;
@@ -479,25 +537,29 @@ static bool rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
                     exact = true;
                     break;
                 }
-                if (s <= r2) cand.push_back(h);
+                if (s <= r2) {
+                    cand.push_back(h);
+                    cpos.push_back((uint32_t)k);
+                }
             }
         }
         if (exact) {
-            if (!svc) return false;
             std::vector<double4> qs{make_double4(c.com[0], c.com[1], c.com[2], r),
                                     make_double4(c.com[0], c.com[1], c.com[2], 2.0 * r)};
             std::vector<std::vector<uint32_t>> l;
-            svc->particle_balls(qs, cs.density_threshold, l);
+            svc.particle_balls(qs, cs.density_threshold, l);
             rb.exact_searches++;
             to_records(hm, l[0], cand);
             to_records(hm, l[1], rb.sup);
-            rb.sd.clear();
+            cpos.clear();
             for (int k = 0; k < 3; ++k) rb.c0[k] = c.com[k];
             rb.R = 2.0 * r;
         }
-        for (uint32_t h : cand) {
-            if (base_flags[h] || mine.count(h) || (earlier && earlier->count(h))) continue;
-            rb.tested.push_back(h);
+        dead.assign(exact ? 0 : live.size(), 0);
+        bool any = false;
+        for (size_t ci = 0; ci < cand.size(); ++ci) {
+            const uint32_t h = cand[ci];
+            if (exact && (flags[h] || mine.count(h))) continue;
             const MemberRec &q = hm.rec[h];
             const double *qp = &q.x, *qv = &q.vx;
             double tm = c.total_mass + q.m;
@@ -516,7 +578,22 @@ static bool rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
                 for (int k = 0; k < 3; ++k) c.vel[k] = (c.total_mass * c.vel[k] + q.m * qv[k]) / c.total_mass;
                 mine.insert(h);
                 rb.claimed.push_back(h);
+                if (!exact) dead[cpos[ci]] = 1;
+                any = true;
             }
+        }
+        if (exact) {
+            rebuild();
+        } else if (any) {
+            size_t w = 0;
+            for (size_t k = 0; k < live.size(); ++k)
+                if (!dead[k]) {
+                    live[w] = live[k];
+                    lsd[w] = lsd[k];
+                    ++w;
+                }
+            live.resize(w);
+            lsd.resize(w);
         }
         if (c.indices.size() > c.particles.size()) {
             std::vector<PD>().swap(c.particles);
@@ -525,15 +602,16 @@ static bool rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
         ops.hill_radius(c);
     } while (c.indices.size() > before);
     rb.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
-    return true;
 }
 
-/* Rebinding of all clumps in key order (src/tree.hpp:4196-4208).  Clumps are
- * first run independently in parallel against the flags set before
- * rebinding; then, in key order, a clump whose tested particles include one
- * claimed by an earlier clump (or that needed an exact device search) is
- * rerun from its saved state with the earlier claims applied.  Accepted runs
- * are exactly the serial ones. */
+/* Rebinding of all clumps in key order (src/tree.hpp:4196-4208).  Every
+ * clump first runs in parallel against the flags set before rebinding (the
+ * device ball searches an iteration may need do not depend on flags).  Then,
+ * in key order, a run is kept when none of its claims was claimed by an
+ * earlier clump; otherwise the clump is rerun from its saved state with the
+ * flags as they now stand.  A particle a run only tested, without claiming
+ * it, leaves the run's state unchanged whoever owns it, so every kept run is
+ * exactly the serial one. */
 static void rebind_all(std::map<uint32_t, HClump> &cl, const HostMembers &hm, const ClumpSettings &cs,
                        const Ops &ops, ClumpServices &svc, std::vector<uint8_t> &in_clump, ClumpStats &st)
 {
@@ -545,85 +623,56 @@ static void rebind_all(std::map<uint32_t, HClump> &cl, const HostMembers &hm, co
     }
     if (v.empty()) return;
     const size_t nc = v.size();
+    auto tt = std::chrono::steady_clock::now();
+    auto part = [&](int k) {
+        auto t = std::chrono::steady_clock::now();
+        st.rparts[k] += std::chrono::duration<double, std::milli>(t - tt).count();
+        tt = t;
+    };
     std::vector<std::vector<uint32_t>> sup;
     svc.particle_balls(qs, cs.density_threshold, sup);
     st.rebind_searches += (uint32_t)qs.size();
-    std::vector<Rebind> base(nc), rb(nc);
+    std::vector<Rebind> rb(nc);
     std::vector<HClump> saved(nc);
-    for (size_t i = 0; i < nc; ++i) {
-        to_records(hm, sup[i], base[i].sup);
-        for (int k = 0; k < 3; ++k) base[i].c0[k] = v[i]->com[k];
-        base[i].R = 2.0 * v[i]->hill;
-        saved[i] = *v[i];
-    }
-    /* Rounds of speculation.  Every open clump runs in parallel from its saved
-     * state against the claims of the clumps validated so far.  A sweep in
-     * key order then accepts a run when none of its claims collides with a
-     * claim accepted during the sweep or with a particle an earlier, still
-     * open clump might claim (its superset).  The first open clump's run is
-     * always exact, so each round accepts at least one clump. */
-    std::vector<char> open(nc, 1);
-    std::vector<uint32_t> added_stamp(hm.rec.size(), 0), blocked_stamp(hm.rec.size(), 0);
-    std::unordered_set<uint32_t> claimed;
-    size_t first_open = 0;
-    uint32_t round = 0;
-    while (first_open < nc) {
-        ++round;
-        std::vector<size_t> todo;
-        for (size_t i = first_open; i < nc; ++i)
-            if (open[i]) todo.push_back(i);
+    auto start = [&](size_t i) {
+        *v[i] = saved[i];
+        rb[i] = Rebind();
+        to_records(hm, sup[i], rb[i].sup);
+        rb[i].c0[0] = qs[i].x;
+        rb[i].c0[1] = qs[i].y;
+        rb[i].c0[2] = qs[i].z;
+        rb[i].R = qs[i].w;
+    };
+    for (size_t i = 0; i < nc; ++i) saved[i] = *v[i];
+    part(0);
 #pragma omp parallel for schedule(dynamic, 1)
-        for (long t = 0; t < (long)todo.size(); ++t) {
-            size_t i = todo[t];
-            *v[i] = saved[i];
-            rb[i] = Rebind();
-            rb[i].sup = base[i].sup;
-            for (int k = 0; k < 3; ++k) rb[i].c0[k] = base[i].c0[k];
-            rb[i].R = base[i].R;
-            rb[i].ok = rebind_one(*v[i], rb[i], hm, in_clump, &claimed, cs, ops, nullptr);
-        }
-        if (round > 1) st.rebind_reruns += (uint32_t)todo.size();
-        bool earlier_open = false;
-        for (size_t i : todo) {
-            bool ok = rb[i].ok;
-            if (ok)
-                for (uint32_t h : rb[i].claimed)
-                    if (added_stamp[h] == round || blocked_stamp[h] == round) {
-                        ok = false;
-                        break;
-                    }
-            if (!ok && !earlier_open && !rb[i].ok) {
-                /* first open clump that needs the exact device search */
-                *v[i] = saved[i];
-                rb[i] = Rebind();
-                rb[i].sup = base[i].sup;
-                for (int k = 0; k < 3; ++k) rb[i].c0[k] = base[i].c0[k];
-                rb[i].R = base[i].R;
-                rebind_one(*v[i], rb[i], hm, in_clump, &claimed, cs, ops, &svc);
-                ok = true;
-            }
-            if (ok) {
-                open[i] = 0;
-                st.rebind_searches += rb[i].exact_searches;
-                st.rebind_iters += rb[i].iters;
-                if (rb[i].ms > st.rebind_max_ms) {
-                    st.rebind_max_ms = rb[i].ms;
-                    st.rebind_max_iters = rb[i].iters;
-                    st.rebind_max_n = (uint32_t)v[i]->indices.size();
-                }
-                for (uint32_t h : rb[i].claimed) {
-                    claimed.insert(h);
-                    added_stamp[h] = round;
-                    in_clump[h] = 1;
-                }
-            } else {
-                earlier_open = true;
-                for (uint32_t h : rb[i].sup) blocked_stamp[h] = round;
-                for (uint32_t h : rb[i].claimed) blocked_stamp[h] = round;
-            }
-        }
-        while (first_open < nc && !open[first_open]) ++first_open;
+    for (long i = 0; i < (long)nc; ++i) {
+        start(i);
+        rebind_one(*v[i], rb[i], hm, in_clump, cs, ops, svc);
     }
+    part(1);
+    for (size_t i = 0; i < nc; ++i) {
+        bool ok = true;
+        for (uint32_t h : rb[i].claimed)
+            if (in_clump[h]) {
+                ok = false;
+                break;
+            }
+        if (!ok) {
+            start(i);
+            rebind_one(*v[i], rb[i], hm, in_clump, cs, ops, svc);
+            st.rebind_reruns++;
+        }
+        for (uint32_t h : rb[i].claimed) in_clump[h] = 1;
+        st.rebind_searches += rb[i].exact_searches;
+        st.rebind_iters += rb[i].iters;
+        if (rb[i].ms > st.rebind_max_ms) {
+            st.rebind_max_ms = rb[i].ms;
+            st.rebind_max_iters = rb[i].iters;
+            st.rebind_max_n = (uint32_t)v[i]->indices.size();
+        }
+    }
+    part(2);
 }
 
 template <class F> static void for_each_clump_parallel(std::map<uint32_t, HClump> &cl, F f)

@@ -81,6 +81,10 @@ struct ClumpStats {
     /* host milliseconds: sort, merge A, merge B, re-key, unbind, rebind,
      * filters, rip, J, ranking */
     double ms[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    /* merge loops: clump-tree searches, saddle densities, applying merges;
+     * rebinding: superset searches, parallel runs, in-order validation */
+    double mparts[3] = {0, 0, 0};
+    double rparts[3] = {0, 0, 0};
 };
 
 void run_clump_pipeline(HostMembers &hm, const ClumpSettings &cs, ClumpServices &svc,

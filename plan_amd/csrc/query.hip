@@ -36,6 +36,9 @@ struct TreeView {
     double to_diag;
     double cell_len[3], half_cell[3], cell_sq[3], three_half_cell[3];
     double cell_volume;
+    /* bound on how far a particle can sit outside its node's box (Morton
+     * quantisation of the key grid), with a wide allowance for rounding */
+    double margin_abs;
 };
 
 static TreeView view_of(const DevTree &t, const plan_derived &d)
@@ -58,6 +61,7 @@ static TreeView view_of(const DevTree &t, const plan_derived &d)
         v.three_half_cell[k] = v.half_cell[k] * 3.;
     }
     v.cell_volume = d.cell_volume;
+    v.margin_abs = 2.5 * t.epsilon;
     return v;
 }
 
@@ -101,6 +105,25 @@ __device__ __forceinline__ bool sphere_intersect(const NodeGeo &g, double qx, do
     if (qz < lo) md = __dadd_rn(md, __dmul_rn(__dsub_rn(qz, lo), __dsub_rn(qz, lo)));
     else if (qz > hi) md = __dadd_rn(md, __dmul_rn(__dsub_rn(qz, hi), __dsub_rn(qz, hi)));
     return md <= __dmul_rn(r, r);
+}
+
+/* Geometric node tests for exact distance queries (a particle lies within
+ * m of its node's box): the whole box within r - m of q (farthest corner),
+ * or some point of the box within r + m of q. */
+__device__ __forceinline__ bool geo_contain(const NodeGeo &g, double qx, double qy, double qz, double r, double m)
+{
+    const double rr = r - m;
+    if (!(rr > 0)) return false;
+    const double mx = fabs(g.cx - qx) + g.hw, my = fabs(g.cy - qy) + g.hw, mz = fabs(g.cz - qz) + g.hw;
+    return mx * mx + my * my + mz * mz < rr * rr;
+}
+
+__device__ __forceinline__ bool geo_intersect(const NodeGeo &g, double qx, double qy, double qz, double r, double m)
+{
+    const double ex = fmax(fabs(g.cx - qx) - g.hw, 0.0), ey = fmax(fabs(g.cy - qy) - g.hw, 0.0),
+                 ez = fmax(fabs(g.cz - qz) - g.hw, 0.0);
+    const double rr = r + m;
+    return ex * ex + ey * ey + ez * ez <= rr * rr;
 }
 
 /* x^1.5 correctly rounded in all but astronomically rare cases:
@@ -332,7 +355,8 @@ __device__ __forceinline__ void query_setup(const TreeView &tv, uint32_t q, cons
 template <int K>
 __global__ __launch_bounds__(64) void k_knn_radius(TreeView tv, uint32_t nq, const uint32_t *__restrict__ qidx,
                                                    const double4 *__restrict__ qpts, const double *__restrict__ est,
-                                                   double *__restrict__ r_out, uint32_t *__restrict__ c_out)
+                                                   const double *__restrict__ rt, double *__restrict__ r_out,
+                                                   uint32_t *__restrict__ c_out)
 {
     __shared__ NodeCache nc;
     const int lane = threadIdx.x;
@@ -342,9 +366,12 @@ __global__ __launch_bounds__(64) void k_knn_radius(TreeView tv, uint32_t nq, con
     uint32_t self = 0;
     if (valid) query_setup<K>(tv, q, qidx, qpts, est, qx, qy, qz, r, self);
     LaneHeap<K> none;
-    bool done = !valid;
+    /* a point query resolved by k_point_tight: S(est) holds >= K particles,
+     * so the schedule stops at est */
+    const bool tight = valid && rt && rt[q] > 0;
+    bool done = !valid || tight;
     int rounds = 0;
-    uint32_t inside = 0;
+    uint32_t inside = tight ? (uint32_t)K : 0u;
     while (__ballot(!done) && rounds++ < 2200) {
         uint32_t cnt = 0, cin = 0;
         walk<K, false>(tv, nc, nullptr, qx, qy, qz, r, !done, cnt, none, &cin);
@@ -361,6 +388,103 @@ __global__ __launch_bounds__(64) void k_knn_radius(TreeView tv, uint32_t nq, con
         r_out[q] = r;
         c_out[q] = inside;
     }
+}
+
+/* Exact count of particles with d^2 <= r^2 from q, by a lock-step walk with
+ * the geometric node tests (whole nodes counted by size, boundary leaves
+ * particle by particle). */
+__device__ __forceinline__ uint32_t walk_geo_count(const TreeView &tv, NodeCache &nc, double qx, double qy, double qz,
+                                                   double r, bool active)
+{
+    uint32_t cov_end = active ? 0u : 0xffffffffu;
+    uint32_t nd = 0, lo = 1, hi = 0, steps = 0, count = 0;
+    const double r2 = r * r, m = tv.margin_abs;
+    while (nd != PLAN_NONE && steps++ <= tv.nn) {
+        nd = uniform_u32(nd);
+        if (nd < lo || nd >= hi) cache_fetch(tv, nc, nd, lo, hi);
+        const uint32_t s = nd - lo;
+        const NodeGeo g = nc.g[s];
+        const NodeTop t = nc.t[s];
+        const bool leaf = (nc.info[s] & 0xffu) == 0u;
+        int act = 0; /* 1 whole, 4 boundary leaf, 2 descend, 3 reject */
+        if (t.begin >= cov_end) {
+            if (geo_contain(g, qx, qy, qz, r, m)) act = 1;
+            else if (geo_intersect(g, qx, qy, qz, r, m)) act = leaf ? 4 : 2;
+            else act = 3;
+            if (act != 2) cov_end = t.end;
+        }
+        if (act == 1) count += t.end - t.begin;
+        if (act == 4)
+            for (uint32_t p = t.begin; p < t.end; ++p) {
+                const double4 x = tv.pm[p];
+                if (dist2(qx, qy, qz, x.x, x.y, x.z) <= r2) ++count;
+            }
+        nd = __ballot(act == 2) ? t.first_daughter : nc.skip[s];
+    }
+    return count;
+}
+
+/* Point queries (saddle points, user points) with a start radius est: when
+ * some radius r_t <= est - margin already holds >= K particles within
+ * distance r_t, every particle within r_t belongs to the reference's
+ * candidate set S(est) (src/tree.hpp:3160-3230), so the K smallest
+ * (d^2, index) of S(est) are the K nearest overall and can be gathered from
+ * the ball of radius r_t instead of all of S(est).  r_t starts at the half
+ * width of the deepest node holding q with >= K/4 particles and doubles.
+ * rt_out[q] = r_t, or 0 when no such radius exists (the query then takes
+ * the candidate-set path). */
+template <int K>
+__global__ __launch_bounds__(64) void k_point_tight(TreeView tv, uint32_t nq, const double4 *__restrict__ qpts,
+                                                    const double *__restrict__ est, double *__restrict__ rt_out)
+{
+    __shared__ NodeCache nc;
+    const int lane = threadIdx.x;
+    const uint32_t q = blockIdx.x * 64 + lane;
+    const bool valid = q < nq;
+    double qx = 0, qy = 0, qz = 0, e = 0, r = 0, lim = 0;
+    if (valid) {
+        const double4 p = qpts[q];
+        qx = p.x;
+        qy = p.y;
+        qz = p.z;
+        e = est[q];
+        lim = e - (tv.margin_abs + 1e-12 * e);
+        uint32_t node = 0;
+        for (int lvl = 0; lvl < 32; ++lvl) {
+            const uint32_t nd = tv.ninfo[node] & 0xffu;
+            if (nd == 0) break;
+            const uint32_t fd = tv.ntop[node].first_daughter;
+            uint32_t next = PLAN_NONE;
+            for (uint32_t c = 0; c < nd; ++c) {
+                const NodeGeo g = tv.ngeo[fd + c];
+                if (fabs(qx - g.cx) <= g.hw && fabs(qy - g.cy) <= g.hw && fabs(qz - g.cz) <= g.hw) {
+                    next = fd + c;
+                    break;
+                }
+            }
+            if (next == PLAN_NONE) break;
+            const NodeTop t = tv.ntop[next];
+            if (t.end - t.begin < (uint32_t)(K / 4 > 0 ? K / 4 : 1)) break;
+            node = next;
+        }
+        r = tv.ngeo[node].hw;
+    }
+    bool done = !valid || !(r > 0) || !(r <= lim);
+    bool ok = false;
+    int rounds = 0;
+    while (__ballot(!done) && rounds++ < 64) {
+        const uint32_t cnt = walk_geo_count(tv, nc, qx, qy, qz, r, !done);
+        if (!done) {
+            if (cnt >= (uint32_t)K) {
+                done = true;
+                ok = true;
+            } else {
+                r = r * 2.0;
+                if (!(r <= lim)) done = true;
+            }
+        }
+    }
+    if (valid) rt_out[q] = ok ? r : 0.0;
 }
 
 /* One entry of a wave's include list: a node whose particles some lanes take
@@ -380,7 +504,7 @@ struct WalkState {
  * fit is re-evaluated on resume (lane state is only updated after it is
  * recorded). */
 __device__ __forceinline__ bool walk_fill(const TreeView &tv, NodeCache &nc, WalkState &w, double qx, double qy,
-                                          double qz, double r, ListEnt *lst, uint32_t &nent)
+                                          double qz, double r, bool geo, ListEnt *lst, uint32_t &nent)
 {
     const int lane = threadIdx.x;
     uint32_t guard = 0;
@@ -393,9 +517,15 @@ __device__ __forceinline__ bool walk_fill(const TreeView &tv, NodeCache &nc, Wal
         const bool leaf = (nc.info[s] & 0xffu) == 0u;
         int act = 0;
         if (t.begin >= w.cov_end) {
-            if (sphere_contain(g, qx, qy, qz, r, tv.to_diag)) act = 1;
-            else if (sphere_intersect(g, qx, qy, qz, r, tv.to_diag)) act = leaf ? 1 : 2;
-            else act = 3;
+            if (geo) {
+                if (geo_contain(g, qx, qy, qz, r, tv.margin_abs)) act = 1;
+                else if (geo_intersect(g, qx, qy, qz, r, tv.margin_abs)) act = leaf ? 1 : 2;
+                else act = 3;
+            } else {
+                if (sphere_contain(g, qx, qy, qz, r, tv.to_diag)) act = 1;
+                else if (sphere_intersect(g, qx, qy, qz, r, tv.to_diag)) act = leaf ? 1 : 2;
+                else act = 3;
+            }
         }
         const uint64_t inc = __ballot(act == 1);
         if (inc) {
@@ -566,9 +696,10 @@ __device__ __forceinline__ void stream_entries(const TreeView &tv, const ListEnt
 template <int K, bool RESUME>
 __global__ __launch_bounds__(64) void k_knn_gather(TreeView tv, uint32_t nq, const uint32_t *__restrict__ qidx,
                                                    const double4 *__restrict__ qpts, const double *__restrict__ r_in,
-                                                   const uint32_t *__restrict__ c_in, double *__restrict__ ld2,
-                                                   uint32_t *__restrict__ lidx, uint32_t q0, uint32_t nslots,
-                                                   const uint32_t *__restrict__ qslot, uint32_t *__restrict__ ovf)
+                                                   const uint32_t *__restrict__ c_in, const double *__restrict__ rt_in,
+                                                   double *__restrict__ ld2, uint32_t *__restrict__ lidx, uint32_t q0,
+                                                   uint32_t nslots, const uint32_t *__restrict__ qslot,
+                                                   uint32_t *__restrict__ ovf)
 {
     __shared__ NodeCache nc;
     __shared__ ListEnt lst[LCAP];
@@ -588,11 +719,18 @@ __global__ __launch_bounds__(64) void k_knn_gather(TreeView tv, uint32_t nq, con
         const double rr = __dadd_rn(r, 1e-9);
         T = __dmul_rn(rr, rr);
     }
+    /* a point query with a tight radius (k_point_tight): >= K particles lie
+     * within r_t, and the K nearest of S(est) are the K nearest within r_t */
+    const bool geo = valid && rt_in && rt_in[q] > 0;
+    if (geo) {
+        r = rt_in[q];
+        T = r * r;
+    }
     WalkState w{0u, valid ? 0u : 0xffffffffu, 1u, 0u};
     RegList<K> best;
     if (!RESUME) {
         uint32_t nent = 0;
-        const bool fin = walk_fill(tv, nc, w, qx, qy, qz, r, lst, nent);
+        const bool fin = walk_fill(tv, nc, w, qx, qy, qz, r, geo, lst, nent);
         if (!fin) {
             if (valid) ovf[q] = 1u;
             return;
@@ -605,7 +743,7 @@ __global__ __launch_bounds__(64) void k_knn_gather(TreeView tv, uint32_t nq, con
         bool fin = false;
         while (!fin) {
             uint32_t nent = 0;
-            fin = walk_fill(tv, nc, w, qx, qy, qz, r, lst, nent);
+            fin = walk_fill(tv, nc, w, qx, qy, qz, r, geo, lst, nent);
             __syncthreads();
             stream_entries<K>(tv, lst, nent, qx, qy, qz, T, best);
             __syncthreads();
@@ -629,8 +767,9 @@ __global__ void k_ovf_compact(const uint32_t *__restrict__ ovf, const uint32_t *
 /* the overflow redo: queries named by slot, with the batch's query arrays */
 __global__ void k_ovf_gather_args(const uint32_t *__restrict__ slots, uint32_t m, const uint32_t *__restrict__ qidx,
                                   const double4 *__restrict__ qpts, const double *__restrict__ r_in,
-                                  const uint32_t *__restrict__ c_in, uint32_t q0, uint32_t *__restrict__ qi2,
-                                  double4 *__restrict__ qp2, double *__restrict__ r2, uint32_t *__restrict__ c2)
+                                  const uint32_t *__restrict__ c_in, const double *__restrict__ rt_in, uint32_t q0,
+                                  uint32_t *__restrict__ qi2, double4 *__restrict__ qp2, double *__restrict__ r2,
+                                  uint32_t *__restrict__ c2, double *__restrict__ rt2)
 {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= m) return;
@@ -638,6 +777,7 @@ __global__ void k_ovf_gather_args(const uint32_t *__restrict__ slots, uint32_t m
     if (qpts) qp2[k] = qpts[q];
     else qi2[k] = qidx ? qidx[q] : q0 + q;
     r2[k] = r_in[q];
+    rt2[k] = rt_in ? rt_in[q] : 0.0;
     c2[k] = c_in[q];
 }
 
@@ -728,8 +868,15 @@ static void launch_knn(const DevTree &t, const TreeView &tv, uint32_t nq, const 
     uint32_t *cbuf = a.get_t<uint32_t>("knn_c", (size_t)nq);
     double *ld2 = a.get_t<double>("knn_ld2", (size_t)batch * K);
     uint32_t *lidx = a.get_t<uint32_t>("knn_lidx", (size_t)batch * K);
+    double *rt = nullptr;
     if (out.ev) HIPCHK(hipEventRecord(out.ev[0], s));
-    hipLaunchKernelGGL(k_knn_radius<K>, dim3(ceil_div(nq, 64)), dim3(64), 0, s, tv, nq, qidx, qpts, est, rbuf, cbuf);
+    if (qpts) {
+        rt = a.get_t<double>("knn_rt", (size_t)nq);
+        hipLaunchKernelGGL(k_point_tight<K>, dim3(ceil_div(nq, 64)), dim3(64), 0, s, tv, nq, qpts, est, rt);
+        LAUNCHCHK();
+    }
+    hipLaunchKernelGGL(k_knn_radius<K>, dim3(ceil_div(nq, 64)), dim3(64), 0, s, tv, nq, qidx, qpts, est, rt, rbuf,
+                       cbuf);
     LAUNCHCHK();
     if (out.ev) HIPCHK(hipEventRecord(out.ev[1], s));
     for (uint32_t b0 = 0; b0 < nq; b0 += batch) {
@@ -743,7 +890,7 @@ static void launch_knn(const DevTree &t, const TreeView &tv, uint32_t nq, const 
         uint32_t *otot = a.get_t<uint32_t>("knn_otot", 4);
         HIPCHK(hipMemsetAsync(ovf, 0, (size_t)nb * 4, s));
         hipLaunchKernelGGL((k_knn_gather<K, false>), dim3(ceil_div(nb, 64)), dim3(64), 0, s, tv, nb, qi, qp,
-                           rbuf + b0, cbuf + b0, ld2, lidx, q0, nb, nullptr, ovf);
+                           rbuf + b0, cbuf + b0, rt ? rt + b0 : nullptr, ld2, lidx, q0, nb, nullptr, ovf);
         LAUNCHCHK();
         scan_exclusive_u32(a, ovf, opos, nb, s, otot);
         uint32_t novf = 0;
@@ -755,13 +902,15 @@ static void launch_knn(const DevTree &t, const TreeView &tv, uint32_t nq, const 
             double4 *qp2 = a.get_t<double4>("knn_oqp", novf);
             double *r2 = a.get_t<double>("knn_or", novf);
             uint32_t *c2 = a.get_t<uint32_t>("knn_oc", novf);
+            double *rt2 = a.get_t<double>("knn_ort", novf);
             hipLaunchKernelGGL(k_ovf_compact, dim3(ceil_div(nb, 256)), dim3(256), 0, s, ovf, opos, nb, slots);
             LAUNCHCHK();
             hipLaunchKernelGGL(k_ovf_gather_args, dim3(ceil_div(novf, 256)), dim3(256), 0, s, slots, novf, qi, qp,
-                               rbuf + b0, cbuf + b0, q0, qi2, qp2, r2, c2);
+                               rbuf + b0, cbuf + b0, rt ? rt + b0 : nullptr, q0, qi2, qp2, r2, c2, rt2);
             LAUNCHCHK();
             hipLaunchKernelGGL((k_knn_gather<K, true>), dim3(ceil_div(novf, 64)), dim3(64), 0, s, tv, novf,
-                               qp ? nullptr : qi2, qp ? qp2 : nullptr, r2, c2, ld2, lidx, 0u, nb, slots, ovf);
+                               qp ? nullptr : qi2, qp ? qp2 : nullptr, r2, c2, rt ? rt2 : nullptr, ld2, lidx, 0u,
+                               nb, slots, ovf);
             LAUNCHCHK();
         }
         if (out.ev && b0 == 0) HIPCHK(hipEventRecord(out.ev[2], s));

@@ -100,6 +100,38 @@ def test_knn_sets(name):
 
 
 @pytest.mark.parametrize("name", NAMES)
+def test_point_knn(name):
+    """Point queries with a start radius (the saddle-point form): random
+    points, points next to particles and midpoints of particle pairs, start
+    radii from far below to far above the K-th neighbour distance."""
+    snap, ot, g = run_case(name)
+    ctx = g["ctx"]
+    K = snap.params["knn"]
+    n = ot.n
+    rng = np.random.default_rng(1)
+    op = ot.particles()
+    pos = op["pos"]
+    lo, hi = pos.min(0), pos.max(0)
+    m = 1500
+    a = rng.integers(0, n, m)
+    b = rng.integers(0, n, m)
+    pts = np.concatenate([
+        rng.uniform(lo, hi, (m, 3)),
+        pos[a] + rng.normal(0, 1e-6 * (hi - lo).max(), (m, 3)),
+        0.5 * (pos[a] + pos[b]),
+    ])
+    est = 10.0 ** rng.uniform(-6, -1.3, len(pts)) * (hi - lo).max()
+    ok = ot.knn(pts, K, est)
+    gk = ctx.knn(K, pts=pts, est=est)
+    tied = ok["ndrop"] > 0
+    assert np.array_equal(gk["rstop"][~tied], ok["rstop"][~tied])
+    same = np.all(np.sort(gk["idx"], 1) == np.sort(ok["idx"], 1), axis=1)
+    assert same[~tied].all(), np.flatnonzero(~same & ~tied)[:10]
+    assert np.array_equal(gk["d2"][~tied], ok["d2"][~tied])
+    print(f"{name}: {len(pts)} point queries, {tied.sum()} tied, candidate sets up to {ok['ncand'].max()}")
+
+
+@pytest.mark.parametrize("name", NAMES)
 def test_hop(name):
     snap, ot, g = run_case(name)
     odn, opk = ot.hop()
