@@ -42,6 +42,7 @@ struct plan_ctx {
     double d2h_ms = 0;
     double density_kernel_ms = 0;
     double density_radius_ms = 0;
+    uint32_t density_ovf_waves = 0;
     hipEvent_t ev[8];
 };
 
@@ -511,6 +512,8 @@ static int run_density(plan_ctx *c, uint32_t K, int kernel)
     KnnOut ko;
     ko.rho = c->tree.rho;
     ko.ev = &c->ev[5];
+    c->density_ovf_waves = 0;
+    ko.novf = &c->density_ovf_waves;
     HIPCHK(hipEventRecord(c->ev[2], c->stream));
     dev_knn(c->tree, K, 0, kernel, c->tree.n, nullptr, nullptr, nullptr, nullptr, c->d, ko, c->stream);
     HIPCHK(hipEventRecord(c->ev[3], c->stream));
@@ -943,7 +946,7 @@ int plan_stage_times(plan_ctx *c, double *ms, int n)
 int plan_clump_stats(plan_ctx *c, double *v, int n)
 {
     if (!c || !v) return PLAN_E_ARG;
-    double a[30] = {(double)c->hop_na, (double)c->hop_groups, (double)c->cstats.naive, (double)c->cstats.after_merge,
+    double a[32] = {(double)c->hop_na, (double)c->hop_groups, (double)c->cstats.naive, (double)c->cstats.after_merge,
                     (double)c->cstats.merge_passes, (double)c->cstats.pair_candidates,
                     (double)c->cstats.saddle_queries, (double)c->cstats.rebind_searches, c->d2h_ms,
                     (double)c->cstats.rebind_reruns};
@@ -954,7 +957,9 @@ int plan_clump_stats(plan_ctx *c, double *v, int n)
     a[23] = c->cstats.rebind_max_ms;
     for (int k = 0; k < 3; ++k) a[24 + k] = c->cstats.mparts[k];
     for (int k = 0; k < 3; ++k) a[27 + k] = c->cstats.rparts[k];
-    for (int i = 0; i < n && i < 30; ++i) v[i] = a[i];
+    a[30] = c->density_ovf_waves;
+    a[31] = c->hop.knn_overflow_waves;
+    for (int i = 0; i < n && i < 32; ++i) v[i] = a[i];
     return PLAN_OK;
 }
 

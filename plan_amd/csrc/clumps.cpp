@@ -17,6 +17,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <exception>
 #include <stdexcept>
 #include <unordered_set>
 
@@ -24,6 +25,23 @@ namespace plan {
 
 typedef std::pair<uint32_t, double> PD;
 static bool pd_less(const PD &a, const PD &b) { return a.second < b.second; }
+
+/* OpenMP loop over [0, n), one index at a time; the first exception thrown by
+ * a body is rethrown on the calling thread after the loop. */
+template <class F> static void parallel_for(long n, F f)
+{
+    std::exception_ptr err = nullptr;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long i = 0; i < n; ++i) {
+        try {
+            f(i);
+        } catch (...) {
+#pragma omp critical(plan_clump_error)
+            if (!err) err = std::current_exception();
+        }
+    }
+    if (err) std::rethrow_exception(err);
+}
 
 uint32_t HostMembers::rec_of_tidx(uint32_t tidx) const
 {
@@ -150,12 +168,13 @@ struct Ops {
     /* std::sort of v = A ++ B by distance, where A (the first na entries) is
      * nearly sorted: A by insertion, B by std::sort, then a linear merge.
      * Without distance ties the sorted order is unique and equals std::sort's;
-     * on a tie (or a far-from-sorted A) std::sort runs on the original v. */
-    static void sort_union(std::vector<PD> &v, size_t na)
+     * on a tie (or a far-from-sorted A) std::sort runs on the original v, or,
+     * when strict, v is left unsorted and false is returned. */
+    static bool sort_union(std::vector<PD> &v, size_t na, bool strict = false)
     {
-        if (v.size() < 64 || na < 32) {
+        if (!strict && (v.size() < 64 || na < 32)) {
             std::sort(v.begin(), v.end(), pd_less);
-            return;
+            return true;
         }
         std::vector<PD> a(v.begin(), v.begin() + na), b(v.begin() + na, v.end());
         size_t budget = 16 * a.size() + 1024, moves = 0;
@@ -184,10 +203,12 @@ struct Ops {
                 }
             if (ok) {
                 v.swap(m);
-                return;
+                return true;
             }
         }
+        if (strict) return false;
         std::sort(v.begin(), v.end(), pd_less);
+        return true;
     }
     /* IsPositionDispersion2Large (src/tree.hpp:4516-4528) */
     static bool dispersion_too_large(const HClump &c, double tol = 0.55)
@@ -217,6 +238,54 @@ struct Ops {
         a.indices.resize(a.particles.size());
         for (size_t i = 0; i < a.particles.size(); ++i) a.indices[i] = a.particles[i].first;
         radii(a);
+    }
+    /* Several MergeAnotherPlanetesimal calls into one predator in a row
+     * (bs in order).  Mass, centre, velocity and peaks are updated merge by
+     * merge as the reference does; the particle list is re-sorted once, by
+     * distance to the final centre, which is the last merge's result whenever
+     * the final distances have no ties (the sorted order is then unique).
+     * With a tie the merges are replayed one by one. */
+    void merge_batch(HClump &a, const std::vector<const HClump *> &bs, const HostMembers &hm, bool hill) const
+    {
+        if (bs.size() == 1) {
+            merge(a, *bs[0], rho_of(hm, a.peak_index), rho_of(hm, bs[0]->peak_index));
+            if (hill) hill_radius(a);
+            return;
+        }
+        HClump a0 = a;
+        const size_t na = a.particles.size();
+        for (const HClump *b : bs) {
+            if (rho_of(hm, a.peak_index) < rho_of(hm, b->peak_index)) {
+                a.subpeaks.push_back(a.peak_index);
+                a.peak_index = b->peak_index;
+            } else {
+                a.subpeaks.push_back(b->peak_index);
+            }
+            double tot = a.total_mass + b->total_mass;
+            for (int k = 0; k < 3; ++k) a.com[k] = (a.com[k] * a.total_mass + b->com[k] * b->total_mass) / tot;
+            for (int k = 0; k < 3; ++k) a.vel[k] = (a.vel[k] * a.total_mass + b->vel[k] * b->total_mass) / tot;
+            a.total_mass = tot;
+            a.particles.insert(a.particles.end(), b->particles.begin(), b->particles.end());
+        }
+        for (auto &it : a.particles) it.second = dist_to(it.first, a.com);
+        if (!sort_union(a.particles, na, true)) {
+            a = a0;
+            for (const HClump *b : bs) {
+                merge(a, *b, rho_of(hm, a.peak_index), rho_of(hm, b->peak_index));
+                if (hill) hill_radius(a);
+            }
+            return;
+        }
+        a.indices.resize(a.particles.size());
+        for (size_t i = 0; i < a.particles.size(); ++i) a.indices[i] = a.particles[i].first;
+        radii(a);
+        if (hill) hill_radius(a);
+    }
+    static double rho_of(const HostMembers &hm, uint32_t tidx)
+    {
+        uint32_t r = hm.rec_of_tidx(tidx);
+        if (r == PLAN_NONE) throw std::runtime_error("clump peak is not a particle above the density threshold");
+        return hm.rec[r].rho;
     }
     /* RemoveUnboundParticles (src/tree.hpp:4615-4674) */
     void remove_unbound(HClump &c) const
@@ -438,11 +507,52 @@ void merge_loop(std::map<uint32_t, HClump> &cl, const HostMembers &hm, const Clu
             }
             if (pairs.size() > before) happened = true;
         }
-        for (auto &pr : pairs) {
-            HClump &a = cl[pr.first];
-            const HClump &b = cl[pr.second];
-            ops.merge(a, b, rho_of_key(hm, a.peak_index), rho_of_key(hm, b.peak_index));
-            if (hill) ops.hill_radius(a);
+        /* The pairs in order, grouped into runs with one predator.  A run
+         * reads its preys' final states, so it waits for the runs in which
+         * they ate (earlier in the list); runs of one level are independent
+         * and go in parallel. */
+        {
+            struct Run {
+                HClump *a;
+                std::vector<const HClump *> bs;
+                uint32_t level;
+            };
+            std::vector<Run> runs;
+            std::map<uint32_t, uint32_t> wlevel, rlevel; /* last level writing / reading a clump */
+            uint32_t max_level = 0;
+            for (size_t i = 0; i < pairs.size();) {
+                size_t j = i;
+                while (j < pairs.size() && pairs[j].first == pairs[i].first) ++j;
+                Run r;
+                r.a = &cl[pairs[i].first];
+                uint32_t lv = 0;
+                auto w = wlevel.find(pairs[i].first);
+                if (w != wlevel.end()) lv = std::max(lv, w->second);
+                auto rd = rlevel.find(pairs[i].first);
+                if (rd != rlevel.end()) lv = std::max(lv, rd->second);
+                for (size_t k = i; k < j; ++k) {
+                    r.bs.push_back(&cl[pairs[k].second]);
+                    auto wb = wlevel.find(pairs[k].second);
+                    if (wb != wlevel.end()) lv = std::max(lv, wb->second);
+                }
+                r.level = lv + 1;
+                max_level = std::max(max_level, r.level);
+                wlevel[pairs[i].first] = r.level;
+                for (size_t k = i; k < j; ++k) {
+                    uint32_t &x = rlevel[pairs[k].second];
+                    x = std::max(x, r.level);
+                }
+                runs.push_back(std::move(r));
+                i = j;
+            }
+            for (uint32_t lv = 1; lv <= max_level; ++lv) {
+                std::vector<Run *> now;
+                for (auto &r : runs)
+                    if (r.level == lv) now.push_back(&r);
+                std::sort(now.begin(), now.end(),
+                          [](const Run *x, const Run *y) { return x->a->particles.size() > y->a->particles.size(); });
+                parallel_for((long)now.size(), [&](long k) { ops.merge_batch(*now[k]->a, now[k]->bs, hm, hill); });
+            }
         }
         st.mparts[2] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t2).count();
         for (auto &it : cl) {
@@ -645,11 +755,16 @@ static void rebind_all(std::map<uint32_t, HClump> &cl, const HostMembers &hm, co
     };
     for (size_t i = 0; i < nc; ++i) saved[i] = *v[i];
     part(0);
-#pragma omp parallel for schedule(dynamic, 1)
-    for (long i = 0; i < (long)nc; ++i) {
+    /* longest first: superset size as the cost estimate */
+    std::vector<size_t> order(nc);
+    for (size_t i = 0; i < nc; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t x, size_t y) { return sup[x].size() > sup[y].size(); });
+    parallel_for((long)nc, [&](long k) {
+        const size_t i = order[k];
         start(i);
         rebind_one(*v[i], rb[i], hm, in_clump, cs, ops, svc);
-    }
+    });
     part(1);
     for (size_t i = 0; i < nc; ++i) {
         bool ok = true;
@@ -680,8 +795,7 @@ template <class F> static void for_each_clump_parallel(std::map<uint32_t, HClump
     std::vector<HClump *> v;
     v.reserve(cl.size());
     for (auto &it : cl) v.push_back(&it.second);
-#pragma omp parallel for schedule(dynamic, 1)
-    for (long i = 0; i < (long)v.size(); ++i) f(*v[i]);
+    parallel_for((long)v.size(), [&](long i) { f(*v[i]); });
 }
 
 void run_clump_pipeline(HostMembers &hm, const ClumpSettings &cs, ClumpServices &svc,
@@ -783,8 +897,7 @@ void run_clump_pipeline(HostMembers &hm, const ClumpSettings &cs, ClumpServices 
             keys.push_back(it.first);
         }
         std::vector<uint8_t> empty(v.size(), 0);
-#pragma omp parallel for schedule(dynamic, 1)
-        for (long i = 0; i < (long)v.size(); ++i) {
+        parallel_for((long)v.size(), [&](long i) {
             HClump &c = *v[i];
             while (c.hill < c.particles.back().second) {
                 auto low = std::lower_bound(c.particles.begin(), c.particles.end(), PD(0, c.hill), pd_less);
@@ -798,7 +911,7 @@ void run_clump_pipeline(HostMembers &hm, const ClumpSettings &cs, ClumpServices 
                 ops.remove_unbound(c);
                 ops.hill_radius(c);
             }
-        }
+        });
         for (size_t i = 0; i < v.size(); ++i)
             if (empty[i]) cl.erase(keys[i]);
     }

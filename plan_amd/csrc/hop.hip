@@ -160,6 +160,8 @@ void dev_hop(const DevTree &t, Arena &a, const plan_derived &d, double thr, uint
     uint32_t *dn = a.get_t<uint32_t>("h_dn", na);
     KnnOut ko;
     ko.densest = dn;
+    res.knn_overflow_waves = 0;
+    ko.novf = &res.knn_overflow_waves;
     dev_knn(t, H, 1, 0, na, res.a_idx, nullptr, nullptr, t.rho, d, ko, s);
     res.densest = dn;
     /* pointer jumping */

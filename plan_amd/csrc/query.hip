@@ -142,105 +142,6 @@ __device__ __forceinline__ double pow15(double x)
     return __dadd_rn(ph, __dadd_rn(pl, c));
 }
 
-/* per-lane bounded max-heap on (d2, idx) in LDS, lane-interleaved */
-template <int K> struct LaneHeap {
-    double (*d)[64];
-    uint32_t (*ix)[64];
-    int lane;
-    int size;
-    double top_d;
-    uint32_t top_i;
-
-    __device__ __forceinline__ static bool less(double ad, uint32_t ai, double bd, uint32_t bi)
-    {
-        return ad < bd || (ad == bd && ai < bi);
-    }
-    __device__ __forceinline__ void insert(double v, uint32_t id)
-    {
-        if (size < K) {
-            int j = size++;
-            while (j > 0) {
-                int p = (j - 1) >> 1;
-                double pd = d[p][lane];
-                uint32_t pi = ix[p][lane];
-                if (less(pd, pi, v, id)) {
-                    d[j][lane] = pd;
-                    ix[j][lane] = pi;
-                    j = p;
-                } else
-                    break;
-            }
-            d[j][lane] = v;
-            ix[j][lane] = id;
-            top_d = d[0][lane];
-            top_i = ix[0][lane];
-        } else if (less(v, id, top_d, top_i)) {
-            sift_down_from_root(v, id, K);
-            top_d = d[0][lane];
-            top_i = ix[0][lane];
-        }
-    }
-    __device__ __forceinline__ void sift_down_from_root(double v, uint32_t id, int lim) { sift_down(0, v, id, lim); }
-    __device__ __forceinline__ void sift_down(int j, double v, uint32_t id, int lim)
-    {
-        while (true) {
-            int l = 2 * j + 1;
-            if (l >= lim) break;
-            int c = l;
-            double cd = d[l][lane];
-            uint32_t ci = ix[l][lane];
-            if (l + 1 < lim) {
-                double rd = d[l + 1][lane];
-                uint32_t ri = ix[l + 1][lane];
-                if (less(cd, ci, rd, ri)) {
-                    c = l + 1;
-                    cd = rd;
-                    ci = ri;
-                }
-            }
-            if (less(v, id, cd, ci)) {
-                d[j][lane] = cd;
-                ix[j][lane] = ci;
-                j = c;
-            } else
-                break;
-        }
-        d[j][lane] = v;
-        ix[j][lane] = id;
-    }
-    /* unordered append while filling, then one bottom-up heapify */
-    __device__ __forceinline__ void push_unordered(double v, uint32_t id)
-    {
-        d[size][lane] = v;
-        ix[size][lane] = id;
-        if (++size == K) {
-            for (int j = K / 2 - 1; j >= 0; --j) sift_down(j, d[j][lane], ix[j][lane], K);
-            top_d = d[0][lane];
-            top_i = ix[0][lane];
-        }
-    }
-    __device__ __forceinline__ void offer(double v, uint32_t id)
-    {
-        if (size < K) push_unordered(v, id);
-        else if (less(v, id, top_d, top_i)) {
-            sift_down(0, v, id, K);
-            top_d = d[0][lane];
-            top_i = ix[0][lane];
-        }
-    }
-    /* in-place heap sort to ascending order */
-    __device__ void sort_ascending()
-    {
-        for (int end = size - 1; end > 0; --end) {
-            double vd = d[end][lane];
-            uint32_t vi = ix[end][lane];
-            d[end][lane] = d[0][lane];
-            ix[end][lane] = ix[0][lane];
-            sift_down_from_root(vd, vi, end);
-        }
-    }
-};
-
 /* Up to 8 consecutive nodes (a sibling group: siblings are contiguous in the
  * level-major array) staged in LDS by one vector load and then read by the
  * whole wave; pre-order walks visit siblings back to back, so one fetch
@@ -267,16 +168,12 @@ __device__ __forceinline__ void cache_fetch(const TreeView &tv, NodeCache &c, ui
     hi = min(v + 8u, tv.nn);
 }
 
-/* Lock-step pre-order walk for the 64 lanes of a wave at per-lane radius r.
- * COUNT: adds |S_q(r)| per lane.  GATHER: offers every candidate of S_q(r)
- * to the lane's heap; candidate particles of a node are staged in LDS 64 at
- * a time with coalesced loads. */
-template <int K, bool GATHER>
-__device__ __forceinline__ void walk(const TreeView &tv, NodeCache &nc, double4 *stage, double qx, double qy,
-                                     double qz, double r, bool active, uint32_t &count, LaneHeap<K> &heap,
-                                     uint32_t *contained = nullptr)
+/* Lock-step pre-order walk for the 64 lanes of a wave at per-lane radius r:
+ * count = |S_q(r)| (whole nodes and intersecting leaves, by size) and
+ * contained = the part of it in nodes the sphere contains. */
+__device__ __forceinline__ void walk_count(const TreeView &tv, NodeCache &nc, double qx, double qy, double qz,
+                                           double r, bool active, uint32_t &count, uint32_t &contained)
 {
-    const int lane = threadIdx.x;
     uint32_t cov_end = active ? 0u : 0xffffffffu;
     uint32_t nd = 0, lo = 1, hi = 0;
     uint32_t steps = 0;
@@ -297,24 +194,8 @@ __device__ __forceinline__ void walk(const TreeView &tv, NodeCache &nc, double4 
             else act = 3;
             if (act != 2) cov_end = t.end;
         }
-        if (!GATHER && contained && whole) *contained += t.end - t.begin;
-        if (GATHER) {
-            if (__ballot(act == 1)) {
-                for (uint32_t b = t.begin; b < t.end; b += 64) {
-                    const uint32_t cnt = min(64u, t.end - b);
-                    __syncthreads();
-                    if ((uint32_t)lane < cnt) stage[lane] = tv.pm[b + lane];
-                    __syncthreads();
-                    if (act == 1)
-                        for (uint32_t j = 0; j < cnt; ++j) {
-                            const double4 x = stage[j];
-                            heap.insert(dist2(qx, qy, qz, x.x, x.y, x.z), b + j);
-                        }
-                }
-            }
-        } else {
-            if (act == 1) count += t.end - t.begin;
-        }
+        if (whole) contained += t.end - t.begin;
+        if (act == 1) count += t.end - t.begin;
         nd = __ballot(act == 2) ? t.first_daughter : nc.skip[s];
     }
 }
@@ -365,7 +246,6 @@ __global__ __launch_bounds__(64) void k_knn_radius(TreeView tv, uint32_t nq, con
     double qx = 0, qy = 0, qz = 0, r = 0;
     uint32_t self = 0;
     if (valid) query_setup<K>(tv, q, qidx, qpts, est, qx, qy, qz, r, self);
-    LaneHeap<K> none;
     /* a point query resolved by k_point_tight: S(est) holds >= K particles,
      * so the schedule stops at est */
     const bool tight = valid && rt && rt[q] > 0;
@@ -374,7 +254,7 @@ __global__ __launch_bounds__(64) void k_knn_radius(TreeView tv, uint32_t nq, con
     uint32_t inside = tight ? (uint32_t)K : 0u;
     while (__ballot(!done) && rounds++ < 2200) {
         uint32_t cnt = 0, cin = 0;
-        walk<K, false>(tv, nc, nullptr, qx, qy, qz, r, !done, cnt, none, &cin);
+        walk_count(tv, nc, qx, qy, qz, r, !done, cnt, cin);
         if (!done) {
             if (cnt >= (uint32_t)K) {
                 done = true;
@@ -493,7 +373,7 @@ struct ListEnt {
     uint32_t begin, end;
     uint64_t mask;
 };
-static constexpr int LCAP = 512;
+static constexpr int LCAP = 256;
 
 struct WalkState {
     uint32_t nd, cov_end, lo, hi;
@@ -539,224 +419,6 @@ __device__ __forceinline__ bool walk_fill(const TreeView &tv, NodeCache &nc, Wal
     return true;
 }
 
-/* Stream the particles of the recorded entries 64 at a time (coalesced,
- * next chunk prefetched into registers while the current one is used) and
- * call f(lane-included?, d2, tree index) for each staged particle. */
-template <class F>
-__device__ __forceinline__ void stream_list(const TreeView &tv, const ListEnt *lst, uint32_t nent, uint32_t *lpre,
-                                            double4 *stage, uint32_t *stage_p, uint64_t *stage_m, double qx,
-                                            double qy, double qz, F f)
-{
-    const int lane = threadIdx.x;
-    __syncthreads();
-    {
-        /* prefix of entry sizes, two entries per lane */
-        uint32_t e0 = 2 * lane, e1 = 2 * lane + 1;
-        uint32_t s0 = e0 < nent ? lst[e0].end - lst[e0].begin : 0u;
-        uint32_t s1 = e1 < nent ? lst[e1].end - lst[e1].begin : 0u;
-        uint32_t sum = s0 + s1, inc = sum;
-        for (int o = 1; o < 64; o <<= 1) {
-            uint32_t tt = __shfl_up(inc, o, 64);
-            if (lane >= o) inc += tt;
-        }
-        uint32_t ex = inc - sum;
-        if (e0 < nent) lpre[e0] = ex;
-        if (e1 < nent) lpre[e1] = ex + s0;
-        if (lane == 63) lpre[nent] = inc;
-    }
-    __syncthreads();
-    const uint32_t total = lpre[nent];
-    auto locate = [&](uint32_t fpos, uint32_t &pidx, uint64_t &m) {
-        uint32_t lo = 0, hi = nent; /* lpre[lo] <= fpos < lpre[hi] */
-        while (hi - lo > 1) {
-            uint32_t mid = (lo + hi) >> 1;
-            if (lpre[mid] <= fpos) lo = mid;
-            else hi = mid;
-        }
-        pidx = lst[lo].begin + (fpos - lpre[lo]);
-        m = lst[lo].mask;
-    };
-    double4 nx = make_double4(0, 0, 0, 0);
-    uint32_t np = 0;
-    uint64_t nm = 0;
-    if ((uint32_t)lane < total) {
-        locate(lane, np, nm);
-        nx = tv.pm[np];
-    }
-    for (uint32_t b = 0; b < total; b += 64) {
-        const uint32_t cnt = min(64u, total - b);
-        __syncthreads();
-        stage[lane] = nx;
-        stage_p[lane] = np;
-        stage_m[lane] = nm;
-        __syncthreads();
-        const uint32_t fn = b + 64 + lane;
-        if (fn < total) {
-            locate(fn, np, nm);
-            nx = tv.pm[np];
-        }
-        for (uint32_t j = 0; j < cnt; ++j) {
-            const bool mine = (stage_m[j] >> lane) & 1ull;
-            if (mine) {
-                const double4 x = stage[j];
-                f(dist2(qx, qy, qz, x.x, x.y, x.z), stage_p[j]);
-            }
-        }
-    }
-}
-
-/* The K smallest (d^2, index) seen so far, ascending, in registers.  Every
- * array index is a compile-time constant (fully unrolled), so the list never
- * touches LDS or scratch; an insertion is one branch-free pass of compares
- * and selects, identical for all lanes that take it. */
-template <int K> struct RegList {
-    double d[K];
-    uint32_t i[K];
-
-    __device__ __forceinline__ static bool less(double ad, uint32_t ai, double bd, uint32_t bi)
-    {
-        return ad < bd || (ad == bd && ai < bi);
-    }
-    __device__ __forceinline__ void init()
-    {
-#pragma unroll
-        for (int j = 0; j < K; ++j) {
-            d[j] = __longlong_as_double(0x7ff0000000000000ll);
-            i[j] = 0xffffffffu;
-        }
-    }
-    __device__ __forceinline__ bool accepts(double x, uint32_t xi) const { return less(x, xi, d[K - 1], i[K - 1]); }
-    /* insert (x, xi): one ascending pass keeps the smaller of (x, entry j)
-     * at j and carries the larger on; the largest of the K + 1 drops out */
-    __device__ __forceinline__ void insert(double x, uint32_t xi)
-    {
-#pragma unroll
-        for (int j = 0; j < K; ++j) {
-            const bool lt = less(x, xi, d[j], i[j]);
-            const double lo = lt ? x : d[j];
-            const double hi = lt ? d[j] : x;
-            const uint32_t ilo = lt ? xi : i[j];
-            const uint32_t ihi = lt ? i[j] : xi;
-            d[j] = lo;
-            i[j] = ilo;
-            x = hi;
-            xi = ihi;
-        }
-    }
-};
-
-/* Stream this lane's entries of the recorded include list into the list. */
-template <int K>
-__device__ __forceinline__ void stream_entries(const TreeView &tv, const ListEnt *lst, uint32_t nent, double qx,
-                                               double qy, double qz, double T, RegList<K> &best)
-{
-    const int lane = threadIdx.x;
-    uint32_t e = 0, p = 0, pe = 0;
-    while (p >= pe && e < nent) {
-        const ListEnt le = lst[e++];
-        if ((le.mask >> lane) & 1ull) {
-            p = le.begin;
-            pe = le.end;
-        }
-    }
-    /* the list is only touched outside divergent code: one insertion per
-     * iteration for the whole wave, with a +inf key on lanes that have
-     * nothing to add, so its registers are updated in place */
-    while (__ballot(p < pe)) {
-        double d2 = __longlong_as_double(0x7ff0000000000000ll);
-        uint32_t pi = 0xffffffffu;
-        if (p < pe) {
-            const double4 x = tv.pm[p];
-            const double dd = dist2(qx, qy, qz, x.x, x.y, x.z);
-            if (dd <= T && best.accepts(dd, p)) {
-                d2 = dd;
-                pi = p;
-            }
-            if (++p >= pe) {
-                while (p >= pe && e < nent) {
-                    const ListEnt le = lst[e++];
-                    if ((le.mask >> lane) & 1ull) {
-                        p = le.begin;
-                        pe = le.end;
-                    }
-                }
-            }
-        }
-        if (__ballot(pi != 0xffffffffu)) best.insert(d2, pi);
-    }
-}
-
-/* Pass 2: the K smallest (d^2, index) of S(r*) per query, written as
- * lists in [j][slot] layout (coalesced across lanes).  The walk is lock-step
- * over the wave and records the nodes some lane takes whole; each lane then
- * streams only its own entries' particles.  RESUME = false: one walk whose
- * list must fit LCAP entries (a wave that overflows sets its flags and is
- * redone with RESUME = true, which alternates walking and streaming).
- * Slots: q (or qslot[q] when given). */
-template <int K, bool RESUME>
-__global__ __launch_bounds__(64) void k_knn_gather(TreeView tv, uint32_t nq, const uint32_t *__restrict__ qidx,
-                                                   const double4 *__restrict__ qpts, const double *__restrict__ r_in,
-                                                   const uint32_t *__restrict__ c_in, const double *__restrict__ rt_in,
-                                                   double *__restrict__ ld2, uint32_t *__restrict__ lidx, uint32_t q0,
-                                                   uint32_t nslots, const uint32_t *__restrict__ qslot,
-                                                   uint32_t *__restrict__ ovf)
-{
-    __shared__ NodeCache nc;
-    __shared__ ListEnt lst[LCAP];
-    const int lane = threadIdx.x;
-    const uint32_t q = blockIdx.x * 64 + lane;
-    const bool valid = q < nq;
-    double qx = 0, qy = 0, qz = 0, r = 0;
-    uint32_t self = 0;
-    const uint32_t slot = valid ? (qslot ? qslot[q] : q) : 0u;
-    if (valid) query_setup<K>(tv, q, qidx, qpts, r_in, qx, qy, qz, r, self, q0);
-    /* Candidates in nodes the sphere contains lie within r* (a contained
-     * node's box is inside the sphere with a margin far above the Morton
-     * quantisation); when there are >= K of them the K nearest are within
-     * r*, and only d^2 <= (r* + 1e-9)^2 can enter the list. */
-    double T = __longlong_as_double(0x7ff0000000000000ll);
-    if (valid && c_in[q] >= (uint32_t)K) {
-        const double rr = __dadd_rn(r, 1e-9);
-        T = __dmul_rn(rr, rr);
-    }
-    /* a point query with a tight radius (k_point_tight): >= K particles lie
-     * within r_t, and the K nearest of S(est) are the K nearest within r_t */
-    const bool geo = valid && rt_in && rt_in[q] > 0;
-    if (geo) {
-        r = rt_in[q];
-        T = r * r;
-    }
-    WalkState w{0u, valid ? 0u : 0xffffffffu, 1u, 0u};
-    RegList<K> best;
-    if (!RESUME) {
-        uint32_t nent = 0;
-        const bool fin = walk_fill(tv, nc, w, qx, qy, qz, r, geo, lst, nent);
-        if (!fin) {
-            if (valid) ovf[q] = 1u;
-            return;
-        }
-        __syncthreads();
-        best.init();
-        stream_entries<K>(tv, lst, nent, qx, qy, qz, T, best);
-    } else {
-        best.init();
-        bool fin = false;
-        while (!fin) {
-            uint32_t nent = 0;
-            fin = walk_fill(tv, nc, w, qx, qy, qz, r, geo, lst, nent);
-            __syncthreads();
-            stream_entries<K>(tv, lst, nent, qx, qy, qz, T, best);
-            __syncthreads();
-        }
-    }
-    if (!valid) return;
-#pragma unroll
-    for (int j = 0; j < K; ++j) {
-        ld2[(size_t)j * nslots + slot] = best.d[j];
-        lidx[(size_t)j * nslots + slot] = best.i[j];
-    }
-}
-
 __global__ void k_ovf_compact(const uint32_t *__restrict__ ovf, const uint32_t *__restrict__ pos, uint32_t n,
                               uint32_t *__restrict__ list)
 {
@@ -764,80 +426,214 @@ __global__ void k_ovf_compact(const uint32_t *__restrict__ ovf, const uint32_t *
     if (i < n && ovf[i]) list[pos[i]] = i;
 }
 
-/* the overflow redo: queries named by slot, with the batch's query arrays */
-__global__ void k_ovf_gather_args(const uint32_t *__restrict__ slots, uint32_t m, const uint32_t *__restrict__ qidx,
-                                  const double4 *__restrict__ qpts, const double *__restrict__ r_in,
-                                  const uint32_t *__restrict__ c_in, const double *__restrict__ rt_in, uint32_t q0,
-                                  uint32_t *__restrict__ qi2, double4 *__restrict__ qp2, double *__restrict__ r2,
-                                  uint32_t *__restrict__ c2, double *__restrict__ rt2)
+/* ---------------------------------------------- wave-per-query selection */
+
+__device__ __forceinline__ bool key_less(double a, uint32_t ai, double b, uint32_t bi)
 {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= m) return;
-    const uint32_t q = slots[k];
-    if (qpts) qp2[k] = qpts[q];
-    else qi2[k] = qidx ? qidx[q] : q0 + q;
-    r2[k] = r_in[q];
-    rt2[k] = rt_in ? rt_in[q] : 0.0;
-    c2[k] = c_in[q];
+    return a < b || (a == b && ai < bi);
 }
 
-/* Uses of the lists, one thread per query: 0 density, 1 densest neighbour,
- * 2 row-major copy of the lists. */
-template <int K>
-__global__ __launch_bounds__(256) void k_knn_finish(TreeView tv, uint32_t nq, const uint32_t *__restrict__ qidx,
-                                                    const double4 *__restrict__ qpts,
-                                                    const double *__restrict__ ld2, const uint32_t *__restrict__ lidx,
-                                                    int mode, int kernel, const double *__restrict__ rho_in,
-                                                    double *__restrict__ rho_out, uint32_t *__restrict__ densest,
-                                                    uint32_t *__restrict__ idx_out, double *__restrict__ d2_out,
-                                                    uint32_t q0)
+/* value of a wave-uniform lane j */
+__device__ __forceinline__ double lane_f64(double v, int j)
 {
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nq) return;
-    const uint32_t self = qpts ? 0u : (qidx ? qidx[q] : q0 + q);
-    if (mode == 1) {
-        double bv = rho_in[self];
-        uint32_t bi = self;
-        for (int j = 0; j < K; ++j) {
-            const uint32_t id = lidx[(size_t)j * nq + q];
-            const double rj = rho_in[id];
-            if (bv < rj) {
-                bi = id;
-                bv = rj;
-            } else if (bv == rj && bi > id) {
-                bi = id;
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)b, j);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(b >> 32), j);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+__device__ __forceinline__ uint32_t lane_u32(uint32_t v, int j) { return __builtin_amdgcn_readlane(v, j); }
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+/* Bitonic sort of one (d^2, index) key per lane over the 64 lanes,
+ * ascending in lane order. */
+__device__ __forceinline__ void sort64(double &d, uint32_t &i)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const double pd = __shfl_xor(d, j, 64);
+            const uint32_t pi = __shfl_xor(i, j, 64);
+            const bool asc = (lane & k) == 0;
+            const bool lower = (lane & j) == 0;
+            const bool sw = (lower == asc) ? key_less(pd, pi, d, i) : key_less(d, i, pd, pi);
+            d = sw ? pd : d;
+            i = sw ? pi : i;
+        }
+    }
+}
+
+/* (bd, bi) and (cd, ci) sorted ascending over the lanes: (bd, bi) becomes
+ * the 64 smallest keys of both, sorted (the lane-wise minimum against the
+ * reversed second list is bitonic; six half-cleaners sort it). */
+__device__ __forceinline__ void merge64(double &bd, uint32_t &bi, double cd, uint32_t ci)
+{
+    const int lane = threadIdx.x & 63;
+    const double rd = __shfl(cd, 63 - lane, 64);
+    const uint32_t ri = __shfl(ci, 63 - lane, 64);
+    if (key_less(rd, ri, bd, bi)) {
+        bd = rd;
+        bi = ri;
+    }
+#pragma unroll
+    for (int j = 32; j > 0; j >>= 1) {
+        const double pd = __shfl_xor(bd, j, 64);
+        const uint32_t pi = __shfl_xor(bi, j, 64);
+        const bool lower = (lane & j) == 0;
+        const bool sw = lower ? key_less(pd, pi, bd, bi) : key_less(bd, bi, pd, pi);
+        bd = sw ? pd : bd;
+        bi = sw ? pi : bi;
+    }
+}
+
+/* LDS of one select wave: node cache, the include list of the walk and the
+ * compacted entries of the query being selected */
+struct SelectLds {
+    NodeCache nc;
+    ListEnt lst[LCAP];
+    uint32_t sbeg[LCAP];
+    uint32_t spref[LCAP + 1];
+};
+
+/* Feed the candidates of query bit j of the recorded list to the running
+ * best-64 (bd, bi): the entries of the query are compacted into sbeg/spref,
+ * then candidates are taken 64 at a time (lane c of a chunk locates its
+ * particle by binary search over the prefix), measured, filtered by T,
+ * sorted and merged. */
+__device__ __forceinline__ void select_list(const TreeView &tv, SelectLds &L, uint32_t nent, int j, double qx,
+                                            double qy, double qz, double T, double &bd, uint32_t &bi, bool &first)
+{
+    const int lane = threadIdx.x & 63;
+    uint32_t ne = 0, np = 0;
+    for (uint32_t e = lane; e < nent; e += 64) {
+        const ListEnt le = L.lst[e];
+        if ((le.mask >> j) & 1ull) {
+            ++ne;
+            np += le.end - le.begin;
+        }
+    }
+    const uint32_t ne_in = wave_incl_scan(ne), np_in = wave_incl_scan(np);
+    const uint32_t M = lane_u32(ne_in, 63), C = lane_u32(np_in, 63);
+    __syncthreads();
+    {
+        uint32_t slot = ne_in - ne, pref = np_in - np;
+        for (uint32_t e = lane; e < nent; e += 64) {
+            const ListEnt le = L.lst[e];
+            if ((le.mask >> j) & 1ull) {
+                L.sbeg[slot] = le.begin;
+                L.spref[slot] = pref;
+                ++slot;
+                pref += le.end - le.begin;
             }
         }
-        densest[q] = bi;
+        if (lane == 0) L.spref[M] = C;
+    }
+    __syncthreads();
+    const double inf = __longlong_as_double(0x7ff0000000000000ll);
+    for (uint32_t base = 0; base < C; base += 64) {
+        const uint32_t c = base + lane;
+        double d = inf;
+        uint32_t id = PLAN_NONE;
+        if (c < C) {
+            uint32_t lo = 0, hi = M; /* spref[lo] <= c < spref[hi] */
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (L.spref[mid] <= c) lo = mid;
+                else hi = mid;
+            }
+            const uint32_t p = L.sbeg[lo] + (c - L.spref[lo]);
+            const double4 x = tv.pm[p];
+            const double dd = dist2(qx, qy, qz, x.x, x.y, x.z);
+            if (dd <= T) {
+                d = dd;
+                id = p;
+            }
+        }
+        if (!__ballot(id != PLAN_NONE)) continue;
+        sort64(d, id);
+        if (first) {
+            bd = d;
+            bi = id;
+            first = false;
+        } else {
+            merge64(bd, bi, d, id);
+        }
+    }
+}
+
+/* What a query's sorted neighbour list (lane l holds the l-th nearest, l < K)
+ * is used for: the densest neighbour (HOP), the density (median spherical or
+ * QS kernel) and/or the lists themselves.  Sums run in the reference's
+ * order (src/tree.hpp:3320-3420). */
+template <int K>
+__device__ __forceinline__ void finish_query(const TreeView &tv, double bd, uint32_t bi, uint32_t q, uint32_t self,
+                                             bool byq, int mode, int kernel, const double *__restrict__ rho_in,
+                                             double *__restrict__ rho_out, uint32_t *__restrict__ densest,
+                                             uint32_t *__restrict__ idx_out, double *__restrict__ d2_out)
+{
+    const int lane = threadIdx.x & 63;
+    const bool mine = lane < K;
+    if (mode == 1) {
+        /* maximum of (rho, -index) over self and the neighbours */
+        double v = -__longlong_as_double(0x7ff0000000000000ll);
+        uint32_t id = PLAN_NONE;
+        if (mine && bi != PLAN_NONE) {
+            v = rho_in[bi];
+            id = bi;
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const double pv = __shfl_xor(v, o, 64);
+            const uint32_t pid = __shfl_xor(id, o, 64);
+            if (pv > v || (pv == v && pid < id)) {
+                v = pv;
+                id = pid;
+            }
+        }
+        if (lane == 0) {
+            const double sv = rho_in[self];
+            uint32_t best = self;
+            if (sv < v || (sv == v && id < self)) best = id;
+            densest[q] = best;
+        }
         return;
     }
     if (mode == 2 || idx_out) {
-        for (int j = 0; j < K; ++j) {
-            if (idx_out) idx_out[(size_t)q * K + j] = lidx[(size_t)j * nq + q];
-            if (d2_out) d2_out[(size_t)q * K + j] = ld2[(size_t)j * nq + q];
+        if (mine) {
+            if (idx_out) idx_out[(size_t)q * K + lane] = bi;
+            if (d2_out) d2_out[(size_t)q * K + lane] = bd;
         }
         if (mode == 2) return;
     }
     double rho = 0;
     if (kernel == 0) {
-        /* MedianSphericalDensityKernel (ascending list): m = 4 d2[K/2];
-         * i* = last index with d2 <= m; masses summed from i* down to 0 */
-        const double m = __dmul_rn(4.0, ld2[(size_t)(K / 2) * nq + q]);
-        int is = K / 2;
-        for (int j = K - 1; j > K / 2; --j)
-            if (!(ld2[(size_t)j * nq + q] > m)) {
-                is = j;
-                break;
-            }
-        const double tm = ld2[(size_t)is * nq + q];
-        for (int j = is; j >= 0; --j) rho = __dadd_rn(rho, tv.pm[lidx[(size_t)j * nq + q]].w);
+        /* MedianSphericalDensityKernel: m = 4 d2[K/2]; i* = last index with
+         * d2 <= m; masses summed from i* down to 0 */
+        const double m = __dmul_rn(4.0, lane_f64(bd, K / 2));
+        const uint64_t in = __ballot(lane > K / 2 && lane < K && !(bd > m));
+        const int is = in ? 63 - __clzll((long long)in) : K / 2;
+        const double tm = lane_f64(bd, is);
+        const double w = (lane <= is) ? tv.pm[bi].w : 0.0;
+        for (int j = is; j >= 0; --j) rho = __dadd_rn(rho, lane_f64(w, j));
         const double four_pi_over_three = (3.14159265358979323846 * 4.) / 3.;
         rho = __ddiv_rn(rho, __dmul_rn(four_pi_over_three, pow15(tm)));
     } else {
         /* QseudoQuadraticSplinesKernel; point queries use particle 0 as "self" */
-        const double4 sp = tv.pm[self];
-        for (int j = 0; j < K; ++j) {
-            const double4 x = tv.pm[lidx[(size_t)j * nq + q]];
+        double term = 0;
+        if (mine) {
+            const double4 sp = tv.pm[self];
+            const double4 x = tv.pm[bi];
             double dd[3] = {fabs(__dsub_rn(x.x, sp.x)), fabs(__dsub_rn(x.y, sp.y)), fabs(__dsub_rn(x.z, sp.z))};
             double w = 1;
             for (int k = 0; k < 3; ++k) {
@@ -851,23 +647,119 @@ __global__ __launch_bounds__(256) void k_knn_finish(TreeView tv, uint32_t nq, co
                     break;
                 }
             }
-            rho = __dadd_rn(rho, __dmul_rn(__ddiv_rn(x.w, tv.cell_volume), w));
+            term = __dmul_rn(__ddiv_rn(x.w, tv.cell_volume), w);
+        }
+        for (int j = 0; j < K; ++j) rho = __dadd_rn(rho, lane_f64(term, j));
+    }
+    if (lane == 0) rho_out[byq ? q : self] = rho;
+}
+
+/* The K nearest of S(r*) per query, and their use, in one kernel.
+ * A wave walks the tree once in lock step for its 64 queries (lane =
+ * query), recording the nodes some lane takes whole; then it selects for one
+ * query at a time, all 64 lanes on that query's candidates.
+ * RESUME = false: a wave whose list overflows LCAP entries flags itself
+ * (ovf[wave]) and stops.  RESUME = true: the flagged waves (waves[block])
+ * walk in several list fills; each query's K best so far are kept in HBM
+ * (sd2/sidx, K per query) between fills. */
+template <int K, bool RESUME>
+__global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, const uint32_t *__restrict__ qidx,
+                                                   const double4 *__restrict__ qpts, const double *__restrict__ r_in,
+                                                   const uint32_t *__restrict__ c_in, const double *__restrict__ rt_in,
+                                                   uint32_t q0, const uint32_t *__restrict__ waves,
+                                                   uint32_t *__restrict__ ovf, double *__restrict__ sd2,
+                                                   uint32_t *__restrict__ sidx, int mode, int kernel,
+                                                   const double *__restrict__ rho_in, double *__restrict__ rho_out,
+                                                   uint32_t *__restrict__ densest, uint32_t *__restrict__ idx_out,
+                                                   double *__restrict__ d2_out)
+{
+    __shared__ SelectLds L;
+    const int lane = threadIdx.x;
+    const uint32_t wv = RESUME ? waves[blockIdx.x] : blockIdx.x;
+    const uint32_t q = wv * 64 + lane;
+    const bool valid = q < nq;
+    double qx = 0, qy = 0, qz = 0, r = 0;
+    uint32_t self = 0;
+    if (valid) query_setup<K>(tv, q, qidx, qpts, r_in, qx, qy, qz, r, self, q0);
+    /* Candidates in nodes the sphere contains lie within r* (a contained
+     * node's box is inside the sphere with a margin far above the Morton
+     * quantisation); when there are >= K of them the K nearest are within
+     * r*, and only d^2 <= (r* + 1e-9)^2 can enter the list.  A point query
+     * with a tight radius (k_point_tight) has >= K particles within r_t and
+     * takes the K nearest within r_t. */
+    const double inf = __longlong_as_double(0x7ff0000000000000ll);
+    double T = inf;
+    if (valid && c_in[q] >= (uint32_t)K) {
+        const double rr = __dadd_rn(r, 1e-9);
+        T = __dmul_rn(rr, rr);
+    }
+    const bool geo = valid && rt_in && rt_in[q] > 0;
+    if (geo) {
+        r = rt_in[q];
+        T = r * r;
+    }
+    WalkState w{0u, valid ? 0u : 0xffffffffu, 1u, 0u};
+    const uint32_t nvalid = (uint32_t)__popcll(__ballot(valid));
+    if (!RESUME) {
+        uint32_t nent = 0;
+        const bool fin = walk_fill(tv, L.nc, w, qx, qy, qz, r, geo, L.lst, nent);
+        if (!fin) {
+            if (lane == 0) ovf[wv] = 1u;
+            return;
+        }
+        __syncthreads();
+        for (uint32_t j = 0; j < nvalid; ++j) {
+            double bd = inf;
+            uint32_t bi = PLAN_NONE;
+            bool first = true;
+            select_list(tv, L, nent, (int)j, lane_f64(qx, j), lane_f64(qy, j), lane_f64(qz, j), lane_f64(T, j), bd,
+                        bi, first);
+            finish_query<K>(tv, bd, bi, wv * 64 + j, lane_u32(self, j), qpts || qidx, mode, kernel, rho_in,
+                            rho_out, densest, idx_out, d2_out);
+        }
+    } else {
+        double *bd2 = sd2 + (size_t)blockIdx.x * 64 * K;
+        uint32_t *bix = sidx + (size_t)blockIdx.x * 64 * K;
+        if (lane < K)
+            for (uint32_t j = 0; j < nvalid; ++j) {
+                bd2[j * K + lane] = inf;
+                bix[j * K + lane] = PLAN_NONE;
+            }
+        bool fin = false;
+        while (!fin) {
+            uint32_t nent = 0;
+            fin = walk_fill(tv, L.nc, w, qx, qy, qz, r, geo, L.lst, nent);
+            __syncthreads();
+            for (uint32_t j = 0; j < nvalid; ++j) {
+                double bd = lane < K ? bd2[j * K + lane] : inf;
+                uint32_t bi = lane < K ? bix[j * K + lane] : PLAN_NONE;
+                bool first = false;
+                select_list(tv, L, nent, (int)j, lane_f64(qx, j), lane_f64(qy, j), lane_f64(qz, j), lane_f64(T, j),
+                            bd, bi, first);
+                if (lane < K) {
+                    bd2[j * K + lane] = bd;
+                    bix[j * K + lane] = bi;
+                }
+            }
+            __syncthreads();
+        }
+        for (uint32_t j = 0; j < nvalid; ++j) {
+            const double bd = lane < K ? bd2[j * K + lane] : inf;
+            const uint32_t bi = lane < K ? bix[j * K + lane] : PLAN_NONE;
+            finish_query<K>(tv, bd, bi, wv * 64 + j, lane_u32(self, j), qpts || qidx, mode, kernel, rho_in,
+                            rho_out, densest, idx_out, d2_out);
         }
     }
-    rho_out[qpts || qidx ? q : self] = rho;
 }
 
 template <int K>
 static void launch_knn(const DevTree &t, const TreeView &tv, uint32_t nq, const uint32_t *qidx, const double4 *qpts,
                        const double *est, int mode, int kernel, const double *rho_in, KnnOut out, hipStream_t s)
 {
-    /* the neighbour lists of one batch live in HBM: 12 K bytes per query */
     const uint32_t batch = (uint32_t)std::min<uint64_t>(nq, (uint64_t)(1u << 24));
     Arena &a = const_cast<DevTree &>(t).arena;
     double *rbuf = a.get_t<double>("knn_r", (size_t)nq);
     uint32_t *cbuf = a.get_t<uint32_t>("knn_c", (size_t)nq);
-    double *ld2 = a.get_t<double>("knn_ld2", (size_t)batch * K);
-    uint32_t *lidx = a.get_t<uint32_t>("knn_lidx", (size_t)batch * K);
     double *rt = nullptr;
     if (out.ev) HIPCHK(hipEventRecord(out.ev[0], s));
     if (qpts) {
@@ -878,47 +770,43 @@ static void launch_knn(const DevTree &t, const TreeView &tv, uint32_t nq, const 
     hipLaunchKernelGGL(k_knn_radius<K>, dim3(ceil_div(nq, 64)), dim3(64), 0, s, tv, nq, qidx, qpts, est, rt, rbuf,
                        cbuf);
     LAUNCHCHK();
-    if (out.ev) HIPCHK(hipEventRecord(out.ev[1], s));
     for (uint32_t b0 = 0; b0 < nq; b0 += batch) {
         const uint32_t nb = std::min(batch, nq - b0);
         const uint32_t *qi = qidx ? qidx + b0 : nullptr;
         const double4 *qp = qpts ? qpts + b0 : nullptr;
         const uint32_t q0 = (qidx || qpts) ? 0u : b0; /* all-particle pass: particles b0.. */
         const size_t o = (qidx || qpts) ? b0 : 0;     /* outputs indexed by query, else by particle */
-        uint32_t *ovf = a.get_t<uint32_t>("knn_ovf", (size_t)nb + 1);
-        uint32_t *opos = a.get_t<uint32_t>("knn_opos", (size_t)nb + 1);
+        double *rho_o = out.rho ? out.rho + o : nullptr;
+        uint32_t *dn_o = out.densest ? out.densest + b0 : nullptr;
+        uint32_t *idx_o = out.idx ? out.idx + (size_t)b0 * K : nullptr;
+        double *d2_o = out.d2 ? out.d2 + (size_t)b0 * K : nullptr;
+        const double *rt_b = rt ? rt + b0 : nullptr;
+        const uint32_t nw = ceil_div(nb, 64);
+        uint32_t *ovf = a.get_t<uint32_t>("knn_ovf", (size_t)nw + 1);
+        uint32_t *opos = a.get_t<uint32_t>("knn_opos", (size_t)nw + 1);
         uint32_t *otot = a.get_t<uint32_t>("knn_otot", 4);
-        HIPCHK(hipMemsetAsync(ovf, 0, (size_t)nb * 4, s));
-        hipLaunchKernelGGL((k_knn_gather<K, false>), dim3(ceil_div(nb, 64)), dim3(64), 0, s, tv, nb, qi, qp,
-                           rbuf + b0, cbuf + b0, rt ? rt + b0 : nullptr, ld2, lidx, q0, nb, nullptr, ovf);
+        HIPCHK(hipMemsetAsync(ovf, 0, (size_t)nw * 4, s));
+        if (out.ev && b0 == 0) HIPCHK(hipEventRecord(out.ev[1], s));
+        hipLaunchKernelGGL((k_knn_select<K, false>), dim3(nw), dim3(64), 0, s, tv, nb, qi, qp, rbuf + b0, cbuf + b0,
+                           rt_b, q0, nullptr, ovf, nullptr, nullptr, mode, kernel, rho_in, rho_o, dn_o, idx_o, d2_o);
         LAUNCHCHK();
-        scan_exclusive_u32(a, ovf, opos, nb, s, otot);
+        if (out.ev && b0 == 0) HIPCHK(hipEventRecord(out.ev[2], s));
+        scan_exclusive_u32(a, ovf, opos, nw, s, otot);
         uint32_t novf = 0;
         HIPCHK(hipMemcpyAsync(&novf, otot, 4, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         if (novf) {
-            uint32_t *slots = a.get_t<uint32_t>("knn_oslot", novf);
-            uint32_t *qi2 = a.get_t<uint32_t>("knn_oqi", novf);
-            double4 *qp2 = a.get_t<double4>("knn_oqp", novf);
-            double *r2 = a.get_t<double>("knn_or", novf);
-            uint32_t *c2 = a.get_t<uint32_t>("knn_oc", novf);
-            double *rt2 = a.get_t<double>("knn_ort", novf);
-            hipLaunchKernelGGL(k_ovf_compact, dim3(ceil_div(nb, 256)), dim3(256), 0, s, ovf, opos, nb, slots);
+            uint32_t *waves = a.get_t<uint32_t>("knn_owaves", novf);
+            double *sd2 = a.get_t<double>("knn_osd2", (size_t)novf * 64 * K);
+            uint32_t *sidx = a.get_t<uint32_t>("knn_osidx", (size_t)novf * 64 * K);
+            hipLaunchKernelGGL(k_ovf_compact, dim3(ceil_div(nw, 256)), dim3(256), 0, s, ovf, opos, nw, waves);
             LAUNCHCHK();
-            hipLaunchKernelGGL(k_ovf_gather_args, dim3(ceil_div(novf, 256)), dim3(256), 0, s, slots, novf, qi, qp,
-                               rbuf + b0, cbuf + b0, rt ? rt + b0 : nullptr, q0, qi2, qp2, r2, c2, rt2);
-            LAUNCHCHK();
-            hipLaunchKernelGGL((k_knn_gather<K, true>), dim3(ceil_div(novf, 64)), dim3(64), 0, s, tv, novf,
-                               qp ? nullptr : qi2, qp ? qp2 : nullptr, r2, c2, rt ? rt2 : nullptr, ld2, lidx, 0u,
-                               nb, slots, ovf);
+            hipLaunchKernelGGL((k_knn_select<K, true>), dim3(novf), dim3(64), 0, s, tv, nb, qi, qp, rbuf + b0,
+                               cbuf + b0, rt_b, q0, waves, ovf, sd2, sidx, mode, kernel, rho_in, rho_o, dn_o, idx_o,
+                               d2_o);
             LAUNCHCHK();
         }
-        if (out.ev && b0 == 0) HIPCHK(hipEventRecord(out.ev[2], s));
-        hipLaunchKernelGGL(k_knn_finish<K>, dim3(ceil_div(nb, 256)), dim3(256), 0, s, tv, nb, qi, qp, ld2, lidx,
-                           mode, kernel, rho_in, out.rho ? out.rho + o : nullptr,
-                           out.densest ? out.densest + b0 : nullptr, out.idx ? out.idx + (size_t)b0 * K : nullptr,
-                           out.d2 ? out.d2 + (size_t)b0 * K : nullptr, q0);
-        LAUNCHCHK();
+        if (out.novf) *out.novf += novf;
     }
     if (out.rstop) HIPCHK(hipMemcpyAsync(out.rstop, rbuf, (size_t)nq * 8, hipMemcpyDeviceToDevice, s));
 }

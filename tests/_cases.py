@@ -17,4 +17,6 @@ CASES = {
     "n32k_k16": lambda: dense_case(32768, 10, 5, a=4e-4, knn=16),
     "n32k_k64_h64": lambda: dense_case(32768, 10, 6, a=4e-4, knn=64, hop=64),
     "n16k_qs": lambda: dense_case(16384, 6, 7, a=3e-4, nx=128),
+    # the benchmark's mix (200 Plummer clumps in 10% of the particles) at 2^20
+    "c2_1m": lambda: synth.config_c2(seed=3, n=1 << 20),
 }
