@@ -35,10 +35,33 @@ PEAK_HBM_GBS = 8000.0
 # algorithmic bytes per tree particle for the density pass: binary64 position
 # (24) + mass (8) read, density (8) written
 DENSITY_BYTES_PER_PARTICLE = 40
+TRAFFIC_SUMMARY = "knn_select_density_c2_16m_pmc.md"
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def measured_traffic(launch_units: int):
+    """HBM bytes per density launch from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE runs, each divided by the factor the
+    same counter reads for a device copy of known size), or None when the
+    summary is absent or was taken at another launch size."""
+    path = os.path.join(ROOT, "profiles", TRAFFIC_SUMMARY)
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    with open(path) as f:
+        for line in f:
+            if ":" in line and line.startswith("- "):
+                k, v = line[2:].split(":", 1)
+                vals[k.strip()] = v.strip().split()[0]
+    try:
+        if int(vals["launch_units"]) != launch_units:
+            return None
+        return float(vals["hbm_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
 
 
 def cpu_baseline(threads: int, n_sample: int) -> dict:
@@ -115,7 +138,10 @@ def main():
     value = args.n / (total / args.steps)
     k_ms = float(np.mean(kms))
     nparticles_tree = info.num_particles
-    achieved = DENSITY_BYTES_PER_PARTICLE * nparticles_tree / (k_ms * 1e-3) / 1e9
+    # the timed launch is the first density launch: min(2^24, tree particles) queries
+    launch_units = min(1 << 24, nparticles_tree)
+    achieved = DENSITY_BYTES_PER_PARTICLE * launch_units / (k_ms * 1e-3) / 1e9
+    traffic = measured_traffic(launch_units)
     mean_st = {k: float(np.mean([s[k] for s in stages])) for k in stages[0]}
     cpu = None
     if not args.no_cpu_baseline:
@@ -143,13 +169,15 @@ def main():
             "stage_ms": mean_st, "h2d_upload_s": upload_s,
         },
         "roofline": {
-            "kernel": "k_knn_gather<32> (density pass: K-nearest selection + median density)",
+            "kernel": "k_knn_select<32,false> (density pass: walk, K-nearest selection, median density)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
             "frac": achieved / PEAK_HBM_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "units_per_launch": launch_units,
+            "bytes_per_unit": DENSITY_BYTES_PER_PARTICLE,
             "kernel_ms": k_ms,
             "radius_kernel_ms": float(np.mean(rms)),
         },

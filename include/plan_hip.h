@@ -127,7 +127,7 @@ int plan_device_memory(plan_ctx *ctx, uint64_t *bytes);
 /* wall-clock milliseconds of the last pipeline stages: ghosts, build, density,
  * hop, clumps, total (hipEvent timing on the context stream; clumps = host). */
 int plan_stage_times(plan_ctx *ctx, double *ms, int n);
-/* counters of the last FindPlanetesimals, up to n of these 32 values:
+/* counters of the last FindPlanetesimals, up to n of these 35 values:
  *  0 particles above threshold, 1 HOP groups, 2 naive clumps, 3 clumps after
  *  merging, 4 merge passes, 5 pair candidates, 6 saddle queries, 7 rebinding
  *  ball searches, 8 member-table copy ms, 9 rebinding reruns,
@@ -138,7 +138,9 @@ int plan_stage_times(plan_ctx *ctx, double *ms, int n);
  *  24-26 merge ms in clump-tree searches, saddle densities, applying merges,
  *  27-29 rebinding ms in superset searches, parallel runs, in-order validation,
  *  30-31 waves of the density and densest-neighbour passes whose node list
- *  overflowed the on-chip capacity (walked in several fills) */
+ *  overflowed the on-chip capacity (walked in several fills),
+ *  32-34 the slowest clump's rebinding: exact device searches, their ms, ms
+ *  re-sorting members */
 int plan_clump_stats(plan_ctx *ctx, double *values, int n);
 /* device milliseconds of the last density pass's gather kernel (the K-nearest
  * selection and density, one launch) and of its radius-schedule kernel */

@@ -40,6 +40,7 @@ struct plan_ctx {
     ClumpStats cstats;
     uint32_t hop_na = 0, hop_groups = 0;
     double d2h_ms = 0;
+    HostPinned hpin;
     double density_kernel_ms = 0;
     double density_radius_ms = 0;
     uint32_t density_ovf_waves = 0;
@@ -578,12 +579,18 @@ int plan_find_planetesimals(plan_ctx *c, plan_catalog **out)
         hipStream_t s = c->stream;
         HostMembers hm;
         const uint32_t na = c->hop.na, nc = c->hop.nclumps;
-        hm.rec.resize(na);
-        hm.sinkcnt.resize(na);
+        {
+            /* pinned, grow-only staging for the member table */
+            const size_t rb = ((size_t)na * sizeof(MemberRec) + 255) & ~(size_t)255;
+            const size_t ub = ((size_t)na * 4 + 255) & ~(size_t)255;
+            char *base = (char *)c->hpin.get(rb + 3 * ub + 256);
+            hm.rec = HostSpan<MemberRec>{(MemberRec *)base, na};
+            hm.sinkcnt = HostSpan<uint32_t>{(uint32_t *)(base + rb), na};
+            hm.sorted_tidx = HostSpan<uint32_t>{(uint32_t *)(base + rb + ub), na};
+            hm.rec_of_sorted = HostSpan<uint32_t>{(uint32_t *)(base + rb + 2 * ub), na};
+        }
         hm.cstart.resize(nc + 1);
         hm.cpeak.resize(nc);
-        hm.sorted_tidx.resize(na);
-        hm.rec_of_sorted.resize(na);
         if (na) {
             uint32_t *roa = c->work.get_t<uint32_t>("rec_of_apos", na);
             uint32_t *cpk = c->work.get_t<uint32_t>("cpeak", nc + 1);
@@ -946,7 +953,7 @@ int plan_stage_times(plan_ctx *c, double *ms, int n)
 int plan_clump_stats(plan_ctx *c, double *v, int n)
 {
     if (!c || !v) return PLAN_E_ARG;
-    double a[32] = {(double)c->hop_na, (double)c->hop_groups, (double)c->cstats.naive, (double)c->cstats.after_merge,
+    double a[35] = {(double)c->hop_na, (double)c->hop_groups, (double)c->cstats.naive, (double)c->cstats.after_merge,
                     (double)c->cstats.merge_passes, (double)c->cstats.pair_candidates,
                     (double)c->cstats.saddle_queries, (double)c->cstats.rebind_searches, c->d2h_ms,
                     (double)c->cstats.rebind_reruns};
@@ -959,7 +966,10 @@ int plan_clump_stats(plan_ctx *c, double *v, int n)
     for (int k = 0; k < 3; ++k) a[27 + k] = c->cstats.rparts[k];
     a[30] = c->density_ovf_waves;
     a[31] = c->hop.knn_overflow_waves;
-    for (int i = 0; i < n && i < 32; ++i) v[i] = a[i];
+    a[32] = c->cstats.rebind_max_exact;
+    a[33] = c->cstats.rebind_max_exact_ms;
+    a[34] = c->cstats.rebind_max_sort_ms;
+    for (int i = 0; i < n && i < 35; ++i) v[i] = a[i];
     return PLAN_OK;
 }
 

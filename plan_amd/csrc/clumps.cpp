@@ -53,7 +53,7 @@ uint32_t HostMembers::rec_of_tidx(uint32_t tidx) const
 namespace {
 
 struct Ops {
-    const std::vector<MemberRec> &P;
+    const MemberRec *P;
     const ClumpSettings &cs;
 
     double dist_to(uint32_t h, const double *c) const
@@ -577,7 +577,7 @@ struct Rebind {
     std::vector<uint32_t> claimed;
     uint32_t exact_searches = 0;
     uint32_t iters = 0;
-    double ms = 0;
+    double ms = 0, ms_exact = 0, ms_sort = 0;
 };
 
 static void to_records(const HostMembers &hm, const std::vector<uint32_t> &tidx, std::vector<uint32_t> &out)
@@ -654,6 +654,7 @@ static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
             }
         }
         if (exact) {
+            auto te = std::chrono::steady_clock::now();
             std::vector<double4> qs{make_double4(c.com[0], c.com[1], c.com[2], r),
                                     make_double4(c.com[0], c.com[1], c.com[2], 2.0 * r)};
             std::vector<std::vector<uint32_t>> l;
@@ -661,6 +662,7 @@ static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
             rb.exact_searches++;
             to_records(hm, l[0], cand);
             to_records(hm, l[1], rb.sup);
+            rb.ms_exact += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - te).count();
             cpos.clear();
             for (int k = 0; k < 3; ++k) rb.c0[k] = c.com[k];
             rb.R = 2.0 * r;
@@ -706,8 +708,10 @@ static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
             lsd.resize(w);
         }
         if (c.indices.size() > c.particles.size()) {
+            auto ts = std::chrono::steady_clock::now();
             std::vector<PD>().swap(c.particles);
             ops.sort_particles(c, true);
+            rb.ms_sort += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ts).count();
         }
         ops.hill_radius(c);
     } while (c.indices.size() > before);
@@ -785,6 +789,9 @@ static void rebind_all(std::map<uint32_t, HClump> &cl, const HostMembers &hm, co
             st.rebind_max_ms = rb[i].ms;
             st.rebind_max_iters = rb[i].iters;
             st.rebind_max_n = (uint32_t)v[i]->indices.size();
+            st.rebind_max_exact = rb[i].exact_searches;
+            st.rebind_max_exact_ms = rb[i].ms_exact;
+            st.rebind_max_sort_ms = rb[i].ms_sort;
         }
     }
     part(2);
@@ -801,7 +808,7 @@ template <class F> static void for_each_clump_parallel(std::map<uint32_t, HClump
 void run_clump_pipeline(HostMembers &hm, const ClumpSettings &cs, ClumpServices &svc,
                         std::map<uint32_t, HClump> &cl, double &Mp_max, double &Mp_tot, ClumpStats &st)
 {
-    Ops ops{hm.rec, cs};
+    Ops ops{hm.rec.data(), cs};
     cl.clear();
     Mp_max = Mp_tot = 0;
     auto tl = std::chrono::steady_clock::now();

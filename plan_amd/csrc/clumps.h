@@ -58,13 +58,24 @@ struct HClump {
     double J[3] = {0, 0, 0}, Jq[3] = {0, 0, 0};
 };
 
+/* a view of n elements in memory owned elsewhere (pinned host staging) */
+template <class T> struct HostSpan {
+    T *p = nullptr;
+    size_t n = 0;
+    T &operator[](size_t i) const { return p[i]; }
+    size_t size() const { return n; }
+    T *data() const { return p; }
+    T *begin() const { return p; }
+    T *end() const { return p + n; }
+};
+
 struct HostMembers {
-    std::vector<MemberRec> rec;         /* grouped order */
+    HostSpan<MemberRec> rec;            /* grouped order */
     std::vector<uint32_t> cstart;       /* naive clump starts, plus na at the end */
     std::vector<uint32_t> cpeak;        /* naive clump peak (tree index), ascending */
-    std::vector<uint32_t> sorted_tidx;  /* tree indices ascending */
-    std::vector<uint32_t> rec_of_sorted;/* record of sorted_tidx[i] */
-    std::vector<uint32_t> sinkcnt;      /* per record: particles merged into it */
+    HostSpan<uint32_t> sorted_tidx;     /* tree indices ascending */
+    HostSpan<uint32_t> rec_of_sorted;   /* record of sorted_tidx[i] */
+    HostSpan<uint32_t> sinkcnt;         /* per record: particles merged into it */
     uint32_t rec_of_tidx(uint32_t tidx) const;
 };
 
@@ -76,8 +87,8 @@ struct ClumpStats {
     uint32_t rebind_searches = 0;
     uint32_t pair_candidates = 0;
     uint32_t rebind_reruns = 0;
-    uint32_t rebind_iters = 0, rebind_max_iters = 0, rebind_max_n = 0;
-    double rebind_max_ms = 0;
+    uint32_t rebind_iters = 0, rebind_max_iters = 0, rebind_max_n = 0, rebind_max_exact = 0;
+    double rebind_max_ms = 0, rebind_max_exact_ms = 0, rebind_max_sort_ms = 0;
     /* host milliseconds: sort, merge A, merge B, re-key, unbind, rebind,
      * filters, rip, J, ranking */
     double ms[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
