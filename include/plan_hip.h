@@ -147,6 +147,58 @@ int plan_clump_stats(plan_ctx *ctx, double *values, int n);
 int plan_last_density_kernel_ms(plan_ctx *ctx, double *ms);
 int plan_last_density_radius_ms(plan_ctx *ctx, double *ms);
 
+/* ---- pairwise-potential boundness (extension: no reference counterpart) ----
+ *
+ * The reference decides boundness by a two-body test against the rest of the
+ * clump (RemoveUnboundParticles, src/tree.hpp:4615-4674).  These entry points
+ * evaluate it with the full pairwise potential instead, per clump with bound
+ * set B (initially all members), iterating until no member leaves:
+ *   phi_i = -G sum_{j in B, x_j != x_i} m_j / |x_i - x_j|
+ *   dv_i = v_i - V + (0, q Omega (X_x - x_i,x), 0)   (V, X: mass-weighted over B)
+ *   e_i = |dv_i|^2 / 2 + phi_i, members with e_i > 0 leave B;
+ *   E_kin = sum_B m |dv|^2 / 2, E_pot = sum_B m phi / 2.
+ * Forms: PLAN_BOUND_FP64 (binary64 pair sums, one clump at a time) and
+ * PLAN_BOUND_FP32 (binary32 pair sums with compensated accumulation, all
+ * clumps per launch, iterations replayed from a HIP graph, binary64 totals,
+ * binary64 re-checks of members within 1e-5 and clumps within 1e-3 of zero
+ * energy). */
+enum { PLAN_BOUND_FP64 = 0, PLAN_BOUND_FP32 = 1 };
+
+typedef struct plan_boundness_in {
+    uint32_t num_clumps;
+    const uint32_t *offsets;   /* num_clumps + 1 member offsets */
+    const double *pos;         /* members x 3 */
+    const double *vel;         /* members x 3 */
+    const double *mass;        /* members */
+    double G, Omega, q;
+} plan_boundness_in;
+
+typedef struct plan_boundness_clump {
+    uint32_t n_in, n_bound, iterations, reserved;
+    double mass, center_of_mass[3], vel_com[3];
+    double E_kin, E_pot;
+} plan_boundness_clump;
+
+typedef struct plan_boundness_stats {
+    double ms;                 /* device time of the unbinding, inputs resident in HBM */
+    double pair_evaluations;   /* pair terms evaluated */
+    uint32_t refit_clumps;     /* clumps redone in binary64 (near-zero energy) */
+    uint32_t graph_replays;
+    int32_t tile_shape;        /* chosen pair-kernel shape (FP32 form), -1 otherwise */
+    int32_t reserved;
+    double tune_ms[4];         /* autotuner timing of each candidate shape */
+} plan_boundness_stats;
+
+/* stage a batch of clumps in HBM (replaces any previous batch) */
+int plan_boundness_load(plan_ctx *ctx, const plan_boundness_in *in);
+/* stage the clumps of a catalog from the last plan_find_planetesimals, with
+ * member positions, velocities and masses taken from the device tree */
+int plan_boundness_load_catalog(plan_ctx *ctx, const plan_catalog *cat);
+/* run one form on the staged batch; max_iter bounds the unbinding iterations */
+int plan_boundness_run(plan_ctx *ctx, int form, uint32_t max_iter, plan_boundness_stats *stats);
+/* results of the last run: bound flag per member (may be NULL), per clump */
+int plan_boundness_result(plan_ctx *ctx, uint8_t *bound, plan_boundness_clump *clumps);
+
 #ifdef __cplusplus
 }
 #endif

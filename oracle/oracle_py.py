@@ -64,6 +64,8 @@ def lib():
         L.orc_catalog.restype = C.POINTER(PlanCatalog)
         L.orc_catalog.argtypes = [vp]
         L.orc_catalog_free.argtypes = [C.POINTER(PlanCatalog)]
+        L.orc_boundness.argtypes = [C.c_uint32, _u32, _d, _d, _d, C.c_double, C.c_double, C.c_double, C.c_uint32,
+                                    _u8, _d]
         L.orc_unbind_once.restype = C.c_uint32
         L.orc_unbind_once.argtypes = [vp, C.c_uint32, _d, _d, _d, _u32, _d]
         _lib = L
@@ -216,3 +218,18 @@ def run_pipeline(snap, params=None, threads: int | None = None):
     t = OracleTree(params, snap.n, gp, gv, gd, gpr, gid)
     t.find_planetesimals()
     return t
+
+
+def boundness(offsets, pos, vel, mass, G, Omega=1.0, q=1.5, max_iter=64):
+    """FP64 brute-force pairwise boundness of clumps in CSR form; returns
+    (bound flags, per-clump dict arrays)."""
+    offsets = np.ascontiguousarray(offsets, np.uint32)
+    nc = len(offsets) - 1
+    pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+    vel = np.ascontiguousarray(vel, np.float64).reshape(-1, 3)
+    mass = np.ascontiguousarray(mass, np.float64)
+    bound = np.zeros(len(mass), np.uint8)
+    out = np.zeros((nc, 8))
+    lib().orc_boundness(nc, offsets, pos.ravel(), vel.ravel(), mass, G, Omega, q, max_iter, bound, out.ravel())
+    return bound.astype(bool), dict(n_bound=out[:, 0].astype(np.int64), iterations=out[:, 1].astype(np.int64),
+                                    E_kin=out[:, 2], E_pot=out[:, 3], mass=out[:, 4], com=out[:, 5:8])

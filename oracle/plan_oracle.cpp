@@ -1479,4 +1479,83 @@ uint32_t orc_unbind_once(const plan_params *pp, uint32_t n, const double *pos, c
     return (uint32_t)c.indices.size();
 }
 
+/* Pairwise-potential boundness: the north star's extension, which has no
+ * counterpart in the reference (its boundness is the two-body test of
+ * src/tree.hpp:4615-4674).  Brute force in binary64, potentials recomputed
+ * from scratch every iteration, in member order:
+ *   B = all members; repeat {
+ *     M, X, V over B;  phi_i = -G sum_{j in B, x_j != x_i} m_j / |x_i - x_j|;
+ *     dv_i = v_i - V + (0, q Omega (X_x - x_i,x), 0);  e_i = |dv_i|^2/2 + phi_i;
+ *     E_kin = sum m |dv|^2 / 2, E_pot = sum m phi / 2;
+ *     remove e_i > 0 } until nothing is removed (or max_iter iterations).
+ * Pinned by tests/test_oracle_boundness.py (two- and three-body closed forms,
+ * a numpy brute force, the Plummer potential energy -3 pi G M^2 / (32 a)).
+ * out per clump: {n_bound, iterations, E_kin, E_pot, M, X, Y, Z}. */
+void orc_boundness(uint32_t nc, const uint32_t *off, const double *pos, const double *vel, const double *mass,
+                   double G, double Omega, double q, uint32_t max_iter, uint8_t *bound, double *out)
+{
+    const double qO = q * Omega;
+#pragma omp parallel for num_threads(g_threads) schedule(dynamic, 1)
+    for (uint32_t c = 0; c < nc; ++c) {
+        std::vector<uint32_t> B;
+        for (uint32_t i = off[c]; i < off[c + 1]; ++i) B.push_back(i);
+        std::vector<double> phi;
+        double M = 0, X[3] = {0, 0, 0}, V[3] = {0, 0, 0}, ek = 0, ep = 0;
+        uint32_t it = 0;
+        for (;;) {
+            ++it;
+            M = 0;
+            double sx[3] = {0, 0, 0}, sv[3] = {0, 0, 0};
+            for (uint32_t i : B) {
+                M += mass[i];
+                for (int k = 0; k < 3; ++k) {
+                    sx[k] += mass[i] * pos[3 * (size_t)i + k];
+                    sv[k] += mass[i] * vel[3 * (size_t)i + k];
+                }
+            }
+            for (int k = 0; k < 3; ++k) {
+                X[k] = M > 0 ? sx[k] / M : 0.0;
+                V[k] = M > 0 ? sv[k] / M : 0.0;
+            }
+            phi.assign(B.size(), 0.0);
+            for (size_t a = 0; a < B.size(); ++a) {
+                const double *pa = pos + 3 * (size_t)B[a];
+                double sum = 0;
+                for (size_t b = 0; b < B.size(); ++b) {
+                    const double *pb = pos + 3 * (size_t)B[b];
+                    const double dx = pb[0] - pa[0], dy = pb[1] - pa[1], dz = pb[2] - pa[2];
+                    const double r2 = (dx * dx + dy * dy) + dz * dz;
+                    if (r2 > 0) sum += mass[B[b]] / std::sqrt(r2);
+                }
+                phi[a] = -G * sum;
+            }
+            ek = ep = 0;
+            std::vector<uint32_t> keep;
+            for (size_t a = 0; a < B.size(); ++a) {
+                const uint32_t i = B[a];
+                const double *p = pos + 3 * (size_t)i, *v = vel + 3 * (size_t)i;
+                const double dvx = v[0] - V[0], dvy = (v[1] - V[1]) + qO * (X[0] - p[0]), dvz = v[2] - V[2];
+                const double kin = 0.5 * ((dvx * dvx + dvy * dvy) + dvz * dvz);
+                ek += mass[i] * kin;
+                ep += 0.5 * mass[i] * phi[a];
+                if (!(kin + phi[a] > 0)) keep.push_back(i);
+            }
+            const bool changed = keep.size() != B.size();
+            if (!changed || it >= max_iter) break;
+            B.swap(keep);
+        }
+        for (uint32_t i = off[c]; i < off[c + 1]; ++i) bound[i] = 0;
+        for (uint32_t i : B) bound[i] = 1;
+        double *o = out + 8 * (size_t)c;
+        o[0] = (double)B.size();
+        o[1] = it;
+        o[2] = ek;
+        o[3] = ep;
+        o[4] = M;
+        o[5] = X[0];
+        o[6] = X[1];
+        o[7] = X[2];
+    }
+}
+
 } /* extern "C" */

@@ -20,7 +20,8 @@ import os
 
 import numpy as np
 
-from .abi import PlanCatalog, PlanParams, catalog_to_py, make_params
+from .abi import (PlanBoundnessClump, PlanBoundnessIn, PlanBoundnessStats, PlanCatalog, PlanParams, catalog_to_py,
+                  make_params)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libplan_hip.so")
@@ -48,7 +49,8 @@ EXPORTS = [
     "plan_catalog_free", "plan_get_tree_particles", "plan_get_keys", "plan_get_nodes", "plan_get_hop",
     "plan_knn_query", "plan_density_at", "plan_ball_query", "plan_stage_times", "plan_last_density_kernel_ms",
     "plan_keep_input", "plan_restore_input", "plan_device_memory", "plan_clump_stats",
-    "plan_last_density_radius_ms",
+    "plan_last_density_radius_ms", "plan_boundness_load", "plan_boundness_load_catalog", "plan_boundness_run",
+    "plan_boundness_result",
 ]
 
 
@@ -90,6 +92,10 @@ def lib():
     L.plan_device_memory.argtypes = [vp, C.POINTER(C.c_uint64)]
     L.plan_clump_stats.argtypes = [vp, _d, C.c_int]
     L.plan_last_density_radius_ms.argtypes = [vp, C.POINTER(C.c_double)]
+    L.plan_boundness_load.argtypes = [vp, C.POINTER(PlanBoundnessIn)]
+    L.plan_boundness_load_catalog.argtypes = [vp, C.POINTER(PlanCatalog)]
+    L.plan_boundness_run.argtypes = [vp, C.c_int, C.c_uint32, C.POINTER(PlanBoundnessStats)]
+    L.plan_boundness_result.argtypes = [vp, vp, C.POINTER(PlanBoundnessClump)]
     for f in EXPORTS:
         if f in ("plan_ctx_destroy", "plan_catalog_free", "plan_last_error"):
             continue
@@ -169,13 +175,56 @@ class Context:
         self._chk(lib().plan_hop(self._h, int(H), float(threshold), C.byref(a), C.byref(g)), "plan_hop")
         return a.value, g.value
 
-    def find_planetesimals(self) -> dict:
+    def find_planetesimals(self, keep_for_boundness: bool = False) -> dict:
+        """FindPlanetesimals; with keep_for_boundness the catalog's clumps are
+        also staged for the pairwise-boundness entry points."""
         out = C.POINTER(PlanCatalog)()
         self._chk(lib().plan_find_planetesimals(self._h, C.byref(out)), "plan_find_planetesimals")
         try:
+            if keep_for_boundness:
+                self._chk(lib().plan_boundness_load_catalog(self._h, out), "plan_boundness_load_catalog")
+                self._bnd_nc = out.contents.num_clumps
+                self._bnd_nm = out.contents.num_members_total
             return catalog_to_py(out.contents)
         finally:
             lib().plan_catalog_free(out)
+
+    # --- pairwise boundness (extension) -----------------------------------------
+    FP64, FP32 = 0, 1
+
+    def boundness_load(self, batch):
+        """Stage a synth.ClumpBatch (or any object with offsets, pos, vel,
+        mass, G, Omega, q) in device memory."""
+        self._bnd_keep = (np.ascontiguousarray(batch.offsets, np.uint32),
+                          np.ascontiguousarray(batch.pos, np.float64).reshape(-1, 3),
+                          np.ascontiguousarray(batch.vel, np.float64).reshape(-1, 3),
+                          np.ascontiguousarray(batch.mass, np.float64))
+        off, pos, vel, m = self._bnd_keep
+        b = PlanBoundnessIn(len(off) - 1, off.ctypes.data, pos.ctypes.data, vel.ctypes.data, m.ctypes.data,
+                            float(batch.G), float(batch.Omega), float(batch.q))
+        self._chk(lib().plan_boundness_load(self._h, C.byref(b)), "plan_boundness_load")
+        self._bnd_nc = len(off) - 1
+        self._bnd_nm = int(off[-1])
+        self._bnd_keep = None
+
+    def boundness_run(self, form: int, max_iter: int = 64) -> dict:
+        st = PlanBoundnessStats()
+        self._chk(lib().plan_boundness_run(self._h, int(form), int(max_iter), C.byref(st)), "plan_boundness_run")
+        return dict(ms=st.ms, pair_evaluations=st.pair_evaluations, refit_clumps=st.refit_clumps,
+                    graph_replays=st.graph_replays, tile_shape=st.tile_shape, tune_ms=list(st.tune_ms))
+
+    def boundness_result(self, num_members: int | None = None, num_clumps: int | None = None):
+        nm = self._bnd_nm if num_members is None else num_members
+        nc = self._bnd_nc if num_clumps is None else num_clumps
+        bound = np.zeros(nm, np.uint8)
+        arr = (PlanBoundnessClump * max(nc, 1))()
+        self._chk(lib().plan_boundness_result(self._h, bound.ctypes.data, arr), "plan_boundness_result")
+        out = dict(n_in=np.array([arr[i].n_in for i in range(nc)]), n_bound=np.array([arr[i].n_bound for i in range(nc)]),
+                   iterations=np.array([arr[i].iterations for i in range(nc)]),
+                   mass=np.array([arr[i].mass for i in range(nc)]),
+                   com=np.array([list(arr[i].center_of_mass) for i in range(nc)]).reshape(-1, 3),
+                   E_kin=np.array([arr[i].E_kin for i in range(nc)]), E_pot=np.array([arr[i].E_pot for i in range(nc)]))
+        return bound.astype(bool), out
 
     # --- inspection -------------------------------------------------------------
     def tree_particles(self) -> dict:

@@ -128,3 +128,20 @@ def catalog_to_py(cat: PlanCatalog) -> dict:
                 peak_density_threshold=cat.peak_density_threshold,
                 clump_mass_threshold=cat.clump_mass_threshold,
                 Mp_max=cat.Mp_max, Mp_tot=cat.Mp_tot)
+
+
+class PlanBoundnessIn(C.Structure):
+    _fields_ = [("num_clumps", C.c_uint32), ("offsets", C.c_void_p), ("pos", C.c_void_p), ("vel", C.c_void_p),
+                ("mass", C.c_void_p), ("G", C.c_double), ("Omega", C.c_double), ("q", C.c_double)]
+
+
+class PlanBoundnessClump(C.Structure):
+    _fields_ = [("n_in", C.c_uint32), ("n_bound", C.c_uint32), ("iterations", C.c_uint32), ("reserved", C.c_uint32),
+                ("mass", C.c_double), ("center_of_mass", C.c_double * 3), ("vel_com", C.c_double * 3),
+                ("E_kin", C.c_double), ("E_pot", C.c_double)]
+
+
+class PlanBoundnessStats(C.Structure):
+    _fields_ = [("ms", C.c_double), ("pair_evaluations", C.c_double), ("refit_clumps", C.c_uint32),
+                ("graph_replays", C.c_uint32), ("tile_shape", C.c_int32), ("reserved", C.c_int32),
+                ("tune_ms", C.c_double * 4)]

@@ -5,6 +5,7 @@
 #include "clumps.h"
 #include "ctx.h"
 #include "hop.h"
+#include "boundness.h"
 
 #include <chrono>
 #include <cmath>
@@ -29,6 +30,8 @@ struct plan_ctx {
     bool have_hop = false;
     DevParticles ps;
     DevParticles saved;      /* optional pristine copy of the N input particles */
+    BoundnessEngine bnd;     /* staged pairwise-boundness batch */
+    bool have_bnd_result = false;
     DevTree tree;
     DevTree ctree;
     DevParticles cps;
@@ -985,6 +988,65 @@ int plan_last_density_radius_ms(plan_ctx *c, double *ms)
     if (!c || !ms) return PLAN_E_ARG;
     ms[0] = c->density_radius_ms;
     return PLAN_OK;
+}
+
+int plan_boundness_load(plan_ctx *c, const plan_boundness_in *in)
+{
+    return guarded(c, [&]() -> int {
+        if (!in || !in->offsets) return fail(c, PLAN_E_ARG, "boundness: no input");
+        const uint32_t nm = in->offsets[in->num_clumps];
+        if (in->offsets[0] != 0) return fail(c, PLAN_E_ARG, "boundness: offsets[0] must be 0");
+        for (uint32_t k = 0; k < in->num_clumps; ++k)
+            if (in->offsets[k + 1] < in->offsets[k]) return fail(c, PLAN_E_ARG, "boundness: offsets must not decrease");
+        if (nm && (!in->pos || !in->vel || !in->mass)) return fail(c, PLAN_E_ARG, "boundness: missing member arrays");
+        if (!(in->G > 0)) return fail(c, PLAN_E_ARG, "boundness: G must be positive");
+        c->bnd.upload(*in, c->stream);
+        c->have_bnd_result = false;
+        return PLAN_OK;
+    });
+}
+
+int plan_boundness_load_catalog(plan_ctx *c, const plan_catalog *cat)
+{
+    return guarded(c, [&]() -> int {
+        if (!cat) return fail(c, PLAN_E_ARG, "boundness: no catalog");
+        if (!c->have_tree) return fail(c, PLAN_E_STATE, "boundness: no tree");
+        std::vector<uint32_t> off(cat->num_clumps + 1, 0), mem;
+        for (uint32_t k = 0; k < cat->num_clumps; ++k) {
+            const plan_clump &q = cat->clumps[k];
+            off[k + 1] = off[k] + q.num_members;
+            for (uint32_t j = 0; j < q.num_members; ++j) {
+                const uint32_t t = cat->members[q.member_offset + j];
+                if (t >= c->tree.n) return fail(c, PLAN_E_ARG, "boundness: catalog member outside the tree");
+                mem.push_back(t);
+            }
+        }
+        c->bnd.upload_tree(cat->num_clumps, off.data(), mem.data(), c->tree.pm, c->tree.va, c->d.grav_constant,
+                           c->d.Omega, c->d.q, c->stream);
+        c->have_bnd_result = false;
+        return PLAN_OK;
+    });
+}
+
+int plan_boundness_run(plan_ctx *c, int form, uint32_t max_iter, plan_boundness_stats *st)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->bnd.loaded()) return fail(c, PLAN_E_STATE, "boundness: no batch loaded");
+        if (form != PLAN_BOUND_FP64 && form != PLAN_BOUND_FP32) return fail(c, PLAN_E_ARG, "boundness: unknown form");
+        if (max_iter == 0) return fail(c, PLAN_E_ARG, "boundness: max_iter must be positive");
+        c->bnd.run(form, max_iter, c->stream, st);
+        c->have_bnd_result = true;
+        return PLAN_OK;
+    });
+}
+
+int plan_boundness_result(plan_ctx *c, uint8_t *bound, plan_boundness_clump *clumps)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_bnd_result) return fail(c, PLAN_E_STATE, "boundness: no result");
+        c->bnd.download(bound, clumps, c->stream);
+        return PLAN_OK;
+    });
 }
 
 }  // extern "C"

@@ -204,3 +204,117 @@ def small_case(n: int, n_clumps: int, seed: int = 0, nx: int | None = None, knn:
 
 def to_plan_params_dict(snap: Snapshot) -> dict:
     return dict(snap.params)
+
+
+# ---------------------------------------------------------------- boundness batches
+
+@dataclass
+class ClumpBatch:
+    """Clumps in CSR form for the pairwise-boundness entry points."""
+    offsets: np.ndarray     # num_clumps + 1
+    pos: np.ndarray         # members x 3 (float32 values)
+    vel: np.ndarray         # members x 3 (float32 values)
+    mass: np.ndarray        # members
+    n_interlopers: np.ndarray
+    G: float
+    Omega: float = 1.0
+    q: float = 1.5
+
+    @property
+    def num_clumps(self):
+        return len(self.offsets) - 1
+
+
+def power_law_sizes(rng, n, n_min, n_max, slope):
+    """n sizes with dN/dlnN proportional to N^slope on [n_min, n_max]."""
+    a, b = math.log(n_min), math.log(n_max)
+    u = rng.uniform(0.0, 1.0, n)
+    ea, eb = math.exp(slope * a), math.exp(slope * b)
+    x = np.log(ea + u * (eb - ea)) / slope
+    return np.clip(np.rint(np.exp(x)).astype(np.int64), n_min, n_max)
+
+
+def boundness_batch(n_clumps: int = 5000, n_min: int = 32, n_max: int = 2_000_000, slope: float = -0.6,
+                    interloper_max: float = 0.3, hill_frac=(0.1, 0.3), seed: int = 0,
+                    virial_targets: dict | None = None) -> ClumpBatch:
+    """Plummer clumps with power-law member counts and 0..interloper_max
+    extra unbound interlopers from a hot Gaussian shell (radius 2a, width
+    a/2, velocity dispersion 3x the local escape speed).  Particle mass and G
+    follow the 16.8M-snapshot parameters (Omega = 1, q = 1.5); each clump's
+    scale radius is a log-uniform fraction hill_frac of its Hill radius
+    (G M / 3 Omega^2)^(1/3), so the sphere is bound against the shear.
+    Positions and velocities are float32 values.  virial_targets {clump: ratio} rescales the velocities of
+    those clumps about their mean so that E_kin / |E_pot| (pairwise, FP64)
+    equals the ratio (1 puts the total energy at zero)."""
+    rng = np.random.default_rng(seed)
+    params = default_params(16777216)
+    G = grav_constant(params)
+    m_par = particle_mass(params, 16777216)
+    sizes = power_law_sizes(rng, n_clumps, n_min, n_max, slope)
+    frac = rng.uniform(0.0, interloper_max, n_clumps)
+    n_int = np.rint(sizes * frac).astype(np.int64)
+    tot = sizes + n_int
+    offsets = np.zeros(n_clumps + 1, np.int64)
+    offsets[1:] = np.cumsum(tot)
+    nm = int(offsets[-1])
+    pos = np.empty((nm, 3), np.float32)
+    vel = np.empty((nm, 3), np.float32)
+    hill = (G * sizes * m_par / 3.0) ** (1.0 / 3.0)
+    scale = hill * np.exp(rng.uniform(math.log(hill_frac[0]), math.log(hill_frac[1]), n_clumps))
+    centre = rng.uniform(-0.1, 0.1, (n_clumps, 3))
+    for c in range(n_clumps):
+        k, ki, a = int(sizes[c]), int(n_int[c]), float(scale[c])
+        M = k * m_par
+        x, v, _ = plummer_members(rng, k, a, M, G)
+        r = np.abs(rng.normal(2.0 * a, 0.5 * a, ki))
+        xi = _iso(rng, ki) * r[:, None]
+        vesc = np.sqrt(2.0 * G * M / np.sqrt(r * r + a * a))
+        vi = rng.normal(0.0, 1.0, (ki, 3)) * (3.0 * vesc)[:, None]
+        o = int(offsets[c])
+        pos[o:o + k] = (centre[c] + x).astype(np.float32)
+        pos[o + k:o + k + ki] = (centre[c] + xi).astype(np.float32)
+        vel[o:o + k] = v.astype(np.float32)
+        vel[o + k:o + k + ki] = vi.astype(np.float32)
+    mass = np.full(nm, m_par)
+    batch = ClumpBatch(offsets=offsets.astype(np.uint32), pos=pos, vel=vel, mass=mass, n_interlopers=n_int, G=G)
+    for c, ratio in (virial_targets or {}).items():
+        rescale_to_virial(batch, c, ratio)
+    return batch
+
+
+def pairwise_energies_fp64(pos, vel, mass, G, Omega=1.0, q=1.5):
+    """E_kin (shear-corrected, about the centre of mass) and the pairwise
+    E_pot of one member set, binary64 brute force in numpy."""
+    p = np.asarray(pos, np.float64)
+    v = np.asarray(vel, np.float64)
+    m = np.asarray(mass, np.float64)
+    M = m.sum()
+    X = (m[:, None] * p).sum(0) / M
+    V = (m[:, None] * v).sum(0) / M
+    dv = v - V
+    dv[:, 1] += q * Omega * (X[0] - p[:, 0])
+    ek = 0.5 * (m * (dv * dv).sum(1)).sum()
+    ep = 0.0
+    for s in range(0, len(m), 2048):
+        d = p[s:s + 2048, None, :] - p[None, :, :]
+        r = np.sqrt((d * d).sum(-1))
+        with np.errstate(divide="ignore"):
+            inv = np.where(r > 0, 1.0 / r, 0.0)
+        ep += -0.5 * G * (m[s:s + 2048] * (inv * m[None, :]).sum(1)).sum()
+    return ek, ep
+
+
+def rescale_to_virial(batch: ClumpBatch, c: int, ratio: float):
+    """Scale clump c's velocity dispersion about its mass-weighted mean so
+    that E_kin / |E_pot| = ratio for the whole member set."""
+    o0, o1 = int(batch.offsets[c]), int(batch.offsets[c + 1])
+    m = batch.mass[o0:o1]
+    # the shear term does not scale with the dispersion: a few fixed-point steps
+    for _ in range(6):
+        ek, ep = pairwise_energies_fp64(batch.pos[o0:o1], batch.vel[o0:o1], m, batch.G, batch.Omega, batch.q)
+        if ek <= 0 or abs(ek / abs(ep) - ratio) < 1e-5:
+            return
+        lam = math.sqrt(ratio * abs(ep) / ek)
+        v = batch.vel[o0:o1].astype(np.float64)
+        V = (m[:, None] * v).sum(0) / m.sum()
+        batch.vel[o0:o1] = (V + lam * (v - V)).astype(np.float32)

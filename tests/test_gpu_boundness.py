@@ -1,0 +1,144 @@
+"""Pairwise boundness on the GPU (both forms) against the FP64 oracle.
+
+FP64 per-clump form: same members kept, energies to 1e-10 relative.
+FP32 batched form: energies within 1e-6 relative per clump (the north
+star's bound), bound sets identical except for members whose oracle energy is
+within 1e-6 of zero relative to its terms, clumps with near-zero total energy
+re-decided in binary64, results reproducible run to run."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+_cache = {}
+
+
+def _batch():
+    if "b" not in _cache:
+        import oracle_py as O
+        from plan_amd import synth
+
+        b = synth.boundness_batch(n_clumps=60, n_min=32, n_max=6000, seed=1, virial_targets={3: 1.0, 10: 1.0, 20: 1.0})
+        O.set_threads(8)
+        ob, oo = O.boundness(b.offsets, b.pos, b.vel, b.mass, b.G, b.Omega, b.q)
+        _cache["b"] = (b, ob, oo)
+    return _cache["b"]
+
+
+def _ctx():
+    import plan_amd
+
+    if "ctx" not in _cache:
+        _cache["ctx"] = plan_amd.Context(0)
+    return _cache["ctx"]
+
+
+def near_zero_members(b, ob, tol):
+    """members of the oracle's final bound sets whose energy is within tol of
+    zero, and members outside whose energy against that set is"""
+    out = np.zeros(len(b.mass), bool)
+    for c in range(b.num_clumps):
+        a, e = int(b.offsets[c]), int(b.offsets[c + 1])
+        B = np.flatnonzero(ob[a:e]) + a
+        if len(B) == 0:
+            continue
+        p = b.pos.astype(np.float64)
+        v = b.vel.astype(np.float64)
+        m = b.mass
+        M = m[B].sum()
+        X = (m[B, None] * p[B]).sum(0) / M
+        V = (m[B, None] * v[B]).sum(0) / M
+        idx = np.arange(a, e)
+        d = p[idx, None, :] - p[None, B, :]
+        r = np.sqrt((d * d).sum(-1))
+        with np.errstate(divide="ignore"):
+            inv = np.where(r > 0, 1.0 / r, 0.0)
+        phi = -b.G * (inv * m[None, B]).sum(1)
+        dv = v[idx] - V
+        dv[:, 1] += b.q * b.Omega * (X[0] - p[idx, 0])
+        kin = 0.5 * (dv * dv).sum(1)
+        out[idx] = np.abs(kin + phi) <= tol * (kin + np.abs(phi))
+    return out
+
+
+def test_fp64_per_clump_matches_oracle():
+    b, ob, oo = _batch()
+    ctx = _ctx()
+    print(f"[boundness fp64] clumps={b.num_clumps} members={len(b.mass)}", flush=True)
+    ctx.boundness_load(b)
+    st = ctx.boundness_run(ctx.FP64)
+    gb, go = ctx.boundness_result()
+    assert np.array_equal(gb, ob)
+    assert np.array_equal(go["n_bound"], oo["n_bound"])
+    assert np.array_equal(go["iterations"], oo["iterations"])
+    nz = oo["n_bound"] > 1
+    np.testing.assert_allclose(go["E_pot"][nz], oo["E_pot"][nz], rtol=1e-10)
+    np.testing.assert_allclose(go["E_kin"][nz], oo["E_kin"][nz], rtol=1e-10)
+    assert st["pair_evaluations"] > 0
+
+
+def test_fp32_batched_matches_oracle():
+    b, ob, oo = _batch()
+    ctx = _ctx()
+    print(f"[boundness fp32] clumps={b.num_clumps} members={len(b.mass)}", flush=True)
+    ctx.boundness_load(b)
+    st = ctx.boundness_run(ctx.FP32)
+    gb, go = ctx.boundness_result()
+    diff = gb != ob
+    if diff.any():
+        nearz = near_zero_members(b, ob, 1e-6)
+        assert np.all(nearz[diff]), np.flatnonzero(diff & ~nearz)[:10]
+    same = go["n_bound"] == oo["n_bound"]
+    ok = same & (oo["n_bound"] > 1)
+    rel_pot = np.abs(go["E_pot"][ok] - oo["E_pot"][ok]) / np.abs(oo["E_pot"][ok])
+    rel_kin = np.abs(go["E_kin"][ok] - oo["E_kin"][ok]) / np.abs(oo["E_kin"][ok])
+    print(f"fp32 batched: max rel E_pot error {rel_pot.max():.2e}, E_kin {rel_kin.max():.2e} over {ok.sum()} "
+          f"clumps; refit {st['refit_clumps']}, replays {st['graph_replays']}, shape {st['tile_shape']}")
+    assert rel_pot.max() <= 1e-6
+    assert rel_kin.max() <= 1e-6
+    # the three clumps seeded at E_kin/|E_pot| = 1 come within 1e-3 of zero energy
+    assert st["refit_clumps"] >= 1
+    # reproducible
+    st2 = ctx.boundness_run(ctx.FP32)
+    gb2, go2 = ctx.boundness_result()
+    assert np.array_equal(gb2, gb)
+    assert np.array_equal(go2["E_pot"], go["E_pot"])
+
+
+def test_boundness_of_catalog_clumps():
+    """The precision form on the clumps FindPlanetesimals returns, with
+    members taken from the device tree."""
+    import oracle_py as O
+    import plan_amd
+    from _cases import CASES
+
+    snap = CASES["n65k"]()
+    ctx = plan_amd.Context(0)
+    print(f"[boundness catalog] n={snap.n}", flush=True)
+    ctx.set_particles(snap.params, snap.pos, snap.vel, snap.density, snap.property_index, snap.id)
+    ctx.make_ghosts()
+    ctx.build_tree(True)
+    cat = ctx.find_planetesimals(keep_for_boundness=True)
+    assert len(cat["clumps"]) > 0
+    parts = ctx.tree_particles()
+    off = np.zeros(len(cat["clumps"]) + 1, np.uint32)
+    mem = []
+    for k, c in enumerate(cat["clumps"]):
+        m = cat["members"][c["member_offset"]:c["member_offset"] + c["num_members"]]
+        mem.append(m)
+        off[k + 1] = off[k] + len(m)
+    mem = np.concatenate(mem)
+    d = O.derive(snap.params, snap.n)
+    ob, oo = O.boundness(off, parts["pos"][mem], parts["vel"][mem], parts["mass"][mem], d.grav_constant, d.Omega,
+                         d.q)
+    st = ctx.boundness_run(ctx.FP64)
+    gb, go = ctx.boundness_result()
+    assert np.array_equal(gb, ob)
+    np.testing.assert_allclose(go["E_pot"], oo["E_pot"], rtol=1e-10)
+    st = ctx.boundness_run(ctx.FP32)
+    gb, go = ctx.boundness_result()
+    ok = (go["n_bound"] == oo["n_bound"]) & (oo["n_bound"] > 1)
+    assert ok.sum() >= len(ok) - 1
+    rel = np.abs(go["E_pot"][ok] - oo["E_pot"][ok]) / np.abs(oo["E_pot"][ok])
+    print(f"catalog clumps: {len(ok)}, max rel E_pot error (fp32) {rel.max():.2e}")
+    assert rel.max() <= 1e-6
