@@ -32,6 +32,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+PEAK_FP32_TFLOPS = 157.3
+# useful flops per pair term: 3 sub, r^2 (1 mul + 2 FMA = 5), rsqrt 1, x m 1, accumulate 1
+PAIR_FLOPS = 11
 # algorithmic bytes per tree particle for the density pass: binary64 position
 # (24) + mass (8) read, density (8) written
 DENSITY_BYTES_PER_PARTICLE = 40
@@ -64,6 +67,72 @@ def measured_traffic(launch_units: int):
         return None
 
 
+def boundness_bench(ctx, args) -> dict:
+    """The second half of the metric: pairwise boundness clumps/s, FP32
+    compensated + batched against FP64 per clump, on the batching workload
+    (5,000 Plummer clumps, 32..2e6 members with dN/dlnN ~ N^-0.6, 0-30%
+    hot-shell interlopers, seed 0); and the precision form on the catalog
+    of the last clump-finding step (energy error against FP64 per clump)."""
+    import plan_amd
+    from plan_amd import synth
+
+    out = {}
+    # precision form on the clumps just found in the 16.8M snapshot
+    ctx.restore_input()
+    ctx.make_ghosts()
+    ctx.build_tree(True)
+    cat = ctx.find_planetesimals(keep_for_boundness=True)
+    s64 = ctx.boundness_run(ctx.FP64)
+    _, r64 = ctx.boundness_result()
+    s32 = ctx.boundness_run(ctx.FP32)
+    _, r32 = ctx.boundness_result()
+    ok = (r32["n_bound"] == r64["n_bound"]) & (r64["n_bound"] > 1)
+    rel = np.abs(r32["E_pot"][ok] - r64["E_pot"][ok]) / np.abs(r64["E_pot"][ok])
+    relk = np.abs(r32["E_kin"][ok] - r64["E_kin"][ok]) / np.abs(r64["E_kin"][ok])
+    out["precision_form_on_catalog"] = {
+        "clumps": len(cat["clumps"]), "same_bound_count": int(ok.sum()),
+        "max_rel_err_E_pot": float(rel.max()) if ok.any() else None,
+        "max_rel_err_E_kin": float(relk.max()) if ok.any() else None,
+        "fp64_ms": s64["ms"], "fp32_ms": s32["ms"], "refit_clumps": s32["refit_clumps"]}
+    log(f"[bench] boundness on catalog: {out['precision_form_on_catalog']}")
+    # batching workload
+    t0 = time.perf_counter()
+    batch = synth.boundness_batch(n_clumps=args.boundness_clumps, seed=0)
+    log(f"[bench] boundness batch: {batch.num_clumps} clumps, {int(batch.offsets[-1])} members, "
+        f"generated in {time.perf_counter() - t0:.1f}s")
+    ctx.boundness_load(batch)
+    ctx.boundness_run(ctx.FP32)  # warm-up: autotuner and graph
+    st32 = ctx.boundness_run(ctx.FP32)
+    _, b32 = ctx.boundness_result()
+    log(f"[bench] fp32 batched: {st32}")
+    st64 = ctx.boundness_run(ctx.FP64)
+    _, b64 = ctx.boundness_result()
+    log(f"[bench] fp64 per clump: {st64}")
+    ok = (b32["n_bound"] == b64["n_bound"]) & (b64["n_bound"] > 1)
+    rel = np.abs(b32["E_pot"][ok] - b64["E_pot"][ok]) / np.abs(b64["E_pot"][ok])
+    nc = batch.num_clumps
+    tflops = st32["pair_evaluations"] * PAIR_FLOPS / (st32["ms"] * 1e-3) / 1e12
+    out.update({
+        "metric": "boundness clumps/s FP32 vs FP64",
+        "workload": f"{nc} Plummer clumps, 32..2e6 members (dN/dlnN ~ N^-0.6), 0-30% interlopers, seed 0",
+        "members": int(batch.offsets[-1]),
+        "fp32_batched_clumps_per_s": nc / (st32["ms"] * 1e-3),
+        "fp64_per_clump_clumps_per_s": nc / (st64["ms"] * 1e-3),
+        "speedup": st64["ms"] / st32["ms"],
+        "fp32_ms": st32["ms"], "fp64_ms": st64["ms"],
+        "same_bound_count": int(ok.sum()),
+        "max_rel_err_E_pot_vs_fp64": float(rel.max()) if ok.any() else None,
+        "pair_evaluations": st32["pair_evaluations"],
+        "tile_shape": st32["tile_shape"], "autotune_ms": st32["tune_ms"], "graph_replays": st32["graph_replays"],
+        "refit_clumps": st32["refit_clumps"],
+        "roofline": {"kernel": "k_bnd_pairs<float> (pairwise potential, Kahan)", "bound": "valu",
+                     "achieved": tflops, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": tflops / PEAK_FP32_TFLOPS, "flops_per_pair": PAIR_FLOPS,
+                     "note": "stage time incl. lists, totals and energies; pair flops only"},
+    })
+    return out
+
+
 def cpu_baseline(threads: int, n_sample: int) -> dict:
     """The oracle (CPU restatement, OpenMP) end to end on a C2-mix snapshot
     of n_sample particles with the same K and H."""
@@ -92,6 +161,8 @@ def main():
     ap.add_argument("--n", type=int, default=16777216)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=2097152)
+    ap.add_argument("--no-boundness", action="store_true")
+    ap.add_argument("--boundness-clumps", type=int, default=5000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -143,6 +214,9 @@ def main():
     achieved = DENSITY_BYTES_PER_PARTICLE * launch_units / (k_ms * 1e-3) / 1e9
     traffic = measured_traffic(launch_units)
     mean_st = {k: float(np.mean([s[k] for s in stages])) for k in stages[0]}
+    boundness = None
+    if not args.no_boundness:
+        boundness = boundness_bench(ctx, args)
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
@@ -182,6 +256,7 @@ def main():
             "radius_kernel_ms": float(np.mean(rms)),
         },
         "cpu_baseline": cpu,
+        "boundness": boundness,
     }
     print(json.dumps(out))
 

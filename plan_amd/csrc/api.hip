@@ -243,28 +243,7 @@ struct Services : ClumpServices {
     void run_balls(const DevTree &t, const std::vector<double4> &queries, const double *rho, double rho_min,
                    int filter, std::vector<std::vector<uint32_t>> &out)
     {
-        hipStream_t s = c->stream;
-        uint32_t nq = (uint32_t)queries.size();
-        out.assign(nq, {});
-        if (nq == 0) return;
-        double4 *q = c->work.get_t<double4>("bq", nq);
-        uint32_t *cnt = c->work.get_t<uint32_t>("bcnt", nq + 1);
-        uint32_t *off = c->work.get_t<uint32_t>("boff", nq + 1);
-        uint32_t *tot = c->work.get_t<uint32_t>("btot", 4);
-        HIPCHK(hipMemcpyAsync(q, queries.data(), nq * sizeof(double4), hipMemcpyHostToDevice, s));
-        dev_ball(t, nq, q, cnt, nullptr, nullptr, rho, rho_min, filter, s);
-        scan_exclusive_u32(c->work, cnt, off, nq, s, tot);
-        uint32_t total = 0;
-        HIPCHK(hipMemcpyAsync(&total, tot, 4, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        uint32_t *buf = c->work.get_t<uint32_t>("bout", total + 1);
-        dev_ball(t, nq, q, cnt, off, buf, rho, rho_min, filter, s);
-        std::vector<uint32_t> hoff(nq), hcnt(nq), hb(total);
-        HIPCHK(hipMemcpyAsync(hoff.data(), off, nq * 4, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync(hcnt.data(), cnt, nq * 4, hipMemcpyDeviceToHost, s));
-        if (total) HIPCHK(hipMemcpyAsync(hb.data(), buf, (size_t)total * 4, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        for (uint32_t i = 0; i < nq; ++i) out[i].assign(hb.begin() + hoff[i], hb.begin() + hoff[i] + hcnt[i]);
+        dev_ball_lists(t, c->work, queries, rho, rho_min, filter, out, c->stream);
     }
 
     void clump_tree_balls(const std::vector<double4> &com_peak, const std::vector<double4> &vel,

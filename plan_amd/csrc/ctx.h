@@ -81,5 +81,9 @@ void dev_knn(const DevTree &t, uint32_t K, int mode, int kernel, uint32_t nq, co
  * optionally keeping only particles with rho >= rho_min (rho_min_on). */
 void dev_ball(const DevTree &t, uint32_t nq, const double4 *centre_r, uint32_t *counts, const uint32_t *offsets,
               uint32_t *out, const double *rho, double rho_min, int filter, hipStream_t s);
+/* the same searches expanded level by level over all queries at once; results
+ * per query in traversal order (ascending tree index) */
+void dev_ball_lists(const DevTree &t, Arena &a, const std::vector<double4> &queries, const double *rho,
+                    double rho_min, int filter, std::vector<std::vector<uint32_t>> &out, hipStream_t s);
 
 }  // namespace plan

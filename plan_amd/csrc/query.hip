@@ -878,6 +878,205 @@ __global__ __launch_bounds__(64) void k_ball(TreeView tv, uint32_t nq, const dou
     if (!FILL && lane == 0) counts[q] = cnt;
 }
 
+/* ---------------------------------------------- level-parallel ball search
+ *
+ * The reference's BallSearch (src/tree.hpp:3275-3304) visits a node when every
+ * ancestor intersected the sphere without containing it; a visited node that
+ * the sphere contains contributes all its particles, a visited intersecting
+ * leaf those within r.  The visited set does not depend on the visiting order,
+ * so it is expanded level by level for all queries at once (one thread per
+ * (query, node)), the contributing ranges are sorted by (query, first
+ * particle) — the reference's traversal order — and the candidates of all
+ * ranges are tested and compacted in parallel.  A large ball thus spreads over
+ * the whole GPU instead of one wave. */
+
+struct BallTask {
+    uint32_t q, begin, end, leaf;
+};
+
+__global__ void k_bfs_seed(uint32_t nq, uint2 *__restrict__ f, uint32_t *__restrict__ counts)
+{
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < nq) f[q] = make_uint2(q, 0u);
+    if (q == 0) counts[0] = nq;
+}
+
+__global__ void k_bfs_expand(TreeView tv, const double4 *__restrict__ cr, const uint2 *__restrict__ f,
+                             const uint32_t *__restrict__ fcount, uint2 *__restrict__ g, uint32_t *__restrict__ gcount,
+                             uint32_t gcap, BallTask *__restrict__ tasks, uint32_t *__restrict__ tcount, uint32_t tcap,
+                             uint32_t *__restrict__ overflow)
+{
+    const uint32_t n = *fcount;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const uint2 it = f[k];
+        const double4 c = cr[it.x];
+        const NodeGeo geo = tv.ngeo[it.y];
+        const NodeTop top = tv.ntop[it.y];
+        const uint32_t nd = tv.ninfo[it.y] & 0xffu;
+        int act;
+        if (sphere_contain(geo, c.x, c.y, c.z, c.w, tv.to_diag)) act = 1;
+        else if (sphere_intersect(geo, c.x, c.y, c.z, c.w, tv.to_diag)) act = nd == 0 ? 4 : 2;
+        else act = 3;
+        if (act == 1 || act == 4) {
+            const uint32_t slot = atomicAdd(tcount, 1u);
+            if (slot < tcap) tasks[slot] = BallTask{it.x, top.begin, top.end, act == 4 ? 1u : 0u};
+            else *overflow = 1u;
+        } else if (act == 2) {
+            const uint32_t slot = atomicAdd(gcount, nd);
+            if (slot + nd <= gcap)
+                for (uint32_t d = 0; d < nd; ++d) g[slot + d] = make_uint2(it.x, top.first_daughter + d);
+            else *overflow = 1u;
+        }
+    }
+}
+
+__global__ void k_bfs_task_keys(const BallTask *__restrict__ tasks, uint32_t n, uint4 *__restrict__ keys)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) keys[k] = make_uint4(tasks[k].begin, tasks[k].q, 0u, k);
+}
+
+__global__ void k_bfs_task_sizes(const BallTask *__restrict__ tasks, const uint4 *__restrict__ keys, uint32_t n,
+                                 BallTask *__restrict__ sorted, uint32_t *__restrict__ sizes)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const BallTask t = tasks[keys[k].w];
+    sorted[k] = t;
+    sizes[k] = t.end - t.begin;
+}
+
+/* one thread per candidate: its task by binary search over the size prefix */
+__global__ void k_bfs_candidates(TreeView tv, const double4 *__restrict__ cr, const BallTask *__restrict__ sorted,
+                                 const uint32_t *__restrict__ pre, uint32_t ntask, uint32_t ncand,
+                                 const double *__restrict__ rho, double rho_min, int filter,
+                                 uint32_t *__restrict__ flag, uint32_t *__restrict__ pidx)
+{
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < ncand; k += gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = ntask;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (pre[mid] <= k) lo = mid;
+            else hi = mid;
+        }
+        const BallTask t = sorted[lo];
+        const uint32_t p = t.begin + (k - pre[lo]);
+        bool take = true;
+        if (t.leaf) {
+            const double4 c = cr[t.q];
+            const double4 x = tv.pm[p];
+            take = dist2(c.x, c.y, c.z, x.x, x.y, x.z) <= __dmul_rn(c.w, c.w);
+        }
+        if (take && filter) take = rho[p] >= rho_min;
+        flag[k] = take ? 1u : 0u;
+        pidx[k] = p;
+    }
+}
+
+__global__ void k_bfs_compact(const uint32_t *__restrict__ flag, const uint32_t *__restrict__ pos,
+                              const uint32_t *__restrict__ pidx, uint32_t ncand, uint32_t *__restrict__ out)
+{
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < ncand; k += gridDim.x * blockDim.x)
+        if (flag[k]) out[pos[k]] = pidx[k];
+}
+
+/* per task: (query, output position of its first candidate) */
+__global__ void k_bfs_task_out(const BallTask *__restrict__ sorted, const uint32_t *__restrict__ pre,
+                               const uint32_t *__restrict__ opos, uint32_t ntask, uint2 *__restrict__ tq)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < ntask) tq[k] = make_uint2(sorted[k].q, opos[pre[k]]);
+}
+
+void dev_ball_lists(const DevTree &t, Arena &a, const std::vector<double4> &queries, const double *rho,
+                    double rho_min, int filter, std::vector<std::vector<uint32_t>> &out, hipStream_t s)
+{
+    const uint32_t nq = (uint32_t)queries.size();
+    out.assign(nq, {});
+    if (nq == 0 || t.n == 0) return;
+    plan_derived d{};
+    d.cell_length[0] = d.cell_length[1] = d.cell_length[2] = 1;
+    d.cell_volume = 1;
+    const TreeView tv = view_of(t, d);
+    double4 *cr = a.get_t<double4>("bl_q", nq);
+    HIPCHK(hipMemcpyAsync(cr, queries.data(), nq * sizeof(double4), hipMemcpyHostToDevice, s));
+    const int levels = t.depth + 1;
+    uint32_t gcap = std::max<uint32_t>(1u << 20, 8 * nq), tcap = std::max<uint32_t>(1u << 20, 8 * nq);
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        uint2 *f0 = a.get_t<uint2>("bl_f0", gcap);
+        uint2 *f1 = a.get_t<uint2>("bl_f1", gcap);
+        BallTask *tasks = a.get_t<BallTask>("bl_tasks", tcap);
+        uint32_t *cnt = a.get_t<uint32_t>("bl_cnt", 40);
+        HIPCHK(hipMemsetAsync(cnt, 0, 40 * 4, s));
+        hipLaunchKernelGGL(k_bfs_seed, dim3(ceil_div(nq, 256)), dim3(256), 0, s, nq, f0, cnt);
+        LAUNCHCHK();
+        /* cnt[0..33]: frontier sizes per level, cnt[36] tasks, cnt[37] overflow */
+        for (int L = 0; L < levels && L < 34; ++L) {
+            uint2 *f = (L & 1) ? f1 : f0;
+            uint2 *g = (L & 1) ? f0 : f1;
+            hipLaunchKernelGGL(k_bfs_expand, dim3(1024), dim3(256), 0, s, tv, cr, f, cnt + L, g, cnt + L + 1, gcap,
+                               tasks, cnt + 36, tcap, cnt + 37);
+            LAUNCHCHK();
+        }
+        uint32_t h[2];
+        HIPCHK(hipMemcpyAsync(h, cnt + 36, 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (h[1]) {
+            gcap *= 4;
+            tcap *= 4;
+            continue;
+        }
+        const uint32_t nt = h[0];
+        if (nt == 0) return;
+        uint4 *k0 = a.get_t<uint4>("bl_k0", nt), *k1 = a.get_t<uint4>("bl_k1", nt);
+        hipLaunchKernelGGL(k_bfs_task_keys, dim3(ceil_div(nt, 256)), dim3(256), 0, s, tasks, nt, k0);
+        LAUNCHCHK();
+        radix_sort_96(a, &k0, &k1, nt, 2, s);
+        BallTask *sorted = a.get_t<BallTask>("bl_sorted", nt);
+        uint32_t *sizes = a.get_t<uint32_t>("bl_sizes", nt + 1);
+        uint32_t *pre = a.get_t<uint32_t>("bl_pre", nt + 1);
+        hipLaunchKernelGGL(k_bfs_task_sizes, dim3(ceil_div(nt, 256)), dim3(256), 0, s, tasks, k0, nt, sorted, sizes);
+        LAUNCHCHK();
+        scan_exclusive_u32(a, sizes, pre, nt, s, pre + nt);
+        uint32_t ncand = 0;
+        HIPCHK(hipMemcpyAsync(&ncand, pre + nt, 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        uint32_t *flag = a.get_t<uint32_t>("bl_flag", (size_t)ncand + 1);
+        uint32_t *pidx = a.get_t<uint32_t>("bl_pidx", (size_t)ncand + 1);
+        uint32_t *opos = a.get_t<uint32_t>("bl_opos", (size_t)ncand + 1);
+        const unsigned gb = std::min<unsigned>(4096, ceil_div(std::max<uint32_t>(ncand, 1), 256));
+        hipLaunchKernelGGL(k_bfs_candidates, dim3(gb), dim3(256), 0, s, tv, cr, sorted, pre, nt, ncand, rho, rho_min,
+                           filter, flag, pidx);
+        LAUNCHCHK();
+        scan_exclusive_u32(a, flag, opos, ncand, s, opos + ncand);
+        uint32_t total = 0;
+        HIPCHK(hipMemcpyAsync(&total, opos + ncand, 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        uint32_t *res = a.get_t<uint32_t>("bl_out", (size_t)total + 1);
+        hipLaunchKernelGGL(k_bfs_compact, dim3(gb), dim3(256), 0, s, flag, opos, pidx, ncand, res);
+        LAUNCHCHK();
+        uint2 *tq = a.get_t<uint2>("bl_tq", nt);
+        hipLaunchKernelGGL(k_bfs_task_out, dim3(ceil_div(nt, 256)), dim3(256), 0, s, sorted, pre, opos, nt, tq);
+        LAUNCHCHK();
+        std::vector<uint2> htq(nt);
+        std::vector<uint32_t> hres(total);
+        HIPCHK(hipMemcpyAsync(htq.data(), tq, (size_t)nt * sizeof(uint2), hipMemcpyDeviceToHost, s));
+        if (total) HIPCHK(hipMemcpyAsync(hres.data(), res, (size_t)total * 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        /* tasks are grouped by query: a query's results run from its first
+         * task's output position to the next query's */
+        for (uint32_t k = 0; k < nt;) {
+            uint32_t j = k;
+            while (j < nt && htq[j].x == htq[k].x) ++j;
+            const uint32_t b = htq[k].y, e = j < nt ? htq[j].y : total;
+            out[htq[k].x].assign(hres.begin() + b, hres.begin() + e);
+            k = j;
+        }
+        return;
+    }
+    throw std::runtime_error("ball search: node frontier exceeds the work capacity");
+}
+
 void dev_ball(const DevTree &t, uint32_t nq, const double4 *centre_r, uint32_t *counts, const uint32_t *offsets,
               uint32_t *out, const double *rho, double rho_min, int filter, hipStream_t s)
 {

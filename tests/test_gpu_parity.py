@@ -132,6 +132,27 @@ def test_point_knn(name):
 
 
 @pytest.mark.parametrize("name", NAMES)
+def test_ball_queries(name):
+    """BallSearch (src/tree.hpp:3275-3304): the same indices in the same order,
+    from radii far below the particle spacing to a large part of the box."""
+    snap, ot, g = run_case(name)
+    ctx = g["ctx"]
+    rng = np.random.default_rng(5)
+    pos = ot.particles()["pos"]
+    lo, hi = pos.min(0), pos.max(0)
+    span = (hi - lo).max()
+    centres = np.concatenate([pos[rng.integers(0, ot.n, 20)], rng.uniform(lo, hi, (20, 3))])
+    radii = span * 10.0 ** rng.uniform(-4, -0.7, len(centres))
+    total = 0
+    for c, r in zip(centres, radii):
+        want = ot.ball(c, r)
+        got = ctx.ball(c, r)
+        assert np.array_equal(got, want)
+        total += len(want)
+    assert total > 1000
+
+
+@pytest.mark.parametrize("name", NAMES)
 def test_hop(name):
     snap, ot, g = run_case(name)
     odn, opk = ot.hop()
