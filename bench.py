@@ -209,8 +209,8 @@ def main():
     value = args.n / (total / args.steps)
     k_ms = float(np.mean(kms))
     nparticles_tree = info.num_particles
-    # the timed launch is the first density launch: min(2^24, tree particles) queries
-    launch_units = min(1 << 24, nparticles_tree)
+    # the timed launch is the density launch: one query per tree particle
+    launch_units = nparticles_tree
     achieved = DENSITY_BYTES_PER_PARTICLE * launch_units / (k_ms * 1e-3) / 1e9
     traffic = measured_traffic(launch_units)
     mean_st = {k: float(np.mean([s[k] for s in stages])) for k in stages[0]}

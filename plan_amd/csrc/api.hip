@@ -124,6 +124,7 @@ void free_tree_nodes(DevTree &t)
     t.ngeo = nullptr;
     t.ntop = nullptr;
     t.ninfo = nullptr;
+    t.node_cap = 0;
 }
 
 void free_particles(DevParticles &p)
@@ -454,7 +455,6 @@ int plan_build_tree(plan_ctx *c, int check_pos, plan_tree_info *info)
     return guarded(c, [&]() -> int {
         if (!c->have_particles) return fail(c, PLAN_E_STATE, "no particle set loaded");
         HIPCHK(hipEventRecord(c->ev[0], c->stream));
-        free_tree_nodes(c->tree);
         double zc[3] = {0, 0, 0};
         dev_build_tree(c->tree, c->ps, c->d, check_pos != 0, zc, 0.0, c->stream);
         HIPCHK(hipEventRecord(c->ev[1], c->stream));

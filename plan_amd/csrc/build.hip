@@ -531,14 +531,24 @@ void dev_build_tree(DevTree &t, const DevParticles &ps, const plan_derived &d, b
     t.num_sinks = 0;
     if (n != T) t.num_sinks = 1; /* exact count computed on demand by callers */
 
-    /* level-synchronous node construction */
-    size_t cap = (size_t)2 * n + 64;
-    NodeGeo *ngeo = nullptr;
-    NodeTop *ntop = nullptr;
-    uint32_t *ninfo = nullptr;
-    HIPCHK(hipMalloc(&ngeo, cap * sizeof(NodeGeo)));
-    HIPCHK(hipMalloc(&ntop, cap * sizeof(NodeTop)));
-    HIPCHK(hipMalloc(&ninfo, cap * sizeof(uint32_t)));
+    /* level-synchronous node construction into grow-only node arrays */
+    size_t cap = t.node_cap;
+    if (cap < (size_t)n + 64) {
+        if (t.ngeo) HIPCHK(hipFree(t.ngeo));
+        if (t.ntop) HIPCHK(hipFree(t.ntop));
+        if (t.ninfo) HIPCHK(hipFree(t.ninfo));
+        t.ngeo = nullptr;
+        t.ntop = nullptr;
+        t.ninfo = nullptr;
+        cap = (size_t)n + n / 2 + 64;
+        HIPCHK(hipMalloc(&t.ngeo, cap * sizeof(NodeGeo)));
+        HIPCHK(hipMalloc(&t.ntop, cap * sizeof(NodeTop)));
+        HIPCHK(hipMalloc(&t.ninfo, cap * sizeof(uint32_t)));
+        t.node_cap = cap;
+    }
+    NodeGeo *ngeo = t.ngeo;
+    NodeTop *ntop = t.ntop;
+    uint32_t *ninfo = t.ninfo;
     NodeGeo rg{t.root_center[0], t.root_center[1], t.root_center[2], t.half_width};
     NodeTop rt{0u, n, 0u, PLAN_NONE};
     uint32_t ri = 1u << 8;
@@ -572,6 +582,10 @@ void dev_build_tree(DevTree &t, const DevParticles &ps, const plan_derived &d, b
             grow_copy(a, "ntop", ntop, base, ncap, s);
             grow_copy(a, "ninfo", ninfo, base, ncap, s);
             cap = ncap;
+            t.ngeo = ngeo;
+            t.ntop = ntop;
+            t.ninfo = ninfo;
+            t.node_cap = cap;
         }
         hipLaunchKernelGGL(k_lvl_starts, dim3(ceil_div(n, BLK)), dim3(BLK), 0, s, flags, cidx, n, cstart);
         LAUNCHCHK();
@@ -597,13 +611,6 @@ void dev_build_tree(DevTree &t, const DevParticles &ps, const plan_derived &d, b
     t.num_nodes = base;
     t.level_ptr[0] = 0;
     for (int l = 1; l < 32; ++l) t.level_ptr[l] = t.level_ptr[l - 1] + t.level_count[l - 1];
-    /* adopt the node arrays into the arena bookkeeping */
-    if (t.ngeo) HIPCHK(hipFree(t.ngeo));
-    if (t.ntop) HIPCHK(hipFree(t.ntop));
-    if (t.ninfo) HIPCHK(hipFree(t.ninfo));
-    t.ngeo = ngeo;
-    t.ntop = ntop;
-    t.ninfo = ninfo;
     const uint32_t nn = t.num_nodes;
     t.skip = a.get_t<uint32_t>("skip", nn);
     t.leaf_of = a.get_t<uint32_t>("leaf_of", n);

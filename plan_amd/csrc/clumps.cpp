@@ -138,7 +138,8 @@ struct Ops {
             std::sort(v.begin(), v.end(), pd_less);
             return;
         }
-        std::vector<PD> orig(v);
+        static thread_local std::vector<PD> orig;
+        orig.assign(v.begin(), v.end());
         size_t budget = 16 * v.size() + 1024, moves = 0;
         bool ok = true;
         for (size_t i = 1; i < v.size() && ok; ++i) {
@@ -591,6 +592,90 @@ static void to_records(const HostMembers &hm, const std::vector<uint32_t> &tidx,
     }
 }
 
+/* A clump's members as contiguous records in the order of c.indices, so the
+ * per-iteration re-sort of the rebinding loop streams memory instead of
+ * gathering from the member table. */
+struct LRec {
+    double x, y, z, m, vx, vy, vz, rho;
+    uint32_t h;
+};
+
+static LRec lrec_of(const HostMembers &hm, uint32_t h)
+{
+    const MemberRec &r = hm.rec[h];
+    return LRec{r.x, r.y, r.z, r.m, r.vx, r.vy, r.vz, r.rho, h};
+}
+
+/* Ops::sort_particles(c, nearly_sorted = true) on the local records, with the
+ * same arithmetic in the same order (src/tree.hpp:4431-4512): kinematics in
+ * member order, distances, sort, radii.  The sort works on (position,
+ * distance) pairs; std::sort's tie order depends only on the input order and
+ * the distances, so it matches sorting (index, distance) pairs. */
+struct SortWork {
+    std::vector<PD> v;
+    std::vector<LRec> nl;
+};
+
+static void sort_local(HClump &c, std::vector<LRec> &lr, SortWork &w)
+{
+    const size_t n = lr.size();
+    c.total_mass = 0;
+    for (int k = 0; k < 3; ++k) c.com[k] = c.vel[k] = 0;
+    for (const LRec &r : lr) {
+        c.total_mass += r.m;
+        c.com[0] += r.x * r.m;
+        c.com[1] += r.y * r.m;
+        c.com[2] += r.z * r.m;
+        c.vel[0] += r.vx * r.m;
+        c.vel[1] += r.vy * r.m;
+        c.vel[2] += r.vz * r.m;
+    }
+    for (int k = 0; k < 3; ++k) c.com[k] /= c.total_mass;
+    for (int k = 0; k < 3; ++k) c.vel[k] /= c.total_mass;
+    std::vector<PD> &v = w.v;
+    v.resize(n);
+    for (size_t k = 0; k < n; ++k) {
+        const LRec &r = lr[k];
+        double dx = r.x - c.com[0], dy = r.y - c.com[1], dz = r.z - c.com[2];
+        double q = 0;
+        q += dx * dx;
+        q += dy * dy;
+        q += dz * dz;
+        v[k] = PD((uint32_t)k, std::sqrt(q));
+    }
+    Ops::sort_by_distance(v, true);
+    std::vector<LRec> &nl = w.nl;
+    nl.resize(n);
+    c.particles.resize(n);
+    c.indices.resize(n);
+    for (size_t i = 0; i < n; ++i) {
+        nl[i] = lr[v[i].first];
+        c.particles[i] = PD(nl[i].h, v[i].second);
+        c.indices[i] = nl[i].h;
+    }
+    lr.swap(nl);
+    /* radii (Ops::radii) from the local records */
+    const double tenth = lr[n / 2].rho / 10.;
+    size_t i = 0;
+    for (; i < n; ++i)
+        if (lr[i].rho < tenth) break;
+    c.inner = (i < n) ? c.particles[i].second : c.particles.back().second;
+    size_t j = n;
+    while (j > 0 && !(lr[j - 1].rho > tenth)) --j;
+    c.outer = c.particles[j > 0 ? j - 1 : 0].second;
+    c.one10th = (c.inner + c.outer) / 2.;
+    double tm = 0., lim = c.total_mass * (1.0 - 0.95);
+    size_t k2 = n;
+    while (k2 > 0) {
+        tm += lr[k2 - 1].m;
+        if (tm > lim) break;
+        --k2;
+    }
+    c.two_sigma = c.particles[k2 > 0 ? k2 - 1 : 0].second;
+    if (c.inner > 0 && c.outer / c.inner > 2.5) c.one10th = std::min(c.outer, c.two_sigma);
+    else c.one10th = std::min(c.one10th, c.two_sigma);
+}
+
 /* The do-while of SearchBoundParticlesWithinHillRadius + CalculateHillRadius
  * for one clump (src/tree.hpp:4201-4208, 4678-4711).  Each iteration's ball
  * (COM, R_Hill) is cut from the superset with the reference's own distance
@@ -617,6 +702,10 @@ static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
             }
     };
     rebuild();
+    std::vector<LRec> lr;
+    SortWork sw;
+    lr.reserve(c.indices.size() + 64);
+    for (uint32_t h : c.indices) lr.push_back(lrec_of(hm, h));
     std::vector<uint32_t> cand, cpos;
     std::vector<uint8_t> dead;
     size_t before;
@@ -690,6 +779,7 @@ static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
                 for (int k = 0; k < 3; ++k) c.vel[k] = (c.total_mass * c.vel[k] + q.m * qv[k]) / c.total_mass;
                 mine.insert(h);
                 rb.claimed.push_back(h);
+                lr.push_back(lrec_of(hm, h));
                 if (!exact) dead[cpos[ci]] = 1;
                 any = true;
             }
@@ -709,8 +799,7 @@ static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
         }
         if (c.indices.size() > c.particles.size()) {
             auto ts = std::chrono::steady_clock::now();
-            std::vector<PD>().swap(c.particles);
-            ops.sort_particles(c, true);
+            sort_local(c, lr, sw);
             rb.ms_sort += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ts).count();
         }
         ops.hill_radius(c);

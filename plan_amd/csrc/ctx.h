@@ -31,6 +31,7 @@ struct DevTree {
     uint32_t *sinkcnt = nullptr;  /* members merged into this particle (1 = not a sink) */
     uint32_t *sinkoff = nullptr;  /* first member in sorted order (for sink expansion) */
     uint4 *keys = nullptr;        /* sorted unique keys {w0, w1, w2, first sorted slot} */
+    size_t node_cap = 0;          /* capacity of the node arrays (grow-only) */
     NodeGeo *ngeo = nullptr;
     NodeTop *ntop = nullptr;
     uint32_t *ninfo = nullptr;    /* num_daughters | level << 8 | orthant << 16 */

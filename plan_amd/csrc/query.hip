@@ -373,7 +373,10 @@ struct ListEnt {
     uint32_t begin, end;
     uint64_t mask;
 };
-static constexpr int LCAP = 256;
+/* include-list capacity (entries) of one wave: larger K walks larger spheres */
+template <int K> struct ListCap {
+    static constexpr int value = K >= 64 ? 512 : 256;
+};
 
 struct WalkState {
     uint32_t nd, cov_end, lo, hi;
@@ -383,8 +386,10 @@ struct WalkState {
  * the walk is complete, false when the list is full; the node that did not
  * fit is re-evaluated on resume (lane state is only updated after it is
  * recorded). */
+template <int CAP>
 __device__ __forceinline__ bool walk_fill(const TreeView &tv, NodeCache &nc, WalkState &w, double qx, double qy,
-                                          double qz, double r, bool geo, ListEnt *lst, uint32_t &nent)
+                                          double qz, double r, bool geo, ListEnt *lst, uint32_t &nent,
+                                          uint32_t *scnt = nullptr, uint32_t *ccnt = nullptr)
 {
     const int lane = threadIdx.x;
     uint32_t guard = 0;
@@ -396,22 +401,29 @@ __device__ __forceinline__ bool walk_fill(const TreeView &tv, NodeCache &nc, Wal
         const NodeTop t = nc.t[s];
         const bool leaf = (nc.info[s] & 0xffu) == 0u;
         int act = 0;
+        bool whole = false;
         if (t.begin >= w.cov_end) {
             if (geo) {
                 if (geo_contain(g, qx, qy, qz, r, tv.margin_abs)) act = 1;
                 else if (geo_intersect(g, qx, qy, qz, r, tv.margin_abs)) act = leaf ? 1 : 2;
                 else act = 3;
             } else {
-                if (sphere_contain(g, qx, qy, qz, r, tv.to_diag)) act = 1;
-                else if (sphere_intersect(g, qx, qy, qz, r, tv.to_diag)) act = leaf ? 1 : 2;
+                if (sphere_contain(g, qx, qy, qz, r, tv.to_diag)) {
+                    act = 1;
+                    whole = true;
+                } else if (sphere_intersect(g, qx, qy, qz, r, tv.to_diag)) act = leaf ? 1 : 2;
                 else act = 3;
             }
         }
         const uint64_t inc = __ballot(act == 1);
         if (inc) {
-            if (nent == LCAP) return false;
+            if (nent == (uint32_t)CAP) return false;
             if (lane == 0) lst[nent] = ListEnt{t.begin, t.end, inc};
             ++nent;
+            if (scnt && act == 1) {
+                *scnt += t.end - t.begin;
+                if (whole) *ccnt += t.end - t.begin;
+            }
         }
         if (act == 1 || act == 3) w.cov_end = t.end;
         w.nd = __ballot(act == 2) ? t.first_daughter : nc.skip[s];
@@ -499,11 +511,11 @@ __device__ __forceinline__ void merge64(double &bd, uint32_t &bi, double cd, uin
 
 /* LDS of one select wave: node cache, the include list of the walk and the
  * compacted entries of the query being selected */
-struct SelectLds {
+template <int CAP> struct SelectLds {
     NodeCache nc;
-    ListEnt lst[LCAP];
-    uint32_t sbeg[LCAP];
-    uint32_t spref[LCAP + 1];
+    ListEnt lst[CAP];
+    uint32_t sbeg[CAP];
+    uint32_t spref[CAP + 1];
 };
 
 /* Feed the candidates of query bit j of the recorded list to the running
@@ -511,7 +523,8 @@ struct SelectLds {
  * then candidates are taken 64 at a time (lane c of a chunk locates its
  * particle by binary search over the prefix), measured, filtered by T,
  * sorted and merged. */
-__device__ __forceinline__ void select_list(const TreeView &tv, SelectLds &L, uint32_t nent, int j, double qx,
+template <class LDS>
+__device__ __forceinline__ void select_list(const TreeView &tv, LDS &L, uint32_t nent, int j, double qx,
                                             double qy, double qz, double T, double &bd, uint32_t &bi, bool &first)
 {
     const int lane = threadIdx.x & 63;
@@ -658,14 +671,14 @@ __device__ __forceinline__ void finish_query(const TreeView &tv, double bd, uint
  * A wave walks the tree once in lock step for its 64 queries (lane =
  * query), recording the nodes some lane takes whole; then it selects for one
  * query at a time, all 64 lanes on that query's candidates.
- * RESUME = false: a wave whose list overflows LCAP entries flags itself
+ * RESUME = false: a wave whose list overflows its capacity flags itself
  * (ovf[wave]) and stops.  RESUME = true: the flagged waves (waves[block])
  * walk in several list fills; each query's K best so far are kept in HBM
  * (sd2/sidx, K per query) between fills. */
-template <int K, bool RESUME>
+template <int K, bool RESUME, bool FUSE>
 __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, const uint32_t *__restrict__ qidx,
-                                                   const double4 *__restrict__ qpts, const double *__restrict__ r_in,
-                                                   const uint32_t *__restrict__ c_in, const double *__restrict__ rt_in,
+                                                   const double4 *__restrict__ qpts, double *__restrict__ r_io,
+                                                   uint32_t *__restrict__ c_io, const double *__restrict__ rt_in,
                                                    uint32_t q0, const uint32_t *__restrict__ waves,
                                                    uint32_t *__restrict__ ovf, double *__restrict__ sd2,
                                                    uint32_t *__restrict__ sidx, int mode, int kernel,
@@ -673,23 +686,60 @@ __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, con
                                                    uint32_t *__restrict__ densest, uint32_t *__restrict__ idx_out,
                                                    double *__restrict__ d2_out)
 {
-    __shared__ SelectLds L;
+    constexpr int CAP = ListCap<K>::value;
+    __shared__ SelectLds<CAP> L;
     const int lane = threadIdx.x;
     const uint32_t wv = RESUME ? waves[blockIdx.x] : blockIdx.x;
     const uint32_t q = wv * 64 + lane;
     const bool valid = q < nq;
     double qx = 0, qy = 0, qz = 0, r = 0;
     uint32_t self = 0;
-    if (valid) query_setup<K>(tv, q, qidx, qpts, r_in, qx, qy, qz, r, self, q0);
+    if (valid) query_setup<K>(tv, q, qidx, qpts, FUSE ? nullptr : r_io, qx, qy, qz, r, self, q0);
+    const double inf = __longlong_as_double(0x7ff0000000000000ll);
+    const uint32_t nvalid = (uint32_t)__popcll(__ballot(valid));
+    uint32_t cin = 0;
+    WalkState w{0u, valid ? 0u : 0xffffffffu, 1u, 0u};
+    uint32_t nent = 0;
+    bool filled = false;
+    if (FUSE) {
+        /* the radius schedule r0, 2 r0, ... until |S(r)| >= K
+         * (src/tree.hpp:3206-3223), each round a recording walk; the list of
+         * the last round is the one selected from */
+        bool need = valid;
+        for (int round = 0; round < 2200; ++round) {
+            uint32_t sc = 0, cc = 0;
+            w = WalkState{0u, valid ? 0u : 0xffffffffu, 1u, 0u};
+            nent = 0;
+            bool fin = true;
+            if (!RESUME) fin = walk_fill<CAP>(tv, L.nc, w, qx, qy, qz, r, false, L.lst, nent, &sc, &cc);
+            else walk_count(tv, L.nc, qx, qy, qz, r, valid, sc, cc);
+            if (!fin) {
+                if (lane == 0) ovf[wv] = 1u;
+                return;
+            }
+            need = valid && sc < (uint32_t)K;
+            if (!need) cin = cc;
+            if (!__ballot(need)) break;
+            __syncthreads();
+            if (need) r = __dmul_rn(r, 2.0);
+        }
+        filled = !RESUME;
+        if (valid) {
+            r_io[q] = r;
+            c_io[q] = cin;
+        }
+        w = WalkState{0u, valid ? 0u : 0xffffffffu, 1u, 0u};
+    } else if (valid) {
+        cin = c_io[q];
+    }
     /* Candidates in nodes the sphere contains lie within r* (a contained
      * node's box is inside the sphere with a margin far above the Morton
      * quantisation); when there are >= K of them the K nearest are within
      * r*, and only d^2 <= (r* + 1e-9)^2 can enter the list.  A point query
      * with a tight radius (k_point_tight) has >= K particles within r_t and
      * takes the K nearest within r_t. */
-    const double inf = __longlong_as_double(0x7ff0000000000000ll);
     double T = inf;
-    if (valid && c_in[q] >= (uint32_t)K) {
+    if (valid && cin >= (uint32_t)K) {
         const double rr = __dadd_rn(r, 1e-9);
         T = __dmul_rn(rr, rr);
     }
@@ -698,14 +748,14 @@ __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, con
         r = rt_in[q];
         T = r * r;
     }
-    WalkState w{0u, valid ? 0u : 0xffffffffu, 1u, 0u};
-    const uint32_t nvalid = (uint32_t)__popcll(__ballot(valid));
     if (!RESUME) {
-        uint32_t nent = 0;
-        const bool fin = walk_fill(tv, L.nc, w, qx, qy, qz, r, geo, L.lst, nent);
-        if (!fin) {
-            if (lane == 0) ovf[wv] = 1u;
-            return;
+        if (!filled) {
+            nent = 0;
+            const bool fin = walk_fill<CAP>(tv, L.nc, w, qx, qy, qz, r, geo, L.lst, nent);
+            if (!fin) {
+                if (lane == 0) ovf[wv] = 1u;
+                return;
+            }
         }
         __syncthreads();
         for (uint32_t j = 0; j < nvalid; ++j) {
@@ -727,8 +777,8 @@ __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, con
             }
         bool fin = false;
         while (!fin) {
-            uint32_t nent = 0;
-            fin = walk_fill(tv, L.nc, w, qx, qy, qz, r, geo, L.lst, nent);
+            nent = 0;
+            fin = walk_fill<CAP>(tv, L.nc, w, qx, qy, qz, r, geo, L.lst, nent);
             __syncthreads();
             for (uint32_t j = 0; j < nvalid; ++j) {
                 double bd = lane < K ? bd2[j * K + lane] : inf;
@@ -752,61 +802,67 @@ __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, con
     }
 }
 
+template <int K, bool FUSE>
+static void launch_select(const TreeView &tv, Arena &a, uint32_t nb, const uint32_t *qi, const double4 *qp,
+                          double *rbuf, uint32_t *cbuf, const double *rt, uint32_t q0, int mode, int kernel,
+                          const double *rho_in, double *rho_o, uint32_t *dn_o, uint32_t *idx_o, double *d2_o,
+                          KnnOut &out, hipStream_t s)
+{
+    const uint32_t nw = ceil_div(nb, 64);
+    uint32_t *ovf = a.get_t<uint32_t>("knn_ovf", (size_t)nw + 1);
+    uint32_t *opos = a.get_t<uint32_t>("knn_opos", (size_t)nw + 1);
+    uint32_t *otot = a.get_t<uint32_t>("knn_otot", 4);
+    HIPCHK(hipMemsetAsync(ovf, 0, (size_t)nw * 4, s));
+    if (out.ev) HIPCHK(hipEventRecord(out.ev[1], s));
+    hipLaunchKernelGGL((k_knn_select<K, false, FUSE>), dim3(nw), dim3(64), 0, s, tv, nb, qi, qp, rbuf, cbuf, rt, q0,
+                       nullptr, ovf, nullptr, nullptr, mode, kernel, rho_in, rho_o, dn_o, idx_o, d2_o);
+    LAUNCHCHK();
+    if (out.ev) HIPCHK(hipEventRecord(out.ev[2], s));
+    scan_exclusive_u32(a, ovf, opos, nw, s, otot);
+    uint32_t novf = 0;
+    HIPCHK(hipMemcpyAsync(&novf, otot, 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (novf) {
+        uint32_t *waves = a.get_t<uint32_t>("knn_owaves", novf);
+        double *sd2 = a.get_t<double>("knn_osd2", (size_t)novf * 64 * K);
+        uint32_t *sidx = a.get_t<uint32_t>("knn_osidx", (size_t)novf * 64 * K);
+        hipLaunchKernelGGL(k_ovf_compact, dim3(ceil_div(nw, 256)), dim3(256), 0, s, ovf, opos, nw, waves);
+        LAUNCHCHK();
+        hipLaunchKernelGGL((k_knn_select<K, true, FUSE>), dim3(novf), dim3(64), 0, s, tv, nb, qi, qp, rbuf, cbuf, rt,
+                           q0, waves, ovf, sd2, sidx, mode, kernel, rho_in, rho_o, dn_o, idx_o, d2_o);
+        LAUNCHCHK();
+    }
+    if (out.novf) *out.novf += novf;
+}
+
+/* Particle queries with the tree's start radius run the radius schedule inside
+ * the select kernel; point queries (and particle queries with a caller's
+ * start radius) run k_point_tight / k_knn_radius first. */
 template <int K>
 static void launch_knn(const DevTree &t, const TreeView &tv, uint32_t nq, const uint32_t *qidx, const double4 *qpts,
                        const double *est, int mode, int kernel, const double *rho_in, KnnOut out, hipStream_t s)
 {
-    const uint32_t batch = (uint32_t)std::min<uint64_t>(nq, (uint64_t)(1u << 24));
     Arena &a = const_cast<DevTree &>(t).arena;
     double *rbuf = a.get_t<double>("knn_r", (size_t)nq);
     uint32_t *cbuf = a.get_t<uint32_t>("knn_c", (size_t)nq);
-    double *rt = nullptr;
+    const bool fuse = !qpts && !est;
+    const uint32_t q0 = 0u;
     if (out.ev) HIPCHK(hipEventRecord(out.ev[0], s));
-    if (qpts) {
-        rt = a.get_t<double>("knn_rt", (size_t)nq);
-        hipLaunchKernelGGL(k_point_tight<K>, dim3(ceil_div(nq, 64)), dim3(64), 0, s, tv, nq, qpts, est, rt);
-        LAUNCHCHK();
-    }
-    hipLaunchKernelGGL(k_knn_radius<K>, dim3(ceil_div(nq, 64)), dim3(64), 0, s, tv, nq, qidx, qpts, est, rt, rbuf,
-                       cbuf);
-    LAUNCHCHK();
-    for (uint32_t b0 = 0; b0 < nq; b0 += batch) {
-        const uint32_t nb = std::min(batch, nq - b0);
-        const uint32_t *qi = qidx ? qidx + b0 : nullptr;
-        const double4 *qp = qpts ? qpts + b0 : nullptr;
-        const uint32_t q0 = (qidx || qpts) ? 0u : b0; /* all-particle pass: particles b0.. */
-        const size_t o = (qidx || qpts) ? b0 : 0;     /* outputs indexed by query, else by particle */
-        double *rho_o = out.rho ? out.rho + o : nullptr;
-        uint32_t *dn_o = out.densest ? out.densest + b0 : nullptr;
-        uint32_t *idx_o = out.idx ? out.idx + (size_t)b0 * K : nullptr;
-        double *d2_o = out.d2 ? out.d2 + (size_t)b0 * K : nullptr;
-        const double *rt_b = rt ? rt + b0 : nullptr;
-        const uint32_t nw = ceil_div(nb, 64);
-        uint32_t *ovf = a.get_t<uint32_t>("knn_ovf", (size_t)nw + 1);
-        uint32_t *opos = a.get_t<uint32_t>("knn_opos", (size_t)nw + 1);
-        uint32_t *otot = a.get_t<uint32_t>("knn_otot", 4);
-        HIPCHK(hipMemsetAsync(ovf, 0, (size_t)nw * 4, s));
-        if (out.ev && b0 == 0) HIPCHK(hipEventRecord(out.ev[1], s));
-        hipLaunchKernelGGL((k_knn_select<K, false>), dim3(nw), dim3(64), 0, s, tv, nb, qi, qp, rbuf + b0, cbuf + b0,
-                           rt_b, q0, nullptr, ovf, nullptr, nullptr, mode, kernel, rho_in, rho_o, dn_o, idx_o, d2_o);
-        LAUNCHCHK();
-        if (out.ev && b0 == 0) HIPCHK(hipEventRecord(out.ev[2], s));
-        scan_exclusive_u32(a, ovf, opos, nw, s, otot);
-        uint32_t novf = 0;
-        HIPCHK(hipMemcpyAsync(&novf, otot, 4, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        if (novf) {
-            uint32_t *waves = a.get_t<uint32_t>("knn_owaves", novf);
-            double *sd2 = a.get_t<double>("knn_osd2", (size_t)novf * 64 * K);
-            uint32_t *sidx = a.get_t<uint32_t>("knn_osidx", (size_t)novf * 64 * K);
-            hipLaunchKernelGGL(k_ovf_compact, dim3(ceil_div(nw, 256)), dim3(256), 0, s, ovf, opos, nw, waves);
-            LAUNCHCHK();
-            hipLaunchKernelGGL((k_knn_select<K, true>), dim3(novf), dim3(64), 0, s, tv, nb, qi, qp, rbuf + b0,
-                               cbuf + b0, rt_b, q0, waves, ovf, sd2, sidx, mode, kernel, rho_in, rho_o, dn_o, idx_o,
-                               d2_o);
+    if (fuse) {
+        launch_select<K, true>(tv, a, nq, qidx, nullptr, rbuf, cbuf, nullptr, q0, mode, kernel, rho_in, out.rho,
+                               out.densest, out.idx, out.d2, out, s);
+    } else {
+        double *rt = nullptr;
+        if (qpts) {
+            rt = a.get_t<double>("knn_rt", (size_t)nq);
+            hipLaunchKernelGGL(k_point_tight<K>, dim3(ceil_div(nq, 64)), dim3(64), 0, s, tv, nq, qpts, est, rt);
             LAUNCHCHK();
         }
-        if (out.novf) *out.novf += novf;
+        hipLaunchKernelGGL(k_knn_radius<K>, dim3(ceil_div(nq, 64)), dim3(64), 0, s, tv, nq, qidx, qpts, est, rt,
+                           rbuf, cbuf);
+        LAUNCHCHK();
+        launch_select<K, false>(tv, a, nq, qidx, qpts, rbuf, cbuf, rt, q0, mode, kernel, rho_in, out.rho,
+                                out.densest, out.idx, out.d2, out, s);
     }
     if (out.rstop) HIPCHK(hipMemcpyAsync(out.rstop, rbuf, (size_t)nq * 8, hipMemcpyDeviceToDevice, s));
 }
