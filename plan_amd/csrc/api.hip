@@ -467,11 +467,7 @@ int plan_build_tree(plan_ctx *c, int check_pos, plan_tree_info *info)
             info->num_particles = c->tree.n;
             info->num_nodes = c->tree.num_nodes;
             info->num_leaf_nodes = c->tree.num_leaves;
-            std::vector<uint32_t> sc(c->tree.n);
-            HIPCHK(hipMemcpy(sc.data(), c->tree.sinkcnt, (size_t)c->tree.n * 4, hipMemcpyDeviceToHost));
-            uint32_t ns = 0;
-            for (uint32_t v : sc) ns += v > 1;
-            info->num_sink_particles = ns;
+            info->num_sink_particles = c->tree.num_sinks;
             info->depth = (uint32_t)c->tree.depth;
             for (int k = 0; k < 3; ++k) info->root_center[k] = c->tree.root_center[k];
             info->half_width = c->tree.half_width;

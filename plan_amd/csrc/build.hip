@@ -253,6 +253,14 @@ __global__ void k_run_starts(const uint32_t *__restrict__ flags, const uint32_t 
     if (flags[i]) start[ridx[i]] = i;
 }
 
+/* number of sink particles (runs of more than one equal key) */
+__global__ void k_count_sinks(const uint32_t *__restrict__ sinkcnt, uint32_t n, uint32_t *__restrict__ count)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t m = __ballot(i < n && sinkcnt[i] > 1u);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(count, (uint32_t)__popcll(m));
+}
+
 /* gather the unique particles in key order; runs of equal 96-bit keys become
  * one sink particle (first of the run represents; mass and momentum summed
  * in run order) */
@@ -529,7 +537,12 @@ void dev_build_tree(DevTree &t, const DevParticles &ps, const plan_derived &d, b
     LAUNCHCHK();
     HIPCHK(hipMemsetAsync(t.rho, 0, (size_t)n * 8, s));
     t.num_sinks = 0;
-    if (n != T) t.num_sinks = 1; /* exact count computed on demand by callers */
+    if (n != T) {
+        HIPCHK(hipMemsetAsync(tot + 1, 0, 4, s));
+        hipLaunchKernelGGL(k_count_sinks, dim3(ceil_div(n, BLK)), dim3(BLK), 0, s, t.sinkcnt, n, tot + 1);
+        LAUNCHCHK();
+        t.num_sinks = read_u32(tot + 1, s);
+    }
 
     /* level-synchronous node construction into grow-only node arrays */
     size_t cap = t.node_cap;

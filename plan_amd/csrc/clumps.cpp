@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdio>
 #include <exception>
+#include <limits>
 #include <stdexcept>
 #include <unordered_set>
 
@@ -167,35 +168,40 @@ struct Ops {
         }
     }
     /* std::sort of v = A ++ B by distance, where A (the first na entries) is
-     * nearly sorted: A by insertion, B by std::sort, then a linear merge.
-     * Without distance ties the sorted order is unique and equals std::sort's;
-     * on a tie (or a far-from-sorted A) std::sort runs on the original v, or,
-     * when strict, v is left unsorted and false is returned. */
+     * usually nearly sorted: A by insertion (by std::sort when that takes too
+     * many moves), B by std::sort, then a linear merge.  Without distance
+     * ties the sorted order is unique and equals std::sort's; on a tie
+     * std::sort runs on the original v, or, when strict, v is left unsorted
+     * and false is returned. */
     static bool sort_union(std::vector<PD> &v, size_t na, bool strict = false)
     {
         if (!strict && (v.size() < 64 || na < 32)) {
             std::sort(v.begin(), v.end(), pd_less);
             return true;
         }
-        std::vector<PD> a(v.begin(), v.begin() + na), b(v.begin() + na, v.end());
+        static thread_local std::vector<PD> a, b, m;
+        a.assign(v.begin(), v.begin() + na);
+        b.assign(v.begin() + na, v.end());
         size_t budget = 16 * a.size() + 1024, moves = 0;
-        bool ok = true;
-        for (size_t i = 1; i < a.size() && ok; ++i) {
+        bool far = false;
+        for (size_t i = 1; i < a.size() && !far; ++i) {
             PD x = a[i];
             size_t j = i;
             while (j > 0 && x.second < a[j - 1].second) {
                 a[j] = a[j - 1];
                 --j;
                 if (++moves > budget) {
-                    ok = false;
+                    far = true;
                     break;
                 }
             }
             a[j] = x;
         }
-        if (ok) {
+        if (far) std::sort(a.begin(), a.end(), pd_less);
+        bool ok = true;
+        {
             std::sort(b.begin(), b.end(), pd_less);
-            std::vector<PD> m(v.size());
+            m.resize(v.size());
             std::merge(a.begin(), a.end(), b.begin(), b.end(), m.begin(), pd_less);
             for (size_t i = 1; i < m.size(); ++i)
                 if (m[i].second == m[i - 1].second) {
@@ -203,7 +209,7 @@ struct Ops {
                     break;
                 }
             if (ok) {
-                v.swap(m);
+                std::copy(m.begin(), m.end(), v.begin());
                 return true;
             }
         }
@@ -592,82 +598,156 @@ static void to_records(const HostMembers &hm, const std::vector<uint32_t> &tidx,
     }
 }
 
-/* A clump's members as contiguous records in the order of c.indices, so the
- * per-iteration re-sort of the rebinding loop streams memory instead of
- * gathering from the member table. */
-struct LRec {
-    double x, y, z, m, vx, vy, vz, rho;
-    uint32_t h;
+/* A clump's members during rebinding, structure of arrays in the order they
+ * joined (the members at the start, then each claim), with the current
+ * distance order `ord` over those positions.  Only mass, centre and velocity
+ * feed back into the loop, so member lists and radii are rebuilt once at the
+ * end. */
+struct RbLocal {
+    std::vector<double> x, y, z, m, vx, vy, vz, rho, d;
+    std::vector<uint32_t> h, ord;
+    std::vector<PD> v, tmp;
+    std::vector<uint32_t> cnt;
+    size_t n_sorted = 0;
+    size_t size() const { return h.size(); }
+    void push(const MemberRec &r, uint32_t hh)
+    {
+        x.push_back(r.x);
+        y.push_back(r.y);
+        z.push_back(r.z);
+        m.push_back(r.m);
+        vx.push_back(r.vx);
+        vy.push_back(r.vy);
+        vz.push_back(r.vz);
+        rho.push_back(r.rho);
+        h.push_back(hh);
+    }
 };
 
-static LRec lrec_of(const HostMembers &hm, uint32_t h)
+/* std::sort(v) by distance for non-negative distances, in expected linear
+ * time: bucket by distance (a monotone map, so buckets are ordered), then
+ * sort inside each bucket.  Without distance ties the sorted order is
+ * unique, so this is std::sort's result; with a tie std::sort runs on the
+ * original sequence, which fixes libstdc++'s tie order. */
+static void sort_by_distance_buckets(std::vector<PD> &v, std::vector<PD> &tmp, std::vector<uint32_t> &cnt)
 {
-    const MemberRec &r = hm.rec[h];
-    return LRec{r.x, r.y, r.z, r.m, r.vx, r.vy, r.vz, r.rho, h};
+    const size_t n = v.size();
+    double dmax = 0;
+    for (const PD &p : v) dmax = std::max(dmax, p.second);
+    if (n < 64 || !(dmax > 0) || !std::isfinite(dmax)) {
+        std::sort(v.begin(), v.end(), pd_less);
+        return;
+    }
+    const size_t nb = n;
+    const double scale = (double)nb / dmax;
+    cnt.assign(nb + 1, 0);
+    auto bucket = [&](double d) {
+        const double b = d * scale;
+        return b >= (double)(nb - 1) ? nb - 1 : (size_t)b;
+    };
+    for (const PD &p : v) cnt[bucket(p.second) + 1]++;
+    for (size_t b = 0; b < nb; ++b) cnt[b + 1] += cnt[b];
+    tmp.resize(n);
+    for (const PD &p : v) tmp[cnt[bucket(p.second)]++] = p;
+    /* cnt[b] is now the end of bucket b */
+    size_t s = 0;
+    for (size_t b = 0; b < nb; ++b) {
+        const size_t e = cnt[b];
+        if (e - s > 32) {
+            std::sort(tmp.begin() + s, tmp.begin() + e, pd_less);
+        } else {
+            for (size_t i = s + 1; i < e; ++i) {
+                PD x = tmp[i];
+                size_t j = i;
+                while (j > s && x.second < tmp[j - 1].second) {
+                    tmp[j] = tmp[j - 1];
+                    --j;
+                }
+                tmp[j] = x;
+            }
+        }
+        s = e;
+    }
+    for (size_t i = 1; i < n; ++i)
+        if (tmp[i].second == tmp[i - 1].second) {
+            std::sort(v.begin(), v.end(), pd_less);
+            return;
+        }
+    v.swap(tmp);
 }
 
-/* Ops::sort_particles(c, nearly_sorted = true) on the local records, with the
- * same arithmetic in the same order (src/tree.hpp:4431-4512): kinematics in
- * member order, distances, sort, radii.  The sort works on (position,
- * distance) pairs; std::sort's tie order depends only on the input order and
- * the distances, so it matches sorting (index, distance) pairs. */
-struct SortWork {
-    std::vector<PD> v;
-    std::vector<LRec> nl;
-};
-
-static void sort_local(HClump &c, std::vector<LRec> &lr, SortWork &w)
+/* SortParticles after claims (src/tree.hpp:4431-4512 via 4705-4709):
+ * kinematics summed in member order (the last sorted order, then the claims
+ * in claim order), distances to the new centre, sort. */
+static void rb_resort(HClump &c, RbLocal &L)
 {
-    const size_t n = lr.size();
-    c.total_mass = 0;
-    for (int k = 0; k < 3; ++k) c.com[k] = c.vel[k] = 0;
-    for (const LRec &r : lr) {
-        c.total_mass += r.m;
-        c.com[0] += r.x * r.m;
-        c.com[1] += r.y * r.m;
-        c.com[2] += r.z * r.m;
-        c.vel[0] += r.vx * r.m;
-        c.vel[1] += r.vy * r.m;
-        c.vel[2] += r.vz * r.m;
-    }
-    for (int k = 0; k < 3; ++k) c.com[k] /= c.total_mass;
-    for (int k = 0; k < 3; ++k) c.vel[k] /= c.total_mass;
-    std::vector<PD> &v = w.v;
-    v.resize(n);
-    for (size_t k = 0; k < n; ++k) {
-        const LRec &r = lr[k];
-        double dx = r.x - c.com[0], dy = r.y - c.com[1], dz = r.z - c.com[2];
+    const size_t n = L.size(), ns = L.n_sorted;
+    double M = 0, cx = 0, cy = 0, cz = 0, ux = 0, uy = 0, uz = 0;
+    auto add = [&](uint32_t p) {
+        const double mp = L.m[p];
+        M += mp;
+        cx += L.x[p] * mp;
+        cy += L.y[p] * mp;
+        cz += L.z[p] * mp;
+        ux += L.vx[p] * mp;
+        uy += L.vy[p] * mp;
+        uz += L.vz[p] * mp;
+    };
+    for (size_t i = 0; i < ns; ++i) add(L.ord[i]);
+    for (size_t p = ns; p < n; ++p) add((uint32_t)p);
+    c.total_mass = M;
+    c.com[0] = cx / M;
+    c.com[1] = cy / M;
+    c.com[2] = cz / M;
+    c.vel[0] = ux / M;
+    c.vel[1] = uy / M;
+    c.vel[2] = uz / M;
+    L.d.resize(n);
+    const double ox = c.com[0], oy = c.com[1], oz = c.com[2];
+    const double *X = L.x.data(), *Y = L.y.data(), *Z = L.z.data();
+    double *D = L.d.data();
+    for (size_t p = 0; p < n; ++p) {
+        const double dx = X[p] - ox, dy = Y[p] - oy, dz = Z[p] - oz;
         double q = 0;
         q += dx * dx;
         q += dy * dy;
         q += dz * dz;
-        v[k] = PD((uint32_t)k, std::sqrt(q));
+        D[p] = std::sqrt(q);
     }
-    Ops::sort_by_distance(v, true);
-    std::vector<LRec> &nl = w.nl;
-    nl.resize(n);
+    L.v.resize(n);
+    for (size_t i = 0; i < ns; ++i) L.v[i] = PD(L.ord[i], D[L.ord[i]]);
+    for (size_t p = ns; p < n; ++p) L.v[p] = PD((uint32_t)p, D[p]);
+    sort_by_distance_buckets(L.v, L.tmp, L.cnt);
+    L.ord.resize(n);
+    for (size_t i = 0; i < n; ++i) L.ord[i] = L.v[i].first;
+    L.n_sorted = n;
+}
+
+/* member lists and radii of SortParticles (src/tree.hpp:4446-4512) from the
+ * last sort */
+static void rb_finish(HClump &c, const RbLocal &L)
+{
+    const size_t n = L.size();
     c.particles.resize(n);
     c.indices.resize(n);
     for (size_t i = 0; i < n; ++i) {
-        nl[i] = lr[v[i].first];
-        c.particles[i] = PD(nl[i].h, v[i].second);
-        c.indices[i] = nl[i].h;
+        c.particles[i] = L.v[i];
+        c.particles[i].first = L.h[L.v[i].first];
+        c.indices[i] = c.particles[i].first;
     }
-    lr.swap(nl);
-    /* radii (Ops::radii) from the local records */
-    const double tenth = lr[n / 2].rho / 10.;
+    const double tenth = L.rho[L.ord[n / 2]] / 10.;
     size_t i = 0;
     for (; i < n; ++i)
-        if (lr[i].rho < tenth) break;
+        if (L.rho[L.ord[i]] < tenth) break;
     c.inner = (i < n) ? c.particles[i].second : c.particles.back().second;
     size_t j = n;
-    while (j > 0 && !(lr[j - 1].rho > tenth)) --j;
+    while (j > 0 && !(L.rho[L.ord[j - 1]] > tenth)) --j;
     c.outer = c.particles[j > 0 ? j - 1 : 0].second;
     c.one10th = (c.inner + c.outer) / 2.;
     double tm = 0., lim = c.total_mass * (1.0 - 0.95);
     size_t k2 = n;
     while (k2 > 0) {
-        tm += lr[k2 - 1].m;
+        tm += L.m[L.ord[k2 - 1]];
         if (tm > lim) break;
         --k2;
     }
@@ -676,70 +756,151 @@ static void sort_local(HClump &c, std::vector<LRec> &lr, SortWork &w)
     else c.one10th = std::min(c.one10th, c.two_sigma);
 }
 
+/* Superset records that may still be claimed (neither flagged nor claimed by
+ * this run), ascending tree index, structure of arrays, with their distance
+ * from the superset's centre c0. */
+struct LiveSet {
+    std::vector<double> x, y, z, m, vx, vy, vz, lsd, s;
+    std::vector<uint32_t> h;
+    void clear()
+    {
+        for (auto *a : {&x, &y, &z, &m, &vx, &vy, &vz, &lsd}) a->clear();
+        h.clear();
+    }
+    void push(const MemberRec &q, uint32_t hh, double l)
+    {
+        x.push_back(q.x);
+        y.push_back(q.y);
+        z.push_back(q.z);
+        m.push_back(q.m);
+        vx.push_back(q.vx);
+        vy.push_back(q.vy);
+        vz.push_back(q.vz);
+        lsd.push_back(l);
+        h.push_back(hh);
+    }
+    size_t size() const { return h.size(); }
+    /* drop the claimed entries (lsd = +inf), keeping the order */
+    void compact()
+    {
+        size_t w = 0;
+        for (size_t k = 0; k < h.size(); ++k)
+            if (!std::isinf(lsd[k])) {
+                x[w] = x[k];
+                y[w] = y[k];
+                z[w] = z[k];
+                m[w] = m[k];
+                vx[w] = vx[k];
+                vy[w] = vy[k];
+                vz[w] = vz[k];
+                lsd[w] = lsd[k];
+                h[w] = h[k];
+                ++w;
+            }
+        for (auto *a : {&x, &y, &z, &m, &vx, &vy, &vz, &lsd}) a->resize(w);
+        h.resize(w);
+    }
+};
+
+/* The energy of a candidate against the clump's current state
+ * (src/tree.hpp:4693-4696), as one expression so batches of candidates can
+ * be evaluated ahead of the state's updates. */
+static inline double rebind_energy(const HClump &c, double sh, double G, const double *qp, const double *qv, double qm)
+{
+    const double tm = c.total_mass + qm;
+    const double a = c.com[0] - qp[0];
+    const double sv[3] = {0.0 * a, sh * a, 0.0 * a};
+    double n2 = 0;
+    for (int k = 0; k < 3; ++k) {
+        const double dv = (c.vel[k] - sv[k]) - qv[k];
+        n2 += dv * dv;
+    }
+    return (0.5 / tm) * n2 - G / Ops::dist_pts(c.com, qp);
+}
+
 /* The do-while of SearchBoundParticlesWithinHillRadius + CalculateHillRadius
  * for one clump (src/tree.hpp:4201-4208, 4678-4711).  Each iteration's ball
  * (COM, R_Hill) is cut from the superset with the reference's own distance
  * expression; a candidate within `margin` of the sphere, where the tree's
  * box tests could disagree with the plain distance test, or a ball leaving
  * the superset, sends the iteration to the exact device ball search, which
- * also refreshes the superset. */
+ * also refreshes the superset.  The margin test needs the distance only
+ * where the squared distance lies near (r +- margin)^2; elsewhere the
+ * squared distance decides it (the band is widened far beyond rounding).
+ * Candidates are tested in order; energies are evaluated in blocks against
+ * the current state, and after a claim (which changes the state) the
+ * evaluation restarts at the next candidate, so every test sees the state
+ * the serial loop would. */
 static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::vector<uint8_t> &flags,
                        const ClumpSettings &cs, const Ops &ops, ClumpServices &svc)
 {
     const double sh = cs.q * cs.Omega;
     std::unordered_set<uint32_t> mine;
-    /* superset records that are neither flagged nor claimed by this run,
-     * ascending tree index, with their distance from c0 */
-    std::vector<uint32_t> live;
-    std::vector<double> lsd;
+    LiveSet live;
     auto rebuild = [&]() {
         live.clear();
-        lsd.clear();
         for (uint32_t h : rb.sup)
-            if (!flags[h] && !mine.count(h)) {
-                live.push_back(h);
-                lsd.push_back(Ops::dist_pts(rb.c0, &hm.rec[h].x));
-            }
+            if (!flags[h] && !mine.count(h)) live.push(hm.rec[h], h, Ops::dist_pts(rb.c0, &hm.rec[h].x));
     };
     rebuild();
-    std::vector<LRec> lr;
-    SortWork sw;
-    lr.reserve(c.indices.size() + 64);
-    for (uint32_t h : c.indices) lr.push_back(lrec_of(hm, h));
+    RbLocal L;
+    for (uint32_t h : c.indices) L.push(hm.rec[h], h);
+    L.ord.resize(L.size());
+    for (size_t i = 0; i < L.size(); ++i) L.ord[i] = (uint32_t)i;
+    L.n_sorted = L.size();
+    const size_t n0 = L.size();
     std::vector<uint32_t> cand, cpos;
-    std::vector<uint8_t> dead;
+    size_t ndead = 0;
+    constexpr size_t BLK = 16;
+    const double inf = std::numeric_limits<double>::infinity();
+    double eb[BLK];
     size_t before;
     auto t_start = std::chrono::steady_clock::now();
+    auto claim = [&](uint32_t h, const double *qp, const double *qv, double qm) {
+        c.total_mass = c.total_mass + qm;
+        for (int k = 0; k < 3; ++k) c.com[k] = (c.com[k] * c.total_mass + qm * qp[k]) / c.total_mass;
+        for (int k = 0; k < 3; ++k) c.vel[k] = (c.total_mass * c.vel[k] + qm * qv[k]) / c.total_mass;
+        mine.insert(h);
+        rb.claimed.push_back(h);
+        L.push(hm.rec[h], h);
+    };
     do {
         rb.iters++;
-        before = c.indices.size();
+        before = L.size();
         const double r = c.hill, r2 = r * r;
         const double margin = cs.ball_margin_abs + 1e-12 * r;
-        bool exact = Ops::dist_pts(c.com, rb.c0) + r + margin > rb.R;
+        const double dc = Ops::dist_pts(c.com, rb.c0);
+        bool exact = dc + r + margin > rb.R;
         cand.clear();
         cpos.clear();
         if (!exact) {
             /* a member farther than this from c0 is outside the ball by more
              * than the margin (triangle inequality, generous rounding slack) */
-            const double bound = (Ops::dist_pts(c.com, rb.c0) + r + margin) * (1 + 1e-9) + 1e-300;
-            for (size_t k = 0; k < live.size(); ++k) {
-                if (lsd[k] > bound) continue;
-                const uint32_t h = live[k];
-                const MemberRec &q = hm.rec[h];
+            const double bound = (dc + r + margin) * (1 + 1e-9) + 1e-300; /* finite */
+            const double blo = r - margin > 0 ? (r - margin) * (r - margin) * (1 - 1e-9) : -1.0;
+            const double bhi = (r + margin) * (r + margin) * (1 + 1e-9);
+            const double ox = c.com[0], oy = c.com[1], oz = c.com[2];
+            const size_t nl = live.size();
+            live.s.resize(nl);
+            const double *X = live.x.data(), *Y = live.y.data(), *Z = live.z.data();
+            double *S = live.s.data();
+            for (size_t k = 0; k < nl; ++k) {
+                const double t0 = ox - X[k], t1 = oy - Y[k], t2 = oz - Z[k];
                 double s = 0;
-                double t0 = c.com[0] - q.x, t1 = c.com[1] - q.y, t2 = c.com[2] - q.z;
                 s += t0 * t0;
                 s += t1 * t1;
                 s += t2 * t2;
-                double d = std::sqrt(s);
-                if (std::fabs(d - r) <= margin) {
+                S[k] = s;
+            }
+            const double *LS = live.lsd.data();
+            for (size_t k = 0; k < nl; ++k) {
+                if (LS[k] > bound) continue;
+                const double s = S[k];
+                if (s >= blo && s <= bhi && std::fabs(std::sqrt(s) - r) <= margin) {
                     exact = true;
                     break;
                 }
-                if (s <= r2) {
-                    cand.push_back(h);
-                    cpos.push_back((uint32_t)k);
-                }
+                if (s <= r2) cpos.push_back((uint32_t)k);
             }
         }
         if (exact) {
@@ -755,55 +916,55 @@ static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
             cpos.clear();
             for (int k = 0; k < 3; ++k) rb.c0[k] = c.com[k];
             rb.R = 2.0 * r;
-        }
-        dead.assign(exact ? 0 : live.size(), 0);
-        bool any = false;
-        for (size_t ci = 0; ci < cand.size(); ++ci) {
-            const uint32_t h = cand[ci];
-            if (exact && (flags[h] || mine.count(h))) continue;
-            const MemberRec &q = hm.rec[h];
-            const double *qp = &q.x, *qv = &q.vx;
-            double tm = c.total_mass + q.m;
-            double a = c.com[0] - qp[0];
-            double sv[3] = {0.0 * a, sh * a, 0.0 * a};
-            double n2 = 0;
-            for (int k = 0; k < 3; ++k) {
-                double dv = (c.vel[k] - sv[k]) - qv[k];
-                n2 += dv * dv;
+            for (uint32_t h : cand) {
+                if (flags[h] || mine.count(h)) continue;
+                const MemberRec &q = hm.rec[h];
+                if (rebind_energy(c, sh, cs.G, &q.x, &q.vx, q.m) < 0) claim(h, &q.x, &q.vx, q.m);
             }
-            double e = (0.5 / tm) * n2 - cs.G / Ops::dist_pts(c.com, qp);
-            if (e < 0) {
-                c.indices.push_back(h);
-                c.total_mass = tm;
-                for (int k = 0; k < 3; ++k) c.com[k] = (c.com[k] * c.total_mass + q.m * qp[k]) / c.total_mass;
-                for (int k = 0; k < 3; ++k) c.vel[k] = (c.total_mass * c.vel[k] + q.m * qv[k]) / c.total_mass;
-                mine.insert(h);
-                rb.claimed.push_back(h);
-                lr.push_back(lrec_of(hm, h));
-                if (!exact) dead[cpos[ci]] = 1;
-                any = true;
-            }
-        }
-        if (exact) {
             rebuild();
-        } else if (any) {
-            size_t w = 0;
-            for (size_t k = 0; k < live.size(); ++k)
-                if (!dead[k]) {
-                    live[w] = live[k];
-                    lsd[w] = lsd[k];
-                    ++w;
+            ndead = 0;
+        } else if (!cpos.empty()) {
+            size_t i = 0;
+            const size_t nc = cpos.size();
+            while (i < nc) {
+                const size_t e = std::min(nc, i + BLK);
+                for (size_t j = i; j < e; ++j) {
+                    const uint32_t k = cpos[j];
+                    const double qp[3] = {live.x[k], live.y[k], live.z[k]};
+                    const double qv[3] = {live.vx[k], live.vy[k], live.vz[k]};
+                    eb[j - i] = rebind_energy(c, sh, cs.G, qp, qv, live.m[k]);
                 }
-            live.resize(w);
-            lsd.resize(w);
+                size_t f = e;
+                for (size_t j = i; j < e; ++j)
+                    if (eb[j - i] < 0) {
+                        f = j;
+                        break;
+                    }
+                if (f == e) {
+                    i = e;
+                    continue;
+                }
+                const uint32_t k = cpos[f];
+                const double qp[3] = {live.x[k], live.y[k], live.z[k]};
+                const double qv[3] = {live.vx[k], live.vy[k], live.vz[k]};
+                claim(live.h[k], qp, qv, live.m[k]);
+                live.lsd[k] = inf; /* beyond every bound: skipped from now on */
+                ++ndead;
+                i = f + 1;
+            }
+            if (ndead > live.size() / 4) {
+                live.compact();
+                ndead = 0;
+            }
         }
-        if (c.indices.size() > c.particles.size()) {
+        if (L.size() > L.n_sorted) {
             auto ts = std::chrono::steady_clock::now();
-            sort_local(c, lr, sw);
+            rb_resort(c, L);
             rb.ms_sort += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ts).count();
         }
         ops.hill_radius(c);
-    } while (c.indices.size() > before);
+    } while (L.size() > before);
+    if (L.size() > n0) rb_finish(c, L);
     rb.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
 }
 
