@@ -107,6 +107,35 @@ __device__ __forceinline__ bool sphere_intersect(const NodeGeo &g, double qx, do
     return md <= __dmul_rn(r, r);
 }
 
+/* The reference's node decision for one (node, query) pair without branches:
+ * 1 = take the node's particles (contained node, or intersecting leaf),
+ * 2 = descend, 3 = reject.  Same binary64 operations as sphere_contain and
+ * sphere_intersect; an axis where the query lies inside the box adds +0 to
+ * the box distance, which leaves the sum unchanged. */
+__device__ __forceinline__ int classify_sphere(const NodeGeo &g, bool leaf, double qx, double qy, double qz, double r,
+                                               double to_diag)
+{
+    const double t = __dmul_rn(g.hw, to_diag);
+    const double dx = __dsub_rn(g.cx, qx), dy = __dsub_rn(g.cy, qy), dz = __dsub_rn(g.cz, qz);
+    const double r2 = __dmul_rn(r, r);
+    const double mx = __dadd_rn(fabs(dx), t), my = __dadd_rn(fabs(dy), t), mz = __dadd_rn(fabs(dz), t);
+    const double sc = __dadd_rn(__dadd_rn(__dmul_rn(mx, mx), __dmul_rn(my, my)), __dmul_rn(mz, mz));
+    const bool contain = sc < r2;
+    const double c2c = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+    const double tt = __dadd_rn(t, r);
+    const bool far = c2c > __dmul_rn(tt, tt);
+    const bool quirk = c2c < r || c2c < g.hw;
+    const double lx = __dsub_rn(g.cx, g.hw), hx = __dadd_rn(g.cx, g.hw);
+    const double ly = __dsub_rn(g.cy, g.hw), hy = __dadd_rn(g.cy, g.hw);
+    const double lz = __dsub_rn(g.cz, g.hw), hz = __dadd_rn(g.cz, g.hw);
+    const double ex = qx < lx ? __dsub_rn(qx, lx) : (qx > hx ? __dsub_rn(qx, hx) : 0.0);
+    const double ey = qy < ly ? __dsub_rn(qy, ly) : (qy > hy ? __dsub_rn(qy, hy) : 0.0);
+    const double ez = qz < lz ? __dsub_rn(qz, lz) : (qz > hz ? __dsub_rn(qz, hz) : 0.0);
+    const double md = __dadd_rn(__dadd_rn(__dmul_rn(ex, ex), __dmul_rn(ey, ey)), __dmul_rn(ez, ez));
+    const bool inter = !far && (quirk || md <= r2);
+    return contain ? 1 : (inter ? (leaf ? 1 : 2) : 3);
+}
+
 /* Geometric node tests for exact distance queries (a particle lies within
  * m of its node's box): the whole box within r - m of q (farthest corner),
  * or some point of the box within r + m of q. */
@@ -431,6 +460,125 @@ __device__ __forceinline__ bool walk_fill(const TreeView &tv, NodeCache &nc, Wal
     return true;
 }
 
+/* LDS of the group-wise walk: a stack of sibling groups still to visit
+ * (first node, count, mask of the queries that descended into their
+ * parent), the rank -> query table of the group being evaluated and the
+ * per-child result masks. */
+constexpr int WALK_STACK = 160;
+struct DenseWalkLds {
+    uint32_t sfd[WALK_STACK];
+    uint64_t smask[WALK_STACK];
+    uint8_t snd[WALK_STACK];
+    uint32_t rk[64];
+    uint64_t inc[8];
+    uint64_t desc[8];
+};
+
+__device__ __forceinline__ uint32_t mbcnt64(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+/* The walk of RecursiveKNN_OpenMP (src/tree.hpp:3177-3193) for the queries
+ * of mask M0 (lane = query, radius r per lane), one sibling group per step:
+ * the (child, query) pairs of a group whose parent some queries descended
+ * into are packed densely over the lanes and classified, then the children
+ * some query takes go to the include list (with the mask of those queries)
+ * and the children some query descends into push their own groups.  The
+ * set of (node, query) decisions is exactly the recursive walk's.  Returns
+ * false when the list or the group stack is full. */
+template <int CAP>
+__device__ bool walk_dense(const TreeView &tv, NodeCache &nc, DenseWalkLds &D, ListEnt *lst, uint32_t &nent, double qx,
+                           double qy, double qz, double r, uint64_t M0)
+{
+    const int lane = threadIdx.x;
+    if (M0 == 0) return true;
+    __syncthreads();
+    if (lane == 0) {
+        D.sfd[0] = 0;
+        D.snd[0] = 1;
+        D.smask[0] = M0;
+    }
+    uint32_t sp = 1;
+    while (sp > 0) {
+        __syncthreads();
+        --sp;
+        const uint32_t fd = uniform_u32(D.sfd[sp]);
+        const uint32_t nd = uniform_u32(D.snd[sp]);
+        const uint64_t Mv = D.smask[sp];
+        const uint64_t M = ((uint64_t)uniform_u32((uint32_t)(Mv >> 32)) << 32) | uniform_u32((uint32_t)Mv);
+        if ((uint32_t)lane < nd) {
+            nc.g[lane] = tv.ngeo[fd + lane];
+            nc.t[lane] = tv.ntop[fd + lane];
+            nc.info[lane] = tv.ninfo[fd + lane];
+        }
+        if ((M >> lane) & 1ull) D.rk[mbcnt64(M)] = (uint32_t)lane;
+        if (lane < 8) {
+            D.inc[lane] = 0;
+            D.desc[lane] = 0;
+        }
+        __syncthreads();
+        const uint32_t cnt = (uint32_t)__popcll(M);
+        const uint32_t total = nd * cnt;
+        const float rc = 1.0f / (float)cnt;
+        for (uint32_t base = 0; base < total; base += 64) {
+            const uint32_t p = base + (uint32_t)lane;
+            const bool pv = p < total;
+            /* p / cnt, exact for p < 512 (half a step of margin) */
+            uint32_t c = (uint32_t)(((float)p + 0.5f) * rc);
+            c = pv ? c : 0u;
+            const uint32_t k = pv ? p - c * cnt : 0u;
+            const int j = (int)D.rk[k];
+            const double xj = __shfl(qx, j, 64), yj = __shfl(qy, j, 64), zj = __shfl(qz, j, 64), rj = __shfl(r, j, 64);
+            const NodeGeo g = nc.g[c];
+            const bool leaf = (nc.info[c] & 0xffu) == 0u;
+            const int act = classify_sphere(g, leaf, xj, yj, zj, rj, tv.to_diag);
+            if (pv && act == 1) atomicOr((unsigned long long *)&D.inc[c], 1ull << j);
+            if (pv && act == 2) atomicOr((unsigned long long *)&D.desc[c], 1ull << j);
+        }
+        __syncthreads();
+        uint64_t I = 0, Dm = 0;
+        NodeTop t{0u, 0u, 0u, 0u};
+        uint32_t cnd = 0;
+        if ((uint32_t)lane < nd) {
+            I = D.inc[lane];
+            Dm = D.desc[lane];
+            t = nc.t[lane];
+            cnd = nc.info[lane] & 0xffu;
+        }
+        const uint64_t mi = __ballot(I != 0ull);
+        const uint32_t ni = (uint32_t)__popcll(mi);
+        if (nent + ni > (uint32_t)CAP) return false;
+        if (I != 0ull) lst[nent + mbcnt64(mi)] = ListEnt{t.begin, t.end, I};
+        nent += ni;
+        const uint64_t md = __ballot(Dm != 0ull);
+        const uint32_t nD = (uint32_t)__popcll(md);
+        if (sp + nD > (uint32_t)WALK_STACK) return false;
+        __syncthreads();
+        if (Dm != 0ull) {
+            /* reversed, so the first child's group is popped first */
+            const uint32_t slot = sp + nD - 1u - mbcnt64(md);
+            D.sfd[slot] = t.first_daughter;
+            D.snd[slot] = (uint8_t)cnd;
+            D.smask[slot] = Dm;
+        }
+        sp += nD;
+    }
+    return true;
+}
+
+/* |S(r)| of each query from the include-list entries [e0, e1) */
+__device__ __forceinline__ uint32_t list_count(const ListEnt *lst, uint32_t e0, uint32_t e1)
+{
+    const int lane = threadIdx.x;
+    uint32_t s = 0;
+    for (uint32_t e = e0; e < e1; ++e) {
+        const ListEnt le = lst[e];
+        if ((le.mask >> lane) & 1ull) s += le.end - le.begin;
+    }
+    return s;
+}
+
 __global__ void k_ovf_compact(const uint32_t *__restrict__ ovf, const uint32_t *__restrict__ pos, uint32_t n,
                               uint32_t *__restrict__ list)
 {
@@ -466,24 +614,60 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
     return v;
 }
 
+/* lane l reads lane l ^ J: DPP for J = 1, 2 (quad permutes) and 8 (row
+ * rotate by 8), swizzle for 4 and 16, bpermute for 32 */
+template <int J> __device__ __forceinline__ uint32_t xor_lane(uint32_t v)
+{
+    if constexpr (J == 1) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);
+    else if constexpr (J == 2) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false);
+    else if constexpr (J == 4) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x101F);
+    else if constexpr (J == 8) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x128, 0xF, 0xF, false);
+    else if constexpr (J == 16) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x401F);
+    else return (uint32_t)__shfl_xor((int)v, J, 64);
+}
+template <int J> __device__ __forceinline__ double xor_lane_f64(double d)
+{
+    const uint64_t b = (uint64_t)__double_as_longlong(d);
+    const uint32_t lo = xor_lane<J>((uint32_t)b), hi = xor_lane<J>((uint32_t)(b >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+/* lanes l whose compare-exchange keeps the smaller key: for a bitonic stage
+ * (k, j) those with ((l & k) == 0) == ((l & j) == 0); for a half-cleaner
+ * (k = 0) those with (l & j) == 0 */
+constexpr uint64_t keep_low_mask(int k, int j)
+{
+    uint64_t m = 0;
+    for (int l = 0; l < 64; ++l) {
+        const bool asc = k == 0 || (l & k) == 0;
+        if (asc == ((l & j) == 0)) m |= 1ull << l;
+    }
+    return m;
+}
+/* One compare-exchange of (d^2, index) keys with lane l ^ J.  Keys of one
+ * list are distinct except for the (+inf, NONE) padding, and swapping equal
+ * keys changes nothing, so "take the partner's key" is (partner < mine) on
+ * keep-low lanes and !(partner < mine) elsewhere, without branches. */
+template <int K2, int J> __device__ __forceinline__ void cex(double &d, uint32_t &i)
+{
+    const double pd = xor_lane_f64<J>(d);
+    const uint32_t pi = xor_lane<J>(i);
+    const bool lt = (pd < d) | ((pd == d) & (pi < i));
+    const bool low = __builtin_amdgcn_inverse_ballot_w64(keep_low_mask(K2, J));
+    const bool sw = !(lt ^ low);
+    d = sw ? pd : d;
+    i = sw ? pi : i;
+}
+
 /* Bitonic sort of one (d^2, index) key per lane over the 64 lanes,
  * ascending in lane order. */
 __device__ __forceinline__ void sort64(double &d, uint32_t &i)
 {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int k = 2; k <= 64; k <<= 1) {
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            const double pd = __shfl_xor(d, j, 64);
-            const uint32_t pi = __shfl_xor(i, j, 64);
-            const bool asc = (lane & k) == 0;
-            const bool lower = (lane & j) == 0;
-            const bool sw = (lower == asc) ? key_less(pd, pi, d, i) : key_less(d, i, pd, pi);
-            d = sw ? pd : d;
-            i = sw ? pi : i;
-        }
-    }
+    cex<2, 1>(d, i);
+    cex<4, 2>(d, i); cex<4, 1>(d, i);
+    cex<8, 4>(d, i); cex<8, 2>(d, i); cex<8, 1>(d, i);
+    cex<16, 8>(d, i); cex<16, 4>(d, i); cex<16, 2>(d, i); cex<16, 1>(d, i);
+    cex<32, 16>(d, i); cex<32, 8>(d, i); cex<32, 4>(d, i); cex<32, 2>(d, i); cex<32, 1>(d, i);
+    cex<64, 32>(d, i); cex<64, 16>(d, i); cex<64, 8>(d, i); cex<64, 4>(d, i); cex<64, 2>(d, i); cex<64, 1>(d, i);
 }
 
 /* (bd, bi) and (cd, ci) sorted ascending over the lanes: (bd, bi) becomes
@@ -494,19 +678,10 @@ __device__ __forceinline__ void merge64(double &bd, uint32_t &bi, double cd, uin
     const int lane = threadIdx.x & 63;
     const double rd = __shfl(cd, 63 - lane, 64);
     const uint32_t ri = __shfl(ci, 63 - lane, 64);
-    if (key_less(rd, ri, bd, bi)) {
-        bd = rd;
-        bi = ri;
-    }
-#pragma unroll
-    for (int j = 32; j > 0; j >>= 1) {
-        const double pd = __shfl_xor(bd, j, 64);
-        const uint32_t pi = __shfl_xor(bi, j, 64);
-        const bool lower = (lane & j) == 0;
-        const bool sw = lower ? key_less(pd, pi, bd, bi) : key_less(bd, bi, pd, pi);
-        bd = sw ? pd : bd;
-        bi = sw ? pi : bi;
-    }
+    const bool t = (rd < bd) | ((rd == bd) & (ri < bi));
+    bd = t ? rd : bd;
+    bi = t ? ri : bi;
+    cex<0, 32>(bd, bi); cex<0, 16>(bd, bi); cex<0, 8>(bd, bi); cex<0, 4>(bd, bi); cex<0, 2>(bd, bi); cex<0, 1>(bd, bi);
 }
 
 /* LDS of one select wave: node cache, the include list of the walk and the
@@ -514,8 +689,13 @@ __device__ __forceinline__ void merge64(double &bd, uint32_t &bi, double cd, uin
 template <int CAP> struct SelectLds {
     NodeCache nc;
     ListEnt lst[CAP];
-    uint32_t sbeg[CAP];
-    uint32_t spref[CAP + 1];
+    union {
+        struct {
+            uint32_t sbeg[CAP];
+            uint32_t spref[CAP + 1];
+        };
+        DenseWalkLds dw;
+    };
 };
 
 /* Feed the candidates of query bit j of the recorded list to the running
@@ -554,16 +734,18 @@ __device__ __forceinline__ void select_list(const TreeView &tv, LDS &L, uint32_t
     }
     __syncthreads();
     const double inf = __longlong_as_double(0x7ff0000000000000ll);
+    uint32_t steps = 1;
+    while (steps * 2 < M) steps *= 2;
     for (uint32_t base = 0; base < C; base += 64) {
         const uint32_t c = base + lane;
         double d = inf;
         uint32_t id = PLAN_NONE;
         if (c < C) {
-            uint32_t lo = 0, hi = M; /* spref[lo] <= c < spref[hi] */
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (L.spref[mid] <= c) lo = mid;
-                else hi = mid;
+            /* largest lo < M with spref[lo] <= c (spref ascending, spref[0] = 0) */
+            uint32_t lo = 0;
+            for (uint32_t step = steps; step > 0; step >>= 1) {
+                const uint32_t t = lo + step;
+                lo = (t < M && L.spref[t] <= c) ? t : lo;
             }
             const uint32_t p = L.sbeg[lo] + (c - L.spref[lo]);
             const double4 x = tv.pm[p];
@@ -638,7 +820,10 @@ __device__ __forceinline__ void finish_query(const TreeView &tv, double bd, uint
         const int is = in ? 63 - __clzll((long long)in) : K / 2;
         const double tm = lane_f64(bd, is);
         const double w = (lane <= is) ? tv.pm[bi].w : 0.0;
-        for (int j = is; j >= 0; --j) rho = __dadd_rn(rho, lane_f64(w, j));
+        /* masses from i* down to 0; the lanes above i* add +0 to the
+         * leading zero, which changes nothing */
+#pragma unroll
+        for (int j = K - 1; j >= 0; --j) rho = __dadd_rn(rho, lane_f64(w, j));
         const double four_pi_over_three = (3.14159265358979323846 * 4.) / 3.;
         rho = __ddiv_rn(rho, __dmul_rn(four_pi_over_three, pow15(tm)));
     } else {
@@ -701,7 +886,31 @@ __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, con
     WalkState w{0u, valid ? 0u : 0xffffffffu, 1u, 0u};
     uint32_t nent = 0;
     bool filled = false;
-    if (FUSE) {
+    if (FUSE && !RESUME) {
+        /* the radius schedule r0, 2 r0, ... until |S(r)| >= K
+         * (src/tree.hpp:3206-3223): each round walks every query at its
+         * current radius; the queries short of K candidates double theirs.
+         * The list of the last round is the one selected from. */
+        const uint64_t V = __ballot(valid);
+        for (int round = 0; round < 2200; ++round) {
+            nent = 0;
+            if (!walk_dense<CAP>(tv, L.nc, L.dw, L.lst, nent, qx, qy, qz, r, V)) {
+                if (lane == 0) ovf[wv] = 1u;
+                return;
+            }
+            __syncthreads();
+            const uint32_t sc = list_count(L.lst, 0u, nent);
+            const bool need = valid && sc < (uint32_t)K;
+            if (!__ballot(need)) break;
+            if (need) r = __dmul_rn(r, 2.0);
+        }
+        filled = true;
+        if (valid) {
+            r_io[q] = r;
+            c_io[q] = 0u;
+        }
+        w = WalkState{0u, valid ? 0u : 0xffffffffu, 1u, 0u};
+    } else if (FUSE) {
         /* the radius schedule r0, 2 r0, ... until |S(r)| >= K
          * (src/tree.hpp:3206-3223), each round a recording walk; the list of
          * the last round is the one selected from */
@@ -1152,3 +1361,4 @@ void dev_ball(const DevTree &t, uint32_t nq, const double4 *centre_r, uint32_t *
 }
 
 }  // namespace plan
+
