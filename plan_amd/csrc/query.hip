@@ -266,12 +266,12 @@ template <int K>
 __global__ __launch_bounds__(64) void k_knn_radius(TreeView tv, uint32_t nq, const uint32_t *__restrict__ qidx,
                                                    const double4 *__restrict__ qpts, const double *__restrict__ est,
                                                    const double *__restrict__ rt, double *__restrict__ r_out,
-                                                   uint32_t *__restrict__ c_out)
+                                                   uint32_t *__restrict__ c_out, uint32_t qpw)
 {
     __shared__ NodeCache nc;
     const int lane = threadIdx.x;
-    const uint32_t q = blockIdx.x * 64 + lane;
-    const bool valid = q < nq;
+    const uint32_t q = blockIdx.x * qpw + lane;
+    const bool valid = (uint32_t)lane < qpw && q < nq;
     double qx = 0, qy = 0, qz = 0, r = 0;
     uint32_t self = 0;
     if (valid) query_setup<K>(tv, q, qidx, qpts, est, qx, qy, qz, r, self);
@@ -344,12 +344,13 @@ __device__ __forceinline__ uint32_t walk_geo_count(const TreeView &tv, NodeCache
  * the candidate-set path). */
 template <int K>
 __global__ __launch_bounds__(64) void k_point_tight(TreeView tv, uint32_t nq, const double4 *__restrict__ qpts,
-                                                    const double *__restrict__ est, double *__restrict__ rt_out)
+                                                    const double *__restrict__ est, double *__restrict__ rt_out,
+                                                    uint32_t qpw)
 {
     __shared__ NodeCache nc;
     const int lane = threadIdx.x;
-    const uint32_t q = blockIdx.x * 64 + lane;
-    const bool valid = q < nq;
+    const uint32_t q = blockIdx.x * qpw + lane;
+    const bool valid = (uint32_t)lane < qpw && q < nq;
     double qx = 0, qy = 0, qz = 0, e = 0, r = 0, lim = 0;
     if (valid) {
         const double4 p = qpts[q];
@@ -869,14 +870,14 @@ __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, con
                                                    uint32_t *__restrict__ sidx, int mode, int kernel,
                                                    const double *__restrict__ rho_in, double *__restrict__ rho_out,
                                                    uint32_t *__restrict__ densest, uint32_t *__restrict__ idx_out,
-                                                   double *__restrict__ d2_out)
+                                                   double *__restrict__ d2_out, uint32_t qpw)
 {
     constexpr int CAP = ListCap<K>::value;
     __shared__ SelectLds<CAP> L;
     const int lane = threadIdx.x;
     const uint32_t wv = RESUME ? waves[blockIdx.x] : blockIdx.x;
-    const uint32_t q = wv * 64 + lane;
-    const bool valid = q < nq;
+    const uint32_t q = wv * qpw + lane;
+    const bool valid = (uint32_t)lane < qpw && q < nq;
     double qx = 0, qy = 0, qz = 0, r = 0;
     uint32_t self = 0;
     if (valid) query_setup<K>(tv, q, qidx, qpts, FUSE ? nullptr : r_io, qx, qy, qz, r, self, q0);
@@ -973,7 +974,7 @@ __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, con
             bool first = true;
             select_list(tv, L, nent, (int)j, lane_f64(qx, j), lane_f64(qy, j), lane_f64(qz, j), lane_f64(T, j), bd,
                         bi, first);
-            finish_query<K>(tv, bd, bi, wv * 64 + j, lane_u32(self, j), qpts || qidx, mode, kernel, rho_in,
+            finish_query<K>(tv, bd, bi, wv * qpw + j, lane_u32(self, j), qpts || qidx, mode, kernel, rho_in,
                             rho_out, densest, idx_out, d2_out);
         }
     } else {
@@ -1005,7 +1006,7 @@ __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, con
         for (uint32_t j = 0; j < nvalid; ++j) {
             const double bd = lane < K ? bd2[j * K + lane] : inf;
             const uint32_t bi = lane < K ? bix[j * K + lane] : PLAN_NONE;
-            finish_query<K>(tv, bd, bi, wv * 64 + j, lane_u32(self, j), qpts || qidx, mode, kernel, rho_in,
+            finish_query<K>(tv, bd, bi, wv * qpw + j, lane_u32(self, j), qpts || qidx, mode, kernel, rho_in,
                             rho_out, densest, idx_out, d2_out);
         }
     }
@@ -1015,16 +1016,16 @@ template <int K, bool FUSE>
 static void launch_select(const TreeView &tv, Arena &a, uint32_t nb, const uint32_t *qi, const double4 *qp,
                           double *rbuf, uint32_t *cbuf, const double *rt, uint32_t q0, int mode, int kernel,
                           const double *rho_in, double *rho_o, uint32_t *dn_o, uint32_t *idx_o, double *d2_o,
-                          KnnOut &out, hipStream_t s)
+                          KnnOut &out, uint32_t qpw, hipStream_t s)
 {
-    const uint32_t nw = ceil_div(nb, 64);
+    const uint32_t nw = ceil_div(nb, qpw);
     uint32_t *ovf = a.get_t<uint32_t>("knn_ovf", (size_t)nw + 1);
     uint32_t *opos = a.get_t<uint32_t>("knn_opos", (size_t)nw + 1);
     uint32_t *otot = a.get_t<uint32_t>("knn_otot", 4);
     HIPCHK(hipMemsetAsync(ovf, 0, (size_t)nw * 4, s));
     if (out.ev) HIPCHK(hipEventRecord(out.ev[1], s));
     hipLaunchKernelGGL((k_knn_select<K, false, FUSE>), dim3(nw), dim3(64), 0, s, tv, nb, qi, qp, rbuf, cbuf, rt, q0,
-                       nullptr, ovf, nullptr, nullptr, mode, kernel, rho_in, rho_o, dn_o, idx_o, d2_o);
+                       nullptr, ovf, nullptr, nullptr, mode, kernel, rho_in, rho_o, dn_o, idx_o, d2_o, qpw);
     LAUNCHCHK();
     if (out.ev) HIPCHK(hipEventRecord(out.ev[2], s));
     scan_exclusive_u32(a, ovf, opos, nw, s, otot);
@@ -1038,7 +1039,7 @@ static void launch_select(const TreeView &tv, Arena &a, uint32_t nb, const uint3
         hipLaunchKernelGGL(k_ovf_compact, dim3(ceil_div(nw, 256)), dim3(256), 0, s, ovf, opos, nw, waves);
         LAUNCHCHK();
         hipLaunchKernelGGL((k_knn_select<K, true, FUSE>), dim3(novf), dim3(64), 0, s, tv, nb, qi, qp, rbuf, cbuf, rt,
-                           q0, waves, ovf, sd2, sidx, mode, kernel, rho_in, rho_o, dn_o, idx_o, d2_o);
+                           q0, waves, ovf, sd2, sidx, mode, kernel, rho_in, rho_o, dn_o, idx_o, d2_o, qpw);
         LAUNCHCHK();
     }
     if (out.novf) *out.novf += novf;
@@ -1046,7 +1047,10 @@ static void launch_select(const TreeView &tv, Arena &a, uint32_t nb, const uint3
 
 /* Particle queries with the tree's start radius run the radius schedule inside
  * the select kernel; point queries (and particle queries with a caller's
- * start radius) run k_point_tight / k_knn_radius first. */
+ * start radius) run k_point_tight / k_knn_radius first.  Particle queries
+ * are consecutive in Morton order, so 64 of them share one wave's walk;
+ * point queries (saddle points between clumps, callers' points) are
+ * scattered, so each gets a wave of its own. */
 template <int K>
 static void launch_knn(const DevTree &t, const TreeView &tv, uint32_t nq, const uint32_t *qidx, const double4 *qpts,
                        const double *est, int mode, int kernel, const double *rho_in, KnnOut out, hipStream_t s)
@@ -1056,22 +1060,23 @@ static void launch_knn(const DevTree &t, const TreeView &tv, uint32_t nq, const 
     uint32_t *cbuf = a.get_t<uint32_t>("knn_c", (size_t)nq);
     const bool fuse = !qpts && !est;
     const uint32_t q0 = 0u;
+    const uint32_t qpw = qpts ? 1u : 64u;
     if (out.ev) HIPCHK(hipEventRecord(out.ev[0], s));
     if (fuse) {
         launch_select<K, true>(tv, a, nq, qidx, nullptr, rbuf, cbuf, nullptr, q0, mode, kernel, rho_in, out.rho,
-                               out.densest, out.idx, out.d2, out, s);
+                               out.densest, out.idx, out.d2, out, qpw, s);
     } else {
         double *rt = nullptr;
         if (qpts) {
             rt = a.get_t<double>("knn_rt", (size_t)nq);
-            hipLaunchKernelGGL(k_point_tight<K>, dim3(ceil_div(nq, 64)), dim3(64), 0, s, tv, nq, qpts, est, rt);
+            hipLaunchKernelGGL(k_point_tight<K>, dim3(ceil_div(nq, qpw)), dim3(64), 0, s, tv, nq, qpts, est, rt, qpw);
             LAUNCHCHK();
         }
-        hipLaunchKernelGGL(k_knn_radius<K>, dim3(ceil_div(nq, 64)), dim3(64), 0, s, tv, nq, qidx, qpts, est, rt,
-                           rbuf, cbuf);
+        hipLaunchKernelGGL(k_knn_radius<K>, dim3(ceil_div(nq, qpw)), dim3(64), 0, s, tv, nq, qidx, qpts, est, rt,
+                           rbuf, cbuf, qpw);
         LAUNCHCHK();
         launch_select<K, false>(tv, a, nq, qidx, qpts, rbuf, cbuf, rt, q0, mode, kernel, rho_in, out.rho,
-                                out.densest, out.idx, out.d2, out, s);
+                                out.densest, out.idx, out.d2, out, qpw, s);
     }
     if (out.rstop) HIPCHK(hipMemcpyAsync(out.rstop, rbuf, (size_t)nq * 8, hipMemcpyDeviceToDevice, s));
 }
