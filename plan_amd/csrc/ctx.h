@@ -70,7 +70,8 @@ struct KnnOut {
     uint32_t *densest = nullptr; /* nq densest-neighbour ids (hop mode) */
     double *rstop = nullptr;   /* nq final radius (optional) */
     hipEvent_t *ev = nullptr;  /* 3 events: before the radius pass, around the first select launch */
-    uint32_t *novf = nullptr;  /* host counter: waves whose include list overflowed (resumable path) */
+    uint32_t *novf = nullptr;  /* host counter: waves whose include list overflowed */
+    uint32_t *nres = nullptr;  /* host counter: waves that took the resumable path */
 };
 /* mode: 0 density, 1 hop (densest neighbour), 2 lists only.
  * Queries are tree particles qidx[i] (or i when qidx is null) or explicit

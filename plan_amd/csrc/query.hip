@@ -875,7 +875,15 @@ __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, con
     constexpr int CAP = ListCap<K>::value;
     __shared__ SelectLds<CAP> L;
     const int lane = threadIdx.x;
-    const uint32_t wv = RESUME ? waves[blockIdx.x] : blockIdx.x;
+    /* RESUME: waves[] lists the waves to redo.  Otherwise waves[] (when
+     * given) lists 64-query waves to split into 64 / qpw waves each. */
+    uint32_t wv = blockIdx.x;
+    if (RESUME) {
+        wv = waves[blockIdx.x];
+    } else if (waves) {
+        const uint32_t sub = 64u / qpw;
+        wv = waves[blockIdx.x / sub] * sub + blockIdx.x % sub;
+    }
     const uint32_t q = wv * qpw + lane;
     const bool valid = (uint32_t)lane < qpw && q < nq;
     double qx = 0, qy = 0, qz = 0, r = 0;
@@ -1034,13 +1042,40 @@ static void launch_select(const TreeView &tv, Arena &a, uint32_t nb, const uint3
     HIPCHK(hipStreamSynchronize(s));
     if (novf) {
         uint32_t *waves = a.get_t<uint32_t>("knn_owaves", novf);
-        double *sd2 = a.get_t<double>("knn_osd2", (size_t)novf * 64 * K);
-        uint32_t *sidx = a.get_t<uint32_t>("knn_osidx", (size_t)novf * 64 * K);
         hipLaunchKernelGGL(k_ovf_compact, dim3(ceil_div(nw, 256)), dim3(256), 0, s, ovf, opos, nw, waves);
         LAUNCHCHK();
-        hipLaunchKernelGGL((k_knn_select<K, true, FUSE>), dim3(novf), dim3(64), 0, s, tv, nb, qi, qp, rbuf, cbuf, rt,
-                           q0, waves, ovf, sd2, sidx, mode, kernel, rho_in, rho_o, dn_o, idx_o, d2_o, qpw);
-        LAUNCHCHK();
+        uint32_t rq = qpw, nr = novf;
+        if (qpw == 64) {
+            /* a wave whose include list overflowed covers 64 neighbouring
+             * queries whose spheres span many nodes: redo it as 4 waves of
+             * 16 queries (a smaller union of spheres); only the waves that
+             * still overflow take the resumable path */
+            const uint32_t sub = 4, nw2 = nw * sub;
+            rq = 64 / sub;
+            uint32_t *ovf2 = a.get_t<uint32_t>("knn_ovf2", (size_t)nw2 + 1);
+            uint32_t *opos2 = a.get_t<uint32_t>("knn_opos2", (size_t)nw2 + 1);
+            HIPCHK(hipMemsetAsync(ovf2, 0, (size_t)nw2 * 4, s));
+            hipLaunchKernelGGL((k_knn_select<K, false, FUSE>), dim3(novf * sub), dim3(64), 0, s, tv, nb, qi, qp, rbuf,
+                               cbuf, rt, q0, waves, ovf2, nullptr, nullptr, mode, kernel, rho_in, rho_o, dn_o, idx_o,
+                               d2_o, rq);
+            LAUNCHCHK();
+            scan_exclusive_u32(a, ovf2, opos2, nw2, s, otot);
+            HIPCHK(hipMemcpyAsync(&nr, otot, 4, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            if (nr) {
+                waves = a.get_t<uint32_t>("knn_owaves2", nr);
+                hipLaunchKernelGGL(k_ovf_compact, dim3(ceil_div(nw2, 256)), dim3(256), 0, s, ovf2, opos2, nw2, waves);
+                LAUNCHCHK();
+            }
+        }
+        if (nr) {
+            double *sd2 = a.get_t<double>("knn_osd2", (size_t)nr * 64 * K);
+            uint32_t *sidx = a.get_t<uint32_t>("knn_osidx", (size_t)nr * 64 * K);
+            hipLaunchKernelGGL((k_knn_select<K, true, FUSE>), dim3(nr), dim3(64), 0, s, tv, nb, qi, qp, rbuf, cbuf, rt,
+                               q0, waves, ovf, sd2, sidx, mode, kernel, rho_in, rho_o, dn_o, idx_o, d2_o, rq);
+            LAUNCHCHK();
+        }
+        if (out.nres) *out.nres += nr;
     }
     if (out.novf) *out.novf += novf;
 }
