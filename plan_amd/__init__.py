@@ -50,7 +50,7 @@ EXPORTS = [
     "plan_knn_query", "plan_density_at", "plan_ball_query", "plan_stage_times", "plan_last_density_kernel_ms",
     "plan_keep_input", "plan_restore_input", "plan_device_memory", "plan_clump_stats",
     "plan_last_density_radius_ms", "plan_boundness_load", "plan_boundness_load_catalog", "plan_boundness_run",
-    "plan_boundness_result",
+    "plan_boundness_result", "plan_set_knn_list_limit",
 ]
 
 
@@ -96,6 +96,7 @@ def lib():
     L.plan_boundness_load_catalog.argtypes = [vp, C.POINTER(PlanCatalog)]
     L.plan_boundness_run.argtypes = [vp, C.c_int, C.c_uint32, C.POINTER(PlanBoundnessStats)]
     L.plan_boundness_result.argtypes = [vp, vp, C.POINTER(PlanBoundnessClump)]
+    L.plan_set_knn_list_limit.argtypes = [vp, C.c_uint32]
     for f in EXPORTS:
         if f in ("plan_ctx_destroy", "plan_catalog_free", "plan_last_error"):
             continue
@@ -322,6 +323,11 @@ class Context:
         v = C.c_uint64()
         self._chk(lib().plan_device_memory(self._h, C.byref(v)), "plan_device_memory")
         return v.value
+
+    def set_knn_list_limit(self, max_entries: int):
+        """Test hook: cap the per-wave KNN include list (0 = full capacity)
+        so that small inputs take the overflow paths."""
+        self._chk(lib().plan_set_knn_list_limit(self._h, int(max_entries)), "plan_set_knn_list_limit")
 
     def density_kernel_ms(self) -> float:
         v = C.c_double()

@@ -958,6 +958,13 @@ int plan_last_density_kernel_ms(plan_ctx *c, double *ms)
     return PLAN_OK;
 }
 
+int plan_set_knn_list_limit(plan_ctx *c, uint32_t max_entries)
+{
+    if (!c) return PLAN_E_ARG;
+    c->tree.list_limit = max_entries == 0 ? 0xffffffffu : (max_entries < 8 ? 8u : max_entries);
+    return PLAN_OK;
+}
+
 int plan_last_density_radius_ms(plan_ctx *c, double *ms)
 {
     if (!c || !ms) return PLAN_E_ARG;

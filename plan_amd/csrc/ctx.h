@@ -32,6 +32,7 @@ struct DevTree {
     uint32_t *sinkoff = nullptr;  /* first member in sorted order (for sink expansion) */
     uint4 *keys = nullptr;        /* sorted unique keys {w0, w1, w2, first sorted slot} */
     size_t node_cap = 0;          /* capacity of the node arrays (grow-only) */
+    uint32_t list_limit = 0xffffffffu; /* KNN include-list entries per wave (test hook) */
     NodeGeo *ngeo = nullptr;
     NodeTop *ntop = nullptr;
     uint32_t *ninfo = nullptr;    /* num_daughters | level << 8 | orthant << 16 */

@@ -39,6 +39,7 @@ struct TreeView {
     /* bound on how far a particle can sit outside its node's box (Morton
      * quantisation of the key grid), with a wide allowance for rounding */
     double margin_abs;
+    uint32_t list_limit; /* include-list entries per wave, at most the kernel's capacity */
 };
 
 static TreeView view_of(const DevTree &t, const plan_derived &d)
@@ -62,6 +63,7 @@ static TreeView view_of(const DevTree &t, const plan_derived &d)
     }
     v.cell_volume = d.cell_volume;
     v.margin_abs = 2.5 * t.epsilon;
+    v.list_limit = t.list_limit;
     return v;
 }
 
@@ -447,7 +449,7 @@ __device__ __forceinline__ bool walk_fill(const TreeView &tv, NodeCache &nc, Wal
         }
         const uint64_t inc = __ballot(act == 1);
         if (inc) {
-            if (nent == (uint32_t)CAP) return false;
+            if (nent >= min((uint32_t)CAP, tv.list_limit)) return false;
             if (lane == 0) lst[nent] = ListEnt{t.begin, t.end, inc};
             ++nent;
             if (scnt && act == 1) {
@@ -549,7 +551,7 @@ __device__ bool walk_dense(const TreeView &tv, NodeCache &nc, DenseWalkLds &D, L
         }
         const uint64_t mi = __ballot(I != 0ull);
         const uint32_t ni = (uint32_t)__popcll(mi);
-        if (nent + ni > (uint32_t)CAP) return false;
+        if (nent + ni > min((uint32_t)CAP, tv.list_limit)) return false;
         if (I != 0ull) lst[nent + mbcnt64(mi)] = ListEnt{t.begin, t.end, I};
         nent += ni;
         const uint64_t md = __ballot(Dm != 0ull);

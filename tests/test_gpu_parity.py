@@ -177,3 +177,31 @@ def test_catalog(name):
         for k in ("center_of_mass", "vel_com", "J"):
             np.testing.assert_allclose(a[k], b[k], rtol=1e-9, atol=1e-15)
     assert gc["Mp_tot"] == pytest.approx(oc["Mp_tot"], rel=1e-12)
+
+
+@pytest.mark.parametrize("limit", [32, 8])
+def test_overflow_paths(limit):
+    """Waves whose include list overflows are redone as 16-query waves and,
+    if still too long, walked in several fills.  Capping the list (a host
+    test hook) sends the density, densest-neighbour and saddle-point queries
+    of a small case down those paths; every result must stay as above."""
+    snap, ot, g = run_case("n65k")
+    ctx = g["ctx"]
+    ctx.set_knn_list_limit(limit)
+    try:
+        cat = ctx.find_planetesimals()
+        st = ctx.clump_stats()
+        dn, pk = ctx.hop_result()
+        rho = ctx.tree_particles()["rho"]
+    finally:
+        ctx.set_knn_list_limit(0)
+    assert st["density_overflow_waves"] > 0 and st["hop_overflow_waves"] > 0, st
+    orho = ot.densities()
+    rel = np.abs(rho - orho) / np.maximum(np.abs(orho), 1e-300)
+    assert rel.max() <= 1e-5 and np.mean(rho == orho) > 0.999
+    assert np.array_equal(rho, g["parts_after"]["rho"])
+    odn, opk = ot.hop()
+    assert np.array_equal(dn, odn) and np.array_equal(pk, opk)
+    oc = ot.catalog()
+    assert [c["peak_index"] for c in cat["clumps"]] == [c["peak_index"] for c in oc["clumps"]]
+    assert np.array_equal(cat["member_original_ids"], oc["member_original_ids"])
