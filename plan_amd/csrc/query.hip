@@ -109,11 +109,14 @@ __device__ __forceinline__ bool sphere_intersect(const NodeGeo &g, double qx, do
     return md <= __dmul_rn(r, r);
 }
 
-/* The reference's node decision for one (node, query) pair without branches:
- * 1 = take the node's particles (contained node, or intersecting leaf),
- * 2 = descend, 3 = reject.  Same binary64 operations as sphere_contain and
- * sphere_intersect; an axis where the query lies inside the box adds +0 to
- * the box distance, which leaves the sum unchanged. */
+/* The reference's node decision for one (node, query) pair, evaluated by
+ * all lanes of a wave: 1 = take the node's particles (contained node, or
+ * intersecting leaf), 2 = descend, 3 = reject.  Same binary64 operations as
+ * sphere_contain and sphere_intersect.  The bounding-sphere rejection, the
+ * containment test and the squared-vs-linear comparisons decide nearly every
+ * pair; the box distance is computed only when some lane of the wave needs
+ * it (a wave-uniform branch).  An axis where the query lies inside the box
+ * adds +0 to the box distance, which leaves the sum unchanged. */
 __device__ __forceinline__ int classify_sphere(const NodeGeo &g, bool leaf, double qx, double qy, double qz, double r,
                                                double to_diag)
 {
@@ -127,14 +130,18 @@ __device__ __forceinline__ int classify_sphere(const NodeGeo &g, bool leaf, doub
     const double tt = __dadd_rn(t, r);
     const bool far = c2c > __dmul_rn(tt, tt);
     const bool quirk = c2c < r || c2c < g.hw;
-    const double lx = __dsub_rn(g.cx, g.hw), hx = __dadd_rn(g.cx, g.hw);
-    const double ly = __dsub_rn(g.cy, g.hw), hy = __dadd_rn(g.cy, g.hw);
-    const double lz = __dsub_rn(g.cz, g.hw), hz = __dadd_rn(g.cz, g.hw);
-    const double ex = qx < lx ? __dsub_rn(qx, lx) : (qx > hx ? __dsub_rn(qx, hx) : 0.0);
-    const double ey = qy < ly ? __dsub_rn(qy, ly) : (qy > hy ? __dsub_rn(qy, hy) : 0.0);
-    const double ez = qz < lz ? __dsub_rn(qz, lz) : (qz > hz ? __dsub_rn(qz, hz) : 0.0);
-    const double md = __dadd_rn(__dadd_rn(__dmul_rn(ex, ex), __dmul_rn(ey, ey)), __dmul_rn(ez, ez));
-    const bool inter = !far && (quirk || md <= r2);
+    bool inter = !far && quirk;
+    const bool need_box = !far && !quirk && !contain;
+    if (__ballot(need_box)) {
+        const double lx = __dsub_rn(g.cx, g.hw), hx = __dadd_rn(g.cx, g.hw);
+        const double ly = __dsub_rn(g.cy, g.hw), hy = __dadd_rn(g.cy, g.hw);
+        const double lz = __dsub_rn(g.cz, g.hw), hz = __dadd_rn(g.cz, g.hw);
+        const double ex = qx < lx ? __dsub_rn(qx, lx) : (qx > hx ? __dsub_rn(qx, hx) : 0.0);
+        const double ey = qy < ly ? __dsub_rn(qy, ly) : (qy > hy ? __dsub_rn(qy, hy) : 0.0);
+        const double ez = qz < lz ? __dsub_rn(qz, lz) : (qz > hz ? __dsub_rn(qz, hz) : 0.0);
+        const double md = __dadd_rn(__dadd_rn(__dmul_rn(ex, ex), __dmul_rn(ey, ey)), __dmul_rn(ez, ez));
+        inter = inter || (!far && md <= r2);
+    }
     return contain ? 1 : (inter ? (leaf ? 1 : 2) : 3);
 }
 
@@ -476,6 +483,18 @@ struct DenseWalkLds {
     uint64_t inc[8];
     uint64_t desc[8];
 };
+
+/* Workgroups are dispatched round-robin over the 8 XCDs (block b runs on
+ * XCD b % 8), each with its own L2.  Neighbouring waves of Morton-ordered
+ * queries share nodes and candidate particles, so block b is given the
+ * logical wave that makes each XCD walk one contiguous range of waves. */
+constexpr uint32_t NUM_XCD = 8;
+__device__ __forceinline__ uint32_t xcd_local_wave(uint32_t b, uint32_t nb)
+{
+    const uint32_t x = b % NUM_XCD, j = b / NUM_XCD;
+    const uint32_t q = nb / NUM_XCD, rem = nb % NUM_XCD;
+    return x * q + min(x, rem) + j;
+}
 
 __device__ __forceinline__ uint32_t mbcnt64(uint64_t m)
 {
@@ -879,12 +898,13 @@ __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, con
     const int lane = threadIdx.x;
     /* RESUME: waves[] lists the waves to redo.  Otherwise waves[] (when
      * given) lists 64-query waves to split into 64 / qpw waves each. */
-    uint32_t wv = blockIdx.x;
+    const uint32_t lb = xcd_local_wave(blockIdx.x, gridDim.x);
+    uint32_t wv = lb;
     if (RESUME) {
-        wv = waves[blockIdx.x];
+        wv = waves[lb];
     } else if (waves) {
         const uint32_t sub = 64u / qpw;
-        wv = waves[blockIdx.x / sub] * sub + blockIdx.x % sub;
+        wv = waves[lb / sub] * sub + lb % sub;
     }
     const uint32_t q = wv * qpw + lane;
     const bool valid = (uint32_t)lane < qpw && q < nq;
