@@ -68,6 +68,14 @@ def lib():
                                     _u8, _d]
         L.orc_unbind_once.restype = C.c_uint32
         L.orc_unbind_once.argtypes = [vp, C.c_uint32, _d, _d, _d, _u32, _d]
+        L.orc_clump_op.restype = C.c_uint32
+        L.orc_clump_op.argtypes = [vp, C.c_int, C.c_uint32, _d, _d, _d, _d, _d, _u32, _d]
+        L.orc_rebind_candidates.restype = C.c_uint32
+        L.orc_rebind_candidates.argtypes = [vp, _d, C.c_uint32, _d, _d, _d, _d, _u8, C.c_double, _u8]
+        L.orc_merge_two.restype = C.c_uint32
+        L.orc_merge_two.argtypes = [vp, C.c_uint32, C.c_uint32, _d, _d, _d, _d, C.c_uint32, C.c_uint32, _u32, _d]
+        L.orc_clumps_bound.restype = C.c_int
+        L.orc_clumps_bound.argtypes = [vp, _d, _d]
         _lib = L
     return _lib
 
@@ -233,3 +241,69 @@ def boundness(offsets, pos, vel, mass, G, Omega=1.0, q=1.5, max_iter=64):
     lib().orc_boundness(nc, offsets, pos.ravel(), vel.ravel(), mass, G, Omega, q, max_iter, bound, out.ravel())
     return bound.astype(bool), dict(n_bound=out[:, 0].astype(np.int64), iterations=out[:, 1].astype(np.int64),
                                     E_kin=out[:, 2], E_pot=out[:, 3], mass=out[:, 4], com=out[:, 5:8])
+
+
+# ---- single clump operations (closed-form pins of the clump logic) --------
+
+_OPS = {"sort": 0, "unbind": 1, "hill": 2, "J": 3, "dispersion": 4}
+
+
+def _clump_out(n, order, out):
+    return dict(order=order[:n].copy(), M=out[0], X=out[1:4].copy(), V=out[4:7].copy(), inner=out[7],
+                outer=out[8], one10th=out[9], two_sigma=out[10], hill=out[11], J=out[12:15].copy(),
+                Jq=out[15:18].copy(), dispersion=bool(out[18]))
+
+
+def clump_op(params: dict, op: str, pos, vel, mass, rho=None, ath=None) -> dict:
+    """SortParticles and then RemoveUnboundParticles / CalculateHillRadius /
+    CalculateAngularMomentum / IsPositionDispersion2Large on one clump given
+    as arrays (members in order)."""
+    p = make_params(params)
+    pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+    n = pos.shape[0]
+    vel = np.ascontiguousarray(vel, np.float64).reshape(n, 3)
+    mass = np.ascontiguousarray(mass, np.float64)
+    rho = np.ones(n) if rho is None else np.ascontiguousarray(rho, np.float64)
+    ath = np.zeros(n) if ath is None else np.ascontiguousarray(ath, np.float64)
+    order = np.zeros(max(n, 1), np.uint32)
+    out = np.zeros(24)
+    k = lib().orc_clump_op(C.byref(p), _OPS[op], n, pos.ravel(), vel.ravel(), mass, rho, ath, order, out)
+    return _clump_out(k, order, out)
+
+
+def rebind_candidates(params: dict, state, pos, vel, mass, rho, skip, thr):
+    """The candidate loop of SearchBoundParticlesWithinHillRadius for a
+    clump state (M, X, V) and candidates in ball order."""
+    p = make_params(params)
+    st = np.ascontiguousarray(state, np.float64).copy()
+    pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+    n = pos.shape[0]
+    acc = np.zeros(max(n, 1), np.uint8)
+    lib().orc_rebind_candidates(C.byref(p), st, n, pos.ravel(),
+                                np.ascontiguousarray(vel, np.float64).ravel(),
+                                np.ascontiguousarray(mass, np.float64), np.ascontiguousarray(rho, np.float64),
+                                np.ascontiguousarray(skip, np.uint8), float(thr), acc)
+    return acc[:n].astype(bool), st
+
+
+def merge_two(params: dict, na: int, pos, vel, mass, rho, peak_a: int, peak_b: int) -> dict:
+    """MergeAnotherPlanetesimal of members [na, n) into members [0, na)."""
+    p = make_params(params)
+    pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+    n = pos.shape[0]
+    order = np.zeros(n, np.uint32)
+    out = np.zeros(24)
+    k = lib().orc_merge_two(C.byref(p), na, n - na, pos.ravel(), np.ascontiguousarray(vel, np.float64).ravel(),
+                            np.ascontiguousarray(mass, np.float64), np.ascontiguousarray(rho, np.float64),
+                            peak_a, peak_b, order, out)
+    r = _clump_out(k, order, out)
+    r["peak"] = int(out[19])
+    r["subpeak"] = int(out[20])
+    return r
+
+
+def clumps_bound(params: dict, sa, sb) -> bool:
+    """IsGravitationallyBound of two clump states (M, X[3], V[3])."""
+    p = make_params(params)
+    return bool(lib().orc_clumps_bound(C.byref(p), np.ascontiguousarray(sa, np.float64),
+                                       np.ascontiguousarray(sb, np.float64)))

@@ -783,12 +783,15 @@ static void hill_radius(Clump &c, const Ctx &X)     /* src/tree.hpp:4715-4717 */
     c.hill = std::pow((((c.total_mass * X.G) / 3.) / X.d->Omega) / X.d->Omega, 1. / 3.);
 }
 
-static void search_bound(Clump &c, Tree &T, double thr, const Ctx &X)   /* src/tree.hpp:4678-4711 */
+/* the candidate loop of SearchBoundParticlesWithinHillRadius
+ * (src/tree.hpp:4685-4704): candidates in ball order; a bound one joins and
+ * updates the state, with the reference's centre-of-mass update that uses
+ * the new total mass on both sides (src/tree.hpp:4697-4700) */
+static void rebind_candidates(Clump &c, std::vector<IPar> &P, const std::vector<uint32_t> &near, double thr,
+                              const Ctx &X)
 {
-    std::vector<uint32_t> near;
-    T.ball(c.com, c.hill, near);
     for (uint32_t idx : near) {
-        IPar &q = T.p[idx];
+        IPar &q = P[idx];
         if (q.in_clump_flag || q.new_density < thr) continue;
         double tm = c.total_mass + q.mass;
         double a = c.com[0] - q.pos[0];
@@ -806,6 +809,13 @@ static void search_bound(Clump &c, Tree &T, double thr, const Ctx &X)   /* src/t
             q.in_clump_flag = true;
         }
     }
+}
+
+static void search_bound(Clump &c, Tree &T, double thr, const Ctx &X)   /* src/tree.hpp:4678-4711 */
+{
+    std::vector<uint32_t> near;
+    T.ball(c.com, c.hill, near);
+    rebind_candidates(c, T.p, near, thr, X);
     if (c.indices.size() > c.particles.size()) {
         std::vector<PD>().swap(c.particles);
         sort_particles(c, T.p);
@@ -1477,6 +1487,149 @@ uint32_t orc_unbind_once(const plan_params *pp, uint32_t n, const double *pos, c
     out_state[0] = c.total_mass;
     for (int k = 0; k < 3; ++k) { out_state[1 + k] = c.com[k]; out_state[4 + k] = c.vel[k]; }
     return (uint32_t)c.indices.size();
+}
+
+/* ---- single clump operations on plain arrays, for the closed-form pins ----
+ * A clump is given as its members in order (positions, velocities, masses
+ * n x 3 / n, new_density and ath_density per member); op:
+ *   0 SortParticles                                    (src/tree.hpp:4446-4512)
+ *   1 SortParticles, RemoveUnboundParticles             (4615-4674)
+ *   2 SortParticles, CalculateHillRadius                (4715-4717)
+ *   3 SortParticles, CalculateHillRadius,
+ *     CalculateAngularMomentum                          (4721-4747)
+ *   4 SortParticles, IsPositionDispersion2Large(0.55)   (4516-4528)
+ * order[] receives the members (input positions) in final order; out[] =
+ * {M, X[3], V[3], inner, outer, one10th, two_sigma, hill, J[3], Jq[3],
+ * dispersion flag}.  Returns the member count. */
+static void arrays_to_clump(uint32_t n, const double *pos, const double *vel, const double *mass, const double *rho,
+                            const double *ath, std::vector<IPar> &P, Clump &c)
+{
+    P.assign(n, IPar());
+    for (uint32_t i = 0; i < n; ++i) {
+        for (int k = 0; k < 3; ++k) { P[i].pos[k] = pos[3 * i + k]; P[i].vel[k] = vel[3 * i + k]; }
+        P[i].mass = mass[i];
+        P[i].new_density = rho ? rho[i] : 1.0;
+        P[i].ath_density = ath ? ath[i] : 0.0;
+        P[i].in_clump_flag = false;
+    }
+    c = Clump();
+    for (uint32_t i = 0; i < n; ++i) c.indices.push_back(i);
+}
+
+static void clump_out(const Clump &c, double *out)
+{
+    out[0] = c.total_mass;
+    for (int k = 0; k < 3; ++k) { out[1 + k] = c.com[k]; out[4 + k] = c.vel[k]; }
+    out[7] = c.inner; out[8] = c.outer; out[9] = c.one10th; out[10] = c.two_sigma; out[11] = c.hill;
+    for (int k = 0; k < 3; ++k) { out[12 + k] = c.J[k]; out[15 + k] = c.Jq[k]; }
+}
+
+uint32_t orc_clump_op(const plan_params *pp, int op, uint32_t n, const double *pos, const double *vel,
+                      const double *mass, const double *rho, const double *ath, uint32_t *order, double *out)
+{
+    plan_derived d;
+    derive(pp, n ? n : 1, &d);
+    Ctx X;
+    X.d = &d;
+    X.G = d.grav_constant;
+    X.shear = d.q * d.Omega;
+    std::vector<IPar> P;
+    Clump c;
+    arrays_to_clump(n, pos, vel, mass, rho, ath, P, c);
+    sort_particles(c, P);
+    out[18] = 0;
+    if (op == 1) remove_unbound(c, P, X);
+    if (op == 2 || op == 3) hill_radius(c, X);
+    if (op == 3) {
+        Tree T;
+        T.p = P;
+        angular_momentum(c, T, X);
+    }
+    if (op == 4) out[18] = dispersion_too_large(c) ? 1.0 : 0.0;
+    clump_out(c, out);
+    for (size_t i = 0; i < c.indices.size(); ++i) order[i] = c.indices[i];
+    return (uint32_t)c.indices.size();
+}
+
+/* The candidate loop of SearchBoundParticlesWithinHillRadius on a clump
+ * state {M, X[3], V[3]} (state[7]) and candidates in ball order (arrays
+ * n x 3 / n; skip[i] marks in_clump_flag, rho[] against threshold thr).
+ * accepted[i] = 1 for the candidates that joined; state is updated. */
+uint32_t orc_rebind_candidates(const plan_params *pp, double *state, uint32_t n, const double *pos,
+                               const double *vel, const double *mass, const double *rho, const uint8_t *skip,
+                               double thr, uint8_t *accepted)
+{
+    plan_derived d;
+    derive(pp, 1, &d);
+    Ctx X;
+    X.d = &d;
+    X.G = d.grav_constant;
+    X.shear = d.q * d.Omega;
+    std::vector<IPar> P;
+    Clump c;
+    arrays_to_clump(n, pos, vel, mass, rho, nullptr, P, c);
+    c.indices.clear();
+    for (uint32_t i = 0; i < n; ++i) P[i].in_clump_flag = skip[i] != 0;
+    c.total_mass = state[0];
+    for (int k = 0; k < 3; ++k) { c.com[k] = state[1 + k]; c.vel[k] = state[4 + k]; }
+    std::vector<uint32_t> near(n);
+    for (uint32_t i = 0; i < n; ++i) near[i] = i;
+    rebind_candidates(c, P, near, thr, X);
+    for (uint32_t i = 0; i < n; ++i) accepted[i] = 0;
+    for (uint32_t i : c.indices) accepted[i] = 1;
+    state[0] = c.total_mass;
+    for (int k = 0; k < 3; ++k) { state[1 + k] = c.com[k]; state[4 + k] = c.vel[k]; }
+    return (uint32_t)c.indices.size();
+}
+
+/* MergeAnotherPlanetesimal (src/tree.hpp:4532-4611) of clump b (members
+ * [na, na+nb) of the arrays) into clump a (members [0, na)); peak_a / peak_b
+ * are member positions whose new_density decides the merged peak.  Each
+ * clump is first set up by SortParticles.  out as orc_clump_op, plus
+ * out[19] = merged peak (member position), out[20] = the subpeak recorded. */
+uint32_t orc_merge_two(const plan_params *pp, uint32_t na, uint32_t nb, const double *pos, const double *vel,
+                       const double *mass, const double *rho, uint32_t peak_a, uint32_t peak_b, uint32_t *order,
+                       double *out)
+{
+    plan_derived d;
+    derive(pp, na + nb, &d);
+    std::vector<IPar> P;
+    Clump all;
+    arrays_to_clump(na + nb, pos, vel, mass, rho, nullptr, P, all);
+    Clump a, b;
+    for (uint32_t i = 0; i < na; ++i) a.indices.push_back(i);
+    for (uint32_t i = na; i < na + nb; ++i) b.indices.push_back(i);
+    a.peak_index = peak_a;
+    b.peak_index = peak_b;
+    sort_particles(a, P);
+    sort_particles(b, P);
+    merge_into(a, b, P);
+    clump_out(a, out);
+    out[18] = 0;
+    out[19] = a.peak_index;
+    out[20] = a.subpeaks.empty() ? -1.0 : (double)a.subpeaks.back();
+    for (size_t i = 0; i < a.indices.size(); ++i) order[i] = a.indices[i];
+    return (uint32_t)a.indices.size();
+}
+
+/* IsGravitationallyBound (src/tree.hpp:4867-4874) of two clump states
+ * {M, X[3], V[3]} */
+int orc_clumps_bound(const plan_params *pp, const double *sa, const double *sb)
+{
+    plan_derived d;
+    derive(pp, 1, &d);
+    Ctx X;
+    X.d = &d;
+    X.G = d.grav_constant;
+    X.shear = d.q * d.Omega;
+    Clump a, b;
+    a.total_mass = sa[0];
+    b.total_mass = sb[0];
+    for (int k = 0; k < 3; ++k) {
+        a.com[k] = sa[1 + k]; a.vel[k] = sa[4 + k];
+        b.com[k] = sb[1 + k]; b.vel[k] = sb[4 + k];
+    }
+    return grav_bound(a, b, X) ? 1 : 0;
 }
 
 /* Pairwise-potential boundness: the north star's extension, which has no
