@@ -76,6 +76,8 @@ def lib():
         L.orc_merge_two.argtypes = [vp, C.c_uint32, C.c_uint32, _d, _d, _d, _d, C.c_uint32, C.c_uint32, _u32, _d]
         L.orc_clumps_bound.restype = C.c_int
         L.orc_clumps_bound.argtypes = [vp, _d, _d]
+        L.orc_filter_clumps.argtypes = [C.c_uint32, _u32, _u32, _d, _d, _d, C.c_uint, C.c_double, C.c_double, _u8]
+        L.orc_rank_clumps.argtypes = [C.c_uint32, _u32, _d, C.c_uint32, _u8, _d]
         _lib = L
     return _lib
 
@@ -307,3 +309,23 @@ def clumps_bound(params: dict, sa, sb) -> bool:
     p = make_params(params)
     return bool(lib().orc_clumps_bound(C.byref(p), np.ascontiguousarray(sa, np.float64),
                                        np.ascontiguousarray(sb, np.float64)))
+
+
+def filter_clumps(offsets, keys, mass, rho, ath, K, mass_thr, peak_thr):
+    """RemoveSmallMassAndLowPeak on clumps given as member ranges."""
+    off = np.ascontiguousarray(offsets, np.uint32)
+    nc = len(off) - 1
+    keep = np.zeros(nc, np.uint8)
+    lib().orc_filter_clumps(nc, off, np.ascontiguousarray(keys, np.uint32), np.ascontiguousarray(mass, np.float64),
+                            np.ascontiguousarray(rho, np.float64), np.ascontiguousarray(ath, np.float64), int(K),
+                            float(mass_thr), float(peak_thr), keep)
+    return keep.astype(bool)
+
+
+def rank_clumps(keys, mass, num_peaks):
+    """Ranking and cap: kept flags, Mp_tot, Mp_max."""
+    keys = np.ascontiguousarray(keys, np.uint32)
+    keep = np.zeros(len(keys), np.uint8)
+    out = np.zeros(2)
+    lib().orc_rank_clumps(len(keys), keys, np.ascontiguousarray(mass, np.float64), int(num_peaks), keep, out)
+    return keep.astype(bool), out[0], out[1]

@@ -846,6 +846,10 @@ static void angular_momentum(Clump &c, const Tree &T, const Ctx &X)   /* src/tre
     for (int k = 0; k < 3; ++k) { c.J[k] /= hu; c.Jq[k] /= hu; }
 }
 
+/* ranking and cap (src/tree.hpp:4291-4323): sort by (mass, peak index),
+ * drop the lightest beyond num_peaks, total and maximum mass */
+static void rank_and_cap(std::map<uint32_t, Clump> &cl, uint32_t num_peaks, struct Result &R);
+
 /* ------------------------------------------------------- the whole path */
 
 struct Stages {          /* per-particle intermediate results kept for tests */
@@ -863,6 +867,25 @@ struct Result {
     double diffuse = 0.55, hill_frac = 0.25;
     double Mp_max = 0, Mp_tot = 0;
 };
+
+static void rank_and_cap(std::map<uint32_t, Clump> &cl, uint32_t num_peaks, Result &R)
+{
+    if (cl.empty()) return;
+    std::vector<std::pair<uint32_t, double>> pm;
+    for (auto &it : cl) pm.push_back(std::make_pair(it.first, it.second.total_mass));
+    std::sort(pm.begin(), pm.end(), [](const std::pair<uint32_t, double> &a, const std::pair<uint32_t, double> &b) {
+        if (a.second == b.second) return a.first < b.first;
+        return a.second < b.second;
+    });
+    if (num_peaks > 0 && cl.size() > num_peaks) {
+        size_t nd = cl.size() - num_peaks;
+        for (size_t i = 0; i < nd; ++i) cl.erase(pm[i].first);
+        pm.erase(pm.begin(), pm.begin() + nd);
+    }
+    R.Mp_tot = 0.;
+    for (auto &q : pm) R.Mp_tot = R.Mp_tot + q.second;
+    R.Mp_max = pm.back().second;
+}
 
 static void remove_small_mass_low_peak(std::map<uint32_t, Clump> &cl, const Tree &T, const Result &R, unsigned K)
 {   /* RemoveSmallMassAndLowPeak (src/tree.hpp:3524-3546) */
@@ -1178,23 +1201,7 @@ static void find_planetesimals(Tree &T, const plan_params *pp, const plan_derive
     del.clear();
     /* 10 */
     for (auto &it : cl) angular_momentum(it.second, T, X);
-    /* ranking and cap (src/tree.hpp:4291-4323) */
-    if (!cl.empty()) {
-        std::vector<std::pair<uint32_t, double>> pm;
-        for (auto &it : cl) pm.push_back(std::make_pair(it.first, it.second.total_mass));
-        std::sort(pm.begin(), pm.end(), [](const std::pair<uint32_t, double> &a, const std::pair<uint32_t, double> &b) {
-            if (a.second == b.second) return a.first < b.first;
-            return a.second < b.second;
-        });
-        if (pp->num_peaks > 0 && cl.size() > pp->num_peaks) {
-            size_t nd = cl.size() - pp->num_peaks;
-            for (size_t i = 0; i < nd; ++i) cl.erase(pm[i].first);
-            pm.erase(pm.begin(), pm.begin() + nd);
-        }
-        R.Mp_tot = 0.;
-        for (auto &q : pm) R.Mp_tot = R.Mp_tot + q.second;
-        R.Mp_max = pm.back().second;
-    }
+    rank_and_cap(cl, pp->num_peaks, R);
 }
 
 } /* namespace orc */
@@ -1610,6 +1617,51 @@ uint32_t orc_merge_two(const plan_params *pp, uint32_t na, uint32_t nb, const do
     out[20] = a.subpeaks.empty() ? -1.0 : (double)a.subpeaks.back();
     for (size_t i = 0; i < a.indices.size(); ++i) order[i] = a.indices[i];
     return (uint32_t)a.indices.size();
+}
+
+/* RemoveSmallMassAndLowPeak (src/tree.hpp:3524-3546) on clumps given as
+ * member ranges [off[c], off[c+1]) of the member arrays (members in clump
+ * order; key[c] = the peak member); keep[c] = 1 for the clumps kept. */
+void orc_filter_clumps(uint32_t nc, const uint32_t *off, const uint32_t *key, const double *mass,
+                       const double *rho, const double *ath, unsigned K, double mass_thr, double peak_thr,
+                       uint8_t *keep)
+{
+    const uint32_t n = off[nc];
+    Tree T;
+    T.p.assign(n, IPar());
+    for (uint32_t i = 0; i < n; ++i) {
+        T.p[i].mass = mass[i];
+        T.p[i].new_density = rho[i];
+        T.p[i].ath_density = ath[i];
+    }
+    std::map<uint32_t, Clump> cl;
+    for (uint32_t c = 0; c < nc; ++c) {
+        Clump &q = cl[key[c]];
+        q.peak_index = key[c];
+        for (uint32_t i = off[c]; i < off[c + 1]; ++i) {
+            q.indices.push_back(i);
+            q.total_mass += mass[i];
+        }
+    }
+    Result R;
+    R.clump_mass_threshold = mass_thr;
+    R.peak_density_threshold = peak_thr;
+    remove_small_mass_low_peak(cl, T, R, K);
+    for (uint32_t c = 0; c < nc; ++c) keep[c] = cl.count(key[c]) ? 1 : 0;
+}
+
+/* ranking and cap on clumps (key, total mass); keep[c] = 1 for the kept
+ * ones; out = {Mp_tot, Mp_max} */
+void orc_rank_clumps(uint32_t nc, const uint32_t *key, const double *mass, uint32_t num_peaks, uint8_t *keep,
+                     double *out)
+{
+    std::map<uint32_t, Clump> cl;
+    for (uint32_t c = 0; c < nc; ++c) cl[key[c]].total_mass = mass[c];
+    Result R;
+    rank_and_cap(cl, num_peaks, R);
+    for (uint32_t c = 0; c < nc; ++c) keep[c] = cl.count(key[c]) ? 1 : 0;
+    out[0] = R.Mp_tot;
+    out[1] = R.Mp_max;
 }
 
 /* IsGravitationallyBound (src/tree.hpp:4867-4874) of two clump states

@@ -287,3 +287,43 @@ def test_gravitationally_bound_pair():
     assert not O.clumps_bound(P, a, fast)
     fast[6] = math.sqrt(2 * 1.5 * G / d) * 0.99
     assert O.clumps_bound(P, a, fast)
+
+
+# ---------------------------------------------------------------- filters
+
+def test_small_mass_low_peak_and_grid_density_veto():
+    """RemoveSmallMassAndLowPeak (src/tree.hpp:3524-3546): a clump goes when
+    its mass is below the trusted mass, its peak's density below the peak
+    threshold, or when none of its first min(n, K) members has a grid
+    density (ath) at or above the peak threshold (F10)."""
+    K, mthr, pthr = 4, 2.5, 30.0
+    #            kept     light    low peak  ath veto  ath beyond K  ath exactly thr
+    sizes = [5, 2, 5, 5, 6, 5]
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    n = off[-1]
+    mass = np.ones(n)
+    mass[off[1]:off[2]] = 1.0                    # 2 members -> M = 2 < 2.5
+    rho = np.full(n, 40.0)
+    rho[off[2]] = 20.0                           # peak density below 30
+    ath = np.full(n, 1.0)
+    ath[off[0] + 2] = 31.0                       # kept: a member within the first K
+    ath[off[1]] = 99.0
+    ath[off[2]] = 99.0
+    ath[off[4] + 4] = 99.0                       # only the 5th member: beyond K = 4
+    ath[off[5] + 3] = 30.0                       # >= threshold counts
+    keys = off[:-1].copy()                       # the first member is the peak
+    keep = O.filter_clumps(off, keys, mass, rho, ath, K, mthr, pthr)
+    assert list(keep) == [True, False, False, False, False, True]
+
+
+def test_ranking_and_cap():
+    """Ranking by (mass, peak index) and the num_peaks cap drop the lightest;
+    equal masses fall by the smaller peak index first."""
+    keys = [40, 10, 30, 20, 50]
+    mass = [3.0, 1.0, 2.0, 2.0, 5.0]
+    keep, tot, mx = O.rank_clumps(keys, mass, 3)
+    # order (1,10) (2,20) (2,30) (3,40) (5,50): drop peaks 10 and 20
+    assert list(keep) == [True, False, True, False, True]
+    assert tot == 10.0 and mx == 5.0
+    keep, tot, mx = O.rank_clumps(keys, mass, 0)
+    assert keep.all() and tot == 13.0 and mx == 5.0
