@@ -20,7 +20,6 @@
 #include <exception>
 #include <limits>
 #include <stdexcept>
-#include <unordered_set>
 
 namespace plan {
 
@@ -49,6 +48,58 @@ uint32_t HostMembers::rec_of_tidx(uint32_t tidx) const
     auto it = std::lower_bound(sorted_tidx.begin(), sorted_tidx.end(), tidx);
     if (it == sorted_tidx.end() || *it != tidx) return PLAN_NONE;
     return rec_of_sorted[it - sorted_tidx.begin()];
+}
+
+/* std::sort(v) by distance for non-negative distances, in expected linear
+ * time: bucket by distance (a monotone map, so buckets are ordered), then
+ * sort inside each bucket.  Without distance ties the sorted order is
+ * unique, so this is std::sort's result; with a tie std::sort runs on the
+ * original sequence, which fixes libstdc++'s tie order. */
+static void sort_by_distance_buckets(std::vector<PD> &v, std::vector<PD> &tmp, std::vector<uint32_t> &cnt)
+{
+    const size_t n = v.size();
+    double dmax = 0;
+    for (const PD &p : v) dmax = std::max(dmax, p.second);
+    if (n < 64 || !(dmax > 0) || !std::isfinite(dmax)) {
+        std::sort(v.begin(), v.end(), pd_less);
+        return;
+    }
+    const size_t nb = n;
+    const double scale = (double)nb / dmax;
+    cnt.assign(nb + 1, 0);
+    auto bucket = [&](double d) {
+        const double b = d * scale;
+        return b >= (double)(nb - 1) ? nb - 1 : (size_t)b;
+    };
+    for (const PD &p : v) cnt[bucket(p.second) + 1]++;
+    for (size_t b = 0; b < nb; ++b) cnt[b + 1] += cnt[b];
+    tmp.resize(n);
+    for (const PD &p : v) tmp[cnt[bucket(p.second)]++] = p;
+    /* cnt[b] is now the end of bucket b */
+    size_t s = 0;
+    for (size_t b = 0; b < nb; ++b) {
+        const size_t e = cnt[b];
+        if (e - s > 32) {
+            std::sort(tmp.begin() + s, tmp.begin() + e, pd_less);
+        } else {
+            for (size_t i = s + 1; i < e; ++i) {
+                PD x = tmp[i];
+                size_t j = i;
+                while (j > s && x.second < tmp[j - 1].second) {
+                    tmp[j] = tmp[j - 1];
+                    --j;
+                }
+                tmp[j] = x;
+            }
+        }
+        s = e;
+    }
+    for (size_t i = 1; i < n; ++i)
+        if (tmp[i].second == tmp[i - 1].second) {
+            std::sort(v.begin(), v.end(), pd_less);
+            return;
+        }
+    v.swap(tmp);
 }
 
 namespace {
@@ -130,13 +181,15 @@ struct Ops {
     }
     /* Sort by distance.  Without exact distance ties the sorted order is
      * unique, so an insertion sort of an almost sorted list (members re-sorted
-     * around a slightly moved centre) gives std::sort's result in O(n); on a
-     * tie, or when the list is far from sorted, std::sort runs on the original
+     * around a slightly moved centre) or the distance-bucket sort gives
+     * std::sort's result in O(n); on a tie std::sort runs on the original
      * order so libstdc++'s tie order is kept exactly. */
     static void sort_by_distance(std::vector<PD> &v, bool nearly_sorted)
     {
+        static thread_local std::vector<PD> tmp;
+        static thread_local std::vector<uint32_t> cnt;
         if (!nearly_sorted || v.size() < 32) {
-            std::sort(v.begin(), v.end(), pd_less);
+            sort_by_distance_buckets(v, tmp, cnt);
             return;
         }
         static thread_local std::vector<PD> orig;
@@ -164,7 +217,7 @@ struct Ops {
                 }
         if (!ok) {
             v.swap(orig);
-            std::sort(v.begin(), v.end(), pd_less);
+            sort_by_distance_buckets(v, tmp, cnt);
         }
     }
     /* std::sort of v = A ++ B by distance, where A (the first na entries) is
@@ -624,58 +677,6 @@ struct RbLocal {
     }
 };
 
-/* std::sort(v) by distance for non-negative distances, in expected linear
- * time: bucket by distance (a monotone map, so buckets are ordered), then
- * sort inside each bucket.  Without distance ties the sorted order is
- * unique, so this is std::sort's result; with a tie std::sort runs on the
- * original sequence, which fixes libstdc++'s tie order. */
-static void sort_by_distance_buckets(std::vector<PD> &v, std::vector<PD> &tmp, std::vector<uint32_t> &cnt)
-{
-    const size_t n = v.size();
-    double dmax = 0;
-    for (const PD &p : v) dmax = std::max(dmax, p.second);
-    if (n < 64 || !(dmax > 0) || !std::isfinite(dmax)) {
-        std::sort(v.begin(), v.end(), pd_less);
-        return;
-    }
-    const size_t nb = n;
-    const double scale = (double)nb / dmax;
-    cnt.assign(nb + 1, 0);
-    auto bucket = [&](double d) {
-        const double b = d * scale;
-        return b >= (double)(nb - 1) ? nb - 1 : (size_t)b;
-    };
-    for (const PD &p : v) cnt[bucket(p.second) + 1]++;
-    for (size_t b = 0; b < nb; ++b) cnt[b + 1] += cnt[b];
-    tmp.resize(n);
-    for (const PD &p : v) tmp[cnt[bucket(p.second)]++] = p;
-    /* cnt[b] is now the end of bucket b */
-    size_t s = 0;
-    for (size_t b = 0; b < nb; ++b) {
-        const size_t e = cnt[b];
-        if (e - s > 32) {
-            std::sort(tmp.begin() + s, tmp.begin() + e, pd_less);
-        } else {
-            for (size_t i = s + 1; i < e; ++i) {
-                PD x = tmp[i];
-                size_t j = i;
-                while (j > s && x.second < tmp[j - 1].second) {
-                    tmp[j] = tmp[j - 1];
-                    --j;
-                }
-                tmp[j] = x;
-            }
-        }
-        s = e;
-    }
-    for (size_t i = 1; i < n; ++i)
-        if (tmp[i].second == tmp[i - 1].second) {
-            std::sort(v.begin(), v.end(), pd_less);
-            return;
-        }
-    v.swap(tmp);
-}
-
 /* SortParticles after claims (src/tree.hpp:4431-4512 via 4705-4709):
  * kinematics summed in member order (the last sorted order, then the claims
  * in claim order), distances to the new centre, sort. */
@@ -818,6 +819,93 @@ static inline double rebind_energy(const HClump &c, double sh, double G, const d
     return (0.5 / tm) * n2 - G / Ops::dist_pts(c.com, qp);
 }
 
+/* The clump state a block of candidate energies is evaluated against. */
+struct EnergyState {
+    double M, cx, cy, cz, vx, vy, vz, sh, G;
+};
+
+/* rebind_energy for candidates [0, n) of a block held as arrays: the same
+ * expression element by element (written once, compiled for the baseline
+ * and for AVX2; IEEE operations give the same bits at any vector width and
+ * nothing is fused) */
+#define PLAN_ENERGY_BLOCK_BODY                                                    \
+    for (size_t j = 0; j < n; ++j) {                                              \
+        const double tm = S.M + m[j];                                             \
+        const double a = S.cx - x[j];                                             \
+        const double svx = 0.0 * a, svy = S.sh * a, svz = 0.0 * a;                \
+        double n2 = 0;                                                            \
+        const double d0 = (S.vx - svx) - vx[j];                                   \
+        n2 += d0 * d0;                                                            \
+        const double d1 = (S.vy - svy) - vy[j];                                   \
+        n2 += d1 * d1;                                                            \
+        const double d2 = (S.vz - svz) - vz[j];                                   \
+        n2 += d2 * d2;                                                            \
+        double q = 0;                                                             \
+        const double t0 = S.cx - x[j];                                            \
+        q += t0 * t0;                                                             \
+        const double t1 = S.cy - y[j];                                            \
+        q += t1 * t1;                                                             \
+        const double t2 = S.cz - z[j];                                            \
+        q += t2 * t2;                                                             \
+        e[j] = (0.5 / tm) * n2 - S.G / std::sqrt(q);                              \
+    }
+static void energy_block_base(const EnergyState &S, const double *x, const double *y, const double *z,
+                              const double *vx, const double *vy, const double *vz, const double *m, size_t n,
+                              double *e)
+{
+    PLAN_ENERGY_BLOCK_BODY
+}
+__attribute__((target("avx2"))) static void energy_block_avx2(const EnergyState &S, const double *x, const double *y,
+                                                               const double *z, const double *vx, const double *vy,
+                                                               const double *vz, const double *m, size_t n, double *e)
+{
+    PLAN_ENERGY_BLOCK_BODY
+}
+#undef PLAN_ENERGY_BLOCK_BODY
+typedef void (*EnergyBlockFn)(const EnergyState &, const double *, const double *, const double *, const double *,
+                              const double *, const double *, const double *, size_t, double *);
+static EnergyBlockFn energy_block_fn()
+{
+    static const EnergyBlockFn f = __builtin_cpu_supports("avx2") ? energy_block_avx2 : energy_block_base;
+    return f;
+}
+
+/* Open-addressing set of record handles (the particles a rebinding run
+ * claimed). */
+struct HandleSet {
+    std::vector<uint32_t> t;
+    size_t n = 0;
+    HandleSet() : t(64, PLAN_NONE) {}
+    static size_t hash(uint32_t h) { return (size_t)(h * 2654435761u); }
+    bool count(uint32_t h) const
+    {
+        const size_t mask = t.size() - 1;
+        for (size_t i = hash(h) & mask;; i = (i + 1) & mask) {
+            if (t[i] == h) return true;
+            if (t[i] == PLAN_NONE) return false;
+        }
+    }
+    void insert(uint32_t h)
+    {
+        if (2 * (n + 1) > t.size()) {
+            std::vector<uint32_t> old(t.size() * 2, PLAN_NONE);
+            old.swap(t);
+            n = 0;
+            for (uint32_t v : old)
+                if (v != PLAN_NONE) insert(v);
+        }
+        const size_t mask = t.size() - 1;
+        for (size_t i = hash(h) & mask;; i = (i + 1) & mask) {
+            if (t[i] == h) return;
+            if (t[i] == PLAN_NONE) {
+                t[i] = h;
+                ++n;
+                return;
+            }
+        }
+    }
+};
+
 /* The do-while of SearchBoundParticlesWithinHillRadius + CalculateHillRadius
  * for one clump (src/tree.hpp:4201-4208, 4678-4711).  Each iteration's ball
  * (COM, R_Hill) is cut from the superset with the reference's own distance
@@ -835,7 +923,7 @@ static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
                        const ClumpSettings &cs, const Ops &ops, ClumpServices &svc)
 {
     const double sh = cs.q * cs.Omega;
-    std::unordered_set<uint32_t> mine;
+    HandleSet mine;
     LiveSet live;
     auto rebuild = [&]() {
         live.clear();
@@ -850,10 +938,12 @@ static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
     L.n_sorted = L.size();
     const size_t n0 = L.size();
     std::vector<uint32_t> cand, cpos;
+    std::vector<uint8_t> flag;
     size_t ndead = 0;
-    constexpr size_t BLK = 16;
+    constexpr size_t BLK = 32;
     const double inf = std::numeric_limits<double>::infinity();
-    double eb[BLK];
+    double eb[BLK], bx[BLK], by[BLK], bz[BLK], bvx[BLK], bvy[BLK], bvz[BLK], bm[BLK];
+    const EnergyBlockFn energy_block = energy_block_fn();
     size_t before;
     auto t_start = std::chrono::steady_clock::now();
     auto claim = [&](uint32_t h, const double *qp, const double *qv, double qm) {
@@ -884,6 +974,10 @@ static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
             live.s.resize(nl);
             const double *X = live.x.data(), *Y = live.y.data(), *Z = live.z.data();
             double *S = live.s.data();
+            const double *LS = live.lsd.data();
+            flag.resize(nl);
+            uint8_t *F = flag.data();
+            size_t nband = 0;
             for (size_t k = 0; k < nl; ++k) {
                 const double t0 = ox - X[k], t1 = oy - Y[k], t2 = oz - Z[k];
                 double s = 0;
@@ -891,17 +985,20 @@ static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
                 s += t1 * t1;
                 s += t2 * t2;
                 S[k] = s;
+                const bool near = LS[k] <= bound;
+                const bool band = near & (s >= blo) & (s <= bhi);
+                F[k] = (uint8_t)((near & (s <= r2)) | (band << 1));
+                nband += band;
             }
-            const double *LS = live.lsd.data();
-            for (size_t k = 0; k < nl; ++k) {
-                if (LS[k] > bound) continue;
-                const double s = S[k];
-                if (s >= blo && s <= bhi && std::fabs(std::sqrt(s) - r) <= margin) {
-                    exact = true;
-                    break;
-                }
-                if (s <= r2) cpos.push_back((uint32_t)k);
-            }
+            /* a candidate within the margin of the sphere anywhere sends the
+             * iteration to the exact search (its candidates are then the
+             * device's), so the order of this test does not matter */
+            if (nband)
+                for (size_t k = 0; k < nl && !exact; ++k)
+                    if ((F[k] & 2) && std::fabs(std::sqrt(S[k]) - r) <= margin) exact = true;
+            if (!exact)
+                for (size_t k = 0; k < nl; ++k)
+                    if (F[k] & 1) cpos.push_back((uint32_t)k);
         }
         if (exact) {
             auto te = std::chrono::steady_clock::now();
@@ -930,10 +1027,16 @@ static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
                 const size_t e = std::min(nc, i + BLK);
                 for (size_t j = i; j < e; ++j) {
                     const uint32_t k = cpos[j];
-                    const double qp[3] = {live.x[k], live.y[k], live.z[k]};
-                    const double qv[3] = {live.vx[k], live.vy[k], live.vz[k]};
-                    eb[j - i] = rebind_energy(c, sh, cs.G, qp, qv, live.m[k]);
+                    bx[j - i] = live.x[k];
+                    by[j - i] = live.y[k];
+                    bz[j - i] = live.z[k];
+                    bvx[j - i] = live.vx[k];
+                    bvy[j - i] = live.vy[k];
+                    bvz[j - i] = live.vz[k];
+                    bm[j - i] = live.m[k];
                 }
+                const EnergyState es{c.total_mass, c.com[0], c.com[1], c.com[2], c.vel[0], c.vel[1], c.vel[2], sh, cs.G};
+                energy_block(es, bx, by, bz, bvx, bvy, bvz, bm, e - i, eb);
                 size_t f = e;
                 for (size_t j = i; j < e; ++j)
                     if (eb[j - i] < 0) {
@@ -998,10 +1101,11 @@ static void rebind_all(std::map<uint32_t, HClump> &cl, const HostMembers &hm, co
     st.rebind_searches += (uint32_t)qs.size();
     std::vector<Rebind> rb(nc);
     std::vector<HClump> saved(nc);
+    std::vector<std::vector<uint32_t>> suprec(nc);
     auto start = [&](size_t i) {
         *v[i] = saved[i];
         rb[i] = Rebind();
-        to_records(hm, sup[i], rb[i].sup);
+        rb[i].sup = suprec[i];
         rb[i].c0[0] = qs[i].x;
         rb[i].c0[1] = qs[i].y;
         rb[i].c0[2] = qs[i].z;
@@ -1009,11 +1113,21 @@ static void rebind_all(std::map<uint32_t, HClump> &cl, const HostMembers &hm, co
     };
     for (size_t i = 0; i < nc; ++i) saved[i] = *v[i];
     part(0);
-    /* longest first: superset size as the cost estimate */
-    std::vector<size_t> order(nc);
+    /* longest first; the cost estimate is the number of superset records a
+     * clump may still claim (above threshold, in no clump): a small clump in
+     * a rich envelope grows over hundreds of iterations */
+    std::vector<size_t> order(nc), cost(nc, 0);
+    parallel_for((long)nc, [&](long i) {
+        to_records(hm, sup[i], suprec[i]);
+        for (uint32_t h : suprec[i]) cost[i] += !in_clump[h];
+    });
     for (size_t i = 0; i < nc; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](size_t x, size_t y) { return sup[x].size() > sup[y].size(); });
+    /* a clump that can claim many times its own size grows over many
+     * iterations, each re-sorting all of its members */
+    std::vector<double> est(nc);
+    for (size_t i = 0; i < nc; ++i)
+        est[i] = (double)cost[i] * (1.0 + (double)cost[i] / (double)(v[i]->indices.size() + 1));
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return est[x] > est[y]; });
     parallel_for((long)nc, [&](long k) {
         const size_t i = order[k];
         start(i);
