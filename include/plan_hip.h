@@ -148,8 +148,10 @@ int plan_last_density_kernel_ms(plan_ctx *ctx, double *ms);
 int plan_last_density_radius_ms(plan_ctx *ctx, double *ms);
 /* Test hook, no reference counterpart: cap the KNN include list of a wave
  * at max_entries (clamped to 8 .. the on-chip capacity; 0 restores the
- * capacity) so small inputs exercise the overflow paths (16-query waves,
- * resumable multi-fill walks).  Results do not change, only speed. */
+ * capacity) and start the level-parallel ball searches with a frontier of
+ * 8 * max_entries, so small inputs exercise the overflow paths (16-query
+ * waves, resumable multi-fill walks, frontier regrowth).  Results do not
+ * change, only speed. */
 int plan_set_knn_list_limit(plan_ctx *ctx, uint32_t max_entries);
 
 /* ---- pairwise-potential boundness (extension: no reference counterpart) ----

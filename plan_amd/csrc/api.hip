@@ -962,6 +962,7 @@ int plan_set_knn_list_limit(plan_ctx *c, uint32_t max_entries)
 {
     if (!c) return PLAN_E_ARG;
     c->tree.list_limit = max_entries == 0 ? 0xffffffffu : (max_entries < 8 ? 8u : max_entries);
+    c->ctree.list_limit = c->tree.list_limit;
     return PLAN_OK;
 }
 

@@ -1233,7 +1233,11 @@ __global__ void k_bfs_expand(TreeView tv, const double4 *__restrict__ cr, const 
                              uint32_t gcap, BallTask *__restrict__ tasks, uint32_t *__restrict__ tcount, uint32_t tcap,
                              uint32_t *__restrict__ overflow)
 {
-    const uint32_t n = *fcount;
+    /* after an overflow the frontier holds fewer entries than its count says
+     * (and possibly none valid): stop, the search is redone with capacities
+     * from the counts */
+    if (*overflow) return;
+    const uint32_t n = min(*fcount, gcap);
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         const uint2 it = f[k];
         const double4 c = cr[it.x];
@@ -1329,7 +1333,9 @@ void dev_ball_lists(const DevTree &t, Arena &a, const std::vector<double4> &quer
     HIPCHK(hipMemcpyAsync(cr, queries.data(), nq * sizeof(double4), hipMemcpyHostToDevice, s));
     const int levels = t.depth + 1;
     uint32_t gcap = std::max<uint32_t>(1u << 20, 8 * nq), tcap = std::max<uint32_t>(1u << 20, 8 * nq);
-    for (int attempt = 0; attempt < 4; ++attempt) {
+    if (t.list_limit != 0xffffffffu) /* test hook: start small, exercise the regrowth */
+        gcap = tcap = std::max<uint32_t>(64u, t.list_limit * 8u);
+    for (int attempt = 0; attempt < 8; ++attempt) {
         uint2 *f0 = a.get_t<uint2>("bl_f0", gcap);
         uint2 *f1 = a.get_t<uint2>("bl_f1", gcap);
         BallTask *tasks = a.get_t<BallTask>("bl_tasks", tcap);
@@ -1345,15 +1351,24 @@ void dev_ball_lists(const DevTree &t, Arena &a, const std::vector<double4> &quer
                                tasks, cnt + 36, tcap, cnt + 37);
             LAUNCHCHK();
         }
-        uint32_t h[2];
-        HIPCHK(hipMemcpyAsync(h, cnt + 36, 8, hipMemcpyDeviceToHost, s));
+        uint32_t h[40];
+        HIPCHK(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
-        if (h[1]) {
-            gcap *= 4;
-            tcap *= 4;
+        if (h[37]) {
+            /* the counts up to the overflowing level are exact requests;
+             * grow at least 4x so a later level that was not reached fits
+             * on the next attempt more likely */
+            uint32_t need = 0;
+            for (int L = 0; L < 34; ++L) need = std::max(need, h[L]);
+            const uint64_t g2 = std::max<uint64_t>((uint64_t)gcap * 4, (uint64_t)need + need / 4);
+            const uint64_t t2 = std::max<uint64_t>((uint64_t)tcap * 4, (uint64_t)h[36] + h[36] / 4);
+            if (g2 > 0xffffffffull / 2 || t2 > 0xffffffffull / 2)
+                throw std::runtime_error("ball search: node frontier exceeds the work capacity");
+            gcap = (uint32_t)g2;
+            tcap = (uint32_t)t2;
             continue;
         }
-        const uint32_t nt = h[0];
+        const uint32_t nt = h[36];
         if (nt == 0) return;
         uint4 *k0 = a.get_t<uint4>("bl_k0", nt), *k1 = a.get_t<uint4>("bl_k1", nt);
         hipLaunchKernelGGL(k_bfs_task_keys, dim3(ceil_div(nt, 256)), dim3(256), 0, s, tasks, nt, k0);

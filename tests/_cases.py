@@ -19,4 +19,7 @@ CASES = {
     "n16k_qs": lambda: dense_case(16384, 6, 7, a=3e-4, nx=128),
     # the benchmark's mix (200 Plummer clumps in 10% of the particles) at 2^20
     "c2_1m": lambda: synth.config_c2(seed=3, n=1 << 20),
+    # the reference's CPU-runnable configuration: 262,144 particles, 16
+    # Plummer spheres in 10%, K = 32, seed 0
+    "c1": lambda: synth.config_c1(seed=0),
 }

@@ -182,9 +182,11 @@ def test_catalog(name):
 @pytest.mark.parametrize("limit", [32, 8])
 def test_overflow_paths(limit):
     """Waves whose include list overflows are redone as 16-query waves and,
-    if still too long, walked in several fills.  Capping the list (a host
-    test hook) sends the density, densest-neighbour and saddle-point queries
-    of a small case down those paths; every result must stay as above."""
+    if still too long, walked in several fills; ball searches whose node
+    frontier outgrows its buffers are redone with buffers sized from the
+    counts.  Capping both (a host test hook) sends the density,
+    densest-neighbour, saddle-point and ball searches of a small case down
+    those paths; every result must stay as above."""
     snap, ot, g = run_case("n65k")
     ctx = g["ctx"]
     ctx.set_knn_list_limit(limit)
