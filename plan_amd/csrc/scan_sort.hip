@@ -114,7 +114,7 @@ void scan_exclusive_u32(Arena &a, const uint32_t *in, uint32_t *out, uint32_t n,
 /* ----------------------------------------------------------- radix sort */
 
 static constexpr int RS_BLOCK = 256;
-static constexpr int RS_ROUNDS = 16;
+static constexpr int RS_ROUNDS = 8;
 static constexpr int RS_TILE = RS_BLOCK * RS_ROUNDS;
 
 __device__ __forceinline__ uint32_t rs_digit(const uint4 &k, int d)
@@ -139,38 +139,73 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_hist(const uint4 *__restrict__ 
     counts[(uint64_t)threadIdx.x * nb + blockIdx.x] = hist[threadIdx.x];
 }
 
+/* One stable counting-sort pass over a tile of RS_TILE keys: the keys are
+ * ranked by digit in LDS (tile order kept within a digit), then written out
+ * digit run by digit run, so consecutive threads write consecutive global
+ * positions. */
 __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const uint4 *__restrict__ keys, uint4 *__restrict__ out,
                                                          uint32_t n, int d, const uint32_t *__restrict__ offs,
                                                          uint32_t nb)
 {
+    __shared__ uint4 tile[RS_TILE];
+    __shared__ uint32_t lbase[256];
+    __shared__ uint32_t gbase[256];
     __shared__ uint32_t running[256];
+    __shared__ uint32_t wsum[RS_BLOCK / 64];
     __shared__ uint32_t wcount[RS_BLOCK / 64][256];
     const int t = threadIdx.x;
-    const int wave = t >> 6;
-    running[t] = offs[(uint64_t)t * nb + blockIdx.x];
+    const int wave = t >> 6, lane = t & 63;
+    const uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+    const uint32_t cnt = (uint32_t)((uint64_t)n - base < (uint64_t)RS_TILE ? (uint64_t)n - base : (uint64_t)RS_TILE);
+    running[t] = 0;
 #pragma unroll
     for (int w = 0; w < RS_BLOCK / 64; ++w) wcount[w][t] = 0;
     __syncthreads();
-    const uint64_t lt = lanemask_lt();
-    uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+    uint4 kr[RS_ROUNDS];
+#pragma unroll
     for (int r = 0; r < RS_ROUNDS; ++r) {
-        uint64_t j = base + (uint64_t)r * RS_BLOCK + t;
-        bool valid = j < n;
-        uint4 k = valid ? keys[j] : make_uint4(0, 0, 0, 0);
-        uint32_t dig = valid ? rs_digit(k, d) : 256u;
+        const uint32_t i = (uint32_t)r * RS_BLOCK + t;
+        kr[r] = i < cnt ? keys[base + i] : make_uint4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; ++r)
+        if ((uint32_t)r * RS_BLOCK + t < cnt) atomicAdd(&running[rs_digit(kr[r], d)], 1u);
+    __syncthreads();
+    /* exclusive prefix of the tile's digit counts */
+    const uint32_t c = running[t];
+    uint32_t incl = c;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t v = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += v;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    uint32_t wpre = 0;
+    for (int w = 0; w < wave; ++w) wpre += wsum[w];
+    lbase[t] = wpre + incl - c;
+    gbase[t] = offs[(uint64_t)t * nb + blockIdx.x];
+    running[t] = 0;
+    __syncthreads();
+    const uint64_t lt = lanemask_lt();
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        const bool valid = (uint32_t)r * RS_BLOCK + t < cnt;
+        const uint4 k = kr[r];
+        const uint32_t dig = valid ? rs_digit(k, d) : 256u;
         uint64_t peers = __ballot(1);
 #pragma unroll
         for (int b = 0; b < 9; ++b) {
-            uint64_t m = __ballot((dig >> b) & 1u);
+            const uint64_t m = __ballot((dig >> b) & 1u);
             peers &= ((dig >> b) & 1u) ? m : ~m;
         }
-        uint32_t rank = __popcll(peers & lt);
+        const uint32_t rank = __popcll(peers & lt);
         if (valid && rank == 0) wcount[wave][dig] = __popcll(peers);
         __syncthreads();
         if (valid) {
             uint32_t pre = 0;
             for (int w = 0; w < wave; ++w) pre += wcount[w][dig];
-            out[running[dig] + pre + rank] = k;
+            tile[lbase[dig] + running[dig] + pre + rank] = k;
         }
         __syncthreads();
         uint32_t add = 0;
@@ -181,6 +216,11 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const uint4 *__restrict
         }
         running[t] += add;
         __syncthreads();
+    }
+    for (uint32_t i = t; i < cnt; i += RS_BLOCK) {
+        const uint4 k = tile[i];
+        const uint32_t v = rs_digit(k, d);
+        out[gbase[v] + (i - lbase[v])] = k;
     }
 }
 
