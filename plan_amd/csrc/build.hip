@@ -306,81 +306,189 @@ __global__ void k_gather_particles(const uint4 *__restrict__ skeys, const uint32
     ukeys[r] = make_uint4(k.x, k.y, k.z, s);
 }
 
-struct Level {
-    int L;              /* parent level (1..31) */
-    uint32_t base;      /* first node id of level L+1 */
-};
+/* ------------------------------------------------ node array from the keys
+ *
+ * The level-major node array of CountNodesLeaves/FillTree
+ * (src/tree.hpp:2655-2755) follows from the sorted unique keys without a
+ * pass per level.  With lcp(a) the number of leading octal digits keys[a-1]
+ * and keys[a] share, position a starts a level-l range (the particles that
+ * share digits 1..l-1) iff a == 0 or lcp(a) < l - 1.  A level-l range holds
+ * more than 8 particles iff 9 consecutive keys in it share l - 1 digits, so
+ * the leaf holding particle i is at level
+ *     lam(i) = min(32, 2 + max(0, max W(s))),   W(s) = lcp(keys[s], keys[s + 8])
+ * over the windows s <= i <= s + 8 (the root is always split and a level-32
+ * node is always a leaf).  Position a starts the nodes of levels lo(a)..lam(a)
+ * (lo(0) = 1, lo(a) = lcp(a) + 2), and a node's id is the first id of its
+ * level plus the rank of a among the starts of that level.  Sibling groups
+ * are contiguous, so a node that is not its position's lowest takes its
+ * parent from that position, and the others from the nearest preceding
+ * first daughter of their level (at most 7 ids back). */
+static constexpr int NB_TILE = 256;
+static constexpr uint32_t NODE_PENDING = 0xfffffffeu;
 
-__global__ void k_lvl_flags(const uint4 *__restrict__ keys, uint32_t n, const uint32_t *__restrict__ pslot,
-                            const uint32_t *__restrict__ plist, const NodeTop *__restrict__ ntop, int L,
-                            uint32_t *__restrict__ flags)
+/* number of leading octal digits (levels 1..32) two keys share */
+__device__ __forceinline__ int key_lcp(const uint4 &a, const uint4 &b)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t k = pslot[i];
-    uint32_t f = 0;
-    if (k != PLAN_NONE) {
-        uint32_t b = ntop[plist[k]].begin;
-        f = (i == b) || (key_digit(keys[i], L) != key_digit(keys[i - 1], L));
+    const uint32_t z = a.z ^ b.z, y = a.y ^ b.y, x = a.x ^ b.x;
+    int hb;
+    if (z) hb = 95 - __clz((int)z);
+    else if (y) hb = 63 - __clz((int)y);
+    else if (x) hb = 31 - __clz((int)x);
+    else return 32;
+    return (95 - hb) / 3;
+}
+
+/* per position (lo, lam) and, per level, the number of node starts of each
+ * tile of NB_TILE positions (counts laid out level-major: cnt[(l-1) nb + b]) */
+__global__ __launch_bounds__(NB_TILE) void k_node_owners(const uint4 *__restrict__ keys, uint32_t n,
+                                                         uint16_t *__restrict__ lolam, uint32_t *__restrict__ cnt,
+                                                         uint32_t nb)
+{
+    __shared__ uint4 sk[NB_TILE + 16];
+    __shared__ uint8_t sw[NB_TILE + 8];
+    __shared__ uint32_t wc[NB_TILE / 64][33];
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const int64_t b0 = (int64_t)blockIdx.x * NB_TILE;
+    for (int j = t; j < NB_TILE + 16; j += NB_TILE) {
+        const int64_t g = b0 - 8 + j;
+        sk[j] = (g >= 0 && g < (int64_t)n) ? keys[g] : make_uint4(0, 0, 0, 0);
     }
-    flags[i] = f;
+    __syncthreads();
+    for (int j = t; j < NB_TILE + 8; j += NB_TILE) {
+        const int64_t w = b0 - 8 + j;
+        sw[j] = (w >= 0 && w + 8 < (int64_t)n) ? (uint8_t)key_lcp(sk[j], sk[j + 8]) : (uint8_t)0xff;
+    }
+    __syncthreads();
+    const int64_t i = b0 + t;
+    const bool valid = i < (int64_t)n;
+    int lo = 33, lam = 0;
+    if (valid) {
+        int m = 0;
+        for (int d = 0; d <= 8; ++d) {
+            const int w = sw[t + d]; /* window s = i - 8 + d */
+            if (w != 0xff) m = max(m, w);
+        }
+        lam = min(32, m + 2);
+        lo = (i == 0) ? 1 : key_lcp(sk[t + 7], sk[t + 8]) + 2;
+        lolam[i] = (uint16_t)(lo | (lam << 8));
+    }
+    for (int l = 1; l <= 32; ++l) {
+        const uint64_t m = __ballot(valid && lo <= l && l <= lam);
+        if (lane == 0) wc[wave][l] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    if (t < 32) {
+        uint32_t c = 0;
+        for (int w = 0; w < NB_TILE / 64; ++w) c += wc[w][t + 1];
+        cnt[(uint64_t)t * nb + blockIdx.x] = c;
+    }
 }
 
-__global__ void k_lvl_starts(const uint32_t *__restrict__ flags, const uint32_t *__restrict__ cidx, uint32_t n,
-                             uint32_t *__restrict__ cstart)
+/* first id of each level (out[l - 1]) from the scanned counts; out[32], the
+ * node count, is written by the scan */
+__global__ void k_level_ptrs(const uint32_t *__restrict__ off, uint32_t nb, uint32_t *__restrict__ out)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (flags[i]) cstart[cidx[i]] = i;
+    const uint32_t l = threadIdx.x;
+    if (l < 32) out[l] = off[(uint64_t)l * nb];
 }
 
-__global__ void k_lvl_nodes(const uint4 *__restrict__ keys, const uint32_t *__restrict__ cstart, uint32_t C,
-                            const uint32_t *__restrict__ pslot, const uint32_t *__restrict__ plist, int L,
-                            uint32_t base, NodeGeo *__restrict__ ngeo, NodeTop *__restrict__ ntop,
-                            uint32_t *__restrict__ ninfo, uint32_t *__restrict__ nonleaf)
+/* write every node: centre by the parent-to-child chain of binary64 adds from
+ * the root, begin, level, orthant, first daughter, and the parent where the
+ * position knows it */
+__global__ __launch_bounds__(NB_TILE) void k_node_emit(const uint4 *__restrict__ keys, uint32_t n,
+                                                       const uint16_t *__restrict__ lolam,
+                                                       const uint32_t *__restrict__ off, uint32_t nb, NodeGeo root,
+                                                       NodeGeo *__restrict__ ngeo, NodeTop *__restrict__ ntop,
+                                                       uint32_t *__restrict__ ninfo)
 {
-    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= C) return;
-    uint32_t i = cstart[r];
-    uint32_t k = pslot[i];
-    uint32_t P = plist[k];
-    NodeTop pt = ntop[P];
-    NodeGeo pg = ngeo[P];
-    uint32_t e = (r + 1 < C && pslot[cstart[r + 1]] == k) ? cstart[r + 1] : pt.end;
-    uint32_t o = key_digit(keys[i], L);
-    double hw = __dmul_rn(0.5, pg.hw);
-    NodeGeo g;
-    g.cx = __dadd_rn(pg.cx, (o & 1u) ? hw : -hw);
-    g.cy = __dadd_rn(pg.cy, (o & 2u) ? hw : -hw);
-    g.cz = __dadd_rn(pg.cz, (o & 4u) ? hw : -hw);
-    g.hw = hw;
-    uint32_t node = base + r;
-    ngeo[node] = g;
-    ntop[node] = NodeTop{i, e, 0u, P};
-    ninfo[node] = ((uint32_t)(L + 1) << 8) | (o << 16);
-    bool leaf = (e - i <= 8u) || (L == 31);
-    nonleaf[r] = leaf ? 0u : 1u;
-    if (i == pt.begin) ntop[P].first_daughter = node;
-    atomicAdd(&ninfo[P], 1u);
+    __shared__ uint32_t wc[NB_TILE / 64][33];
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const uint64_t i = (uint64_t)blockIdx.x * NB_TILE + t;
+    const bool valid = i < n;
+    int lo = 33, lam = 0;
+    uint4 key = make_uint4(0, 0, 0, 0);
+    if (valid) {
+        const uint32_t v = lolam[i];
+        lo = (int)(v & 0xffu);
+        lam = (int)(v >> 8);
+        key = keys[i];
+    }
+    for (int l = 1; l <= 32; ++l) {
+        const uint64_t m = __ballot(valid && lo <= l && l <= lam);
+        if (lane == 0) wc[wave][l] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    const uint64_t lt = lanemask_lt();
+    double cx = root.cx, cy = root.cy, cz = root.cz, hw = root.hw;
+    uint32_t prev = PLAN_NONE;
+    for (int l = 1; l <= 32; ++l) {
+        uint32_t o = 0;
+        if (l >= 2 && l <= lam) {
+            hw = __dmul_rn(0.5, hw);
+            o = key_digit(key, l - 1);
+            cx = __dadd_rn(cx, (o & 1u) ? hw : -hw);
+            cy = __dadd_rn(cy, (o & 2u) ? hw : -hw);
+            cz = __dadd_rn(cz, (o & 4u) ? hw : -hw);
+        }
+        const bool own = valid && lo <= l && l <= lam;
+        const uint64_t m = __ballot(own);
+        if (own) {
+            uint32_t pre = 0;
+            for (int w = 0; w < wave; ++w) pre += wc[w][l];
+            const uint32_t id = off[(uint64_t)(l - 1) * nb + blockIdx.x] + pre + (uint32_t)__popcll(m & lt);
+            const uint32_t par = (l > lo) ? prev : (i == 0 ? PLAN_NONE : NODE_PENDING);
+            ngeo[id] = NodeGeo{cx, cy, cz, hw};
+            ntop[id] = NodeTop{(uint32_t)i, 0u, 0u, par};
+            ninfo[id] = ((uint32_t)l << 8) | (o << 16);
+            if (l > lo) ntop[prev].first_daughter = id;
+            prev = id;
+        }
+    }
 }
 
-__global__ void k_lvl_next(const uint32_t *__restrict__ nonleaf, const uint32_t *__restrict__ nlidx, uint32_t C,
-                           uint32_t base, uint32_t *__restrict__ plist_next)
+/* parent of every node: a position's lowest node takes the parent of the
+ * nearest preceding first daughter of its level */
+__global__ void k_node_parent(const NodeTop *__restrict__ ntop, uint32_t nn, uint32_t *__restrict__ par,
+                              uint32_t *__restrict__ err)
 {
-    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= C) return;
-    if (nonleaf[r]) plist_next[nlidx[r]] = base + r;
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nn) return;
+    uint32_t p = ntop[v].parent;
+    for (uint32_t u = v, k = 0; p == NODE_PENDING && u > 0 && k < 8; ++k) p = ntop[--u].parent;
+    if (p == NODE_PENDING || (v != 0 && p >= nn)) {
+        atomicAdd(err, 1u);
+        p = 0;
+    }
+    par[v] = p;
 }
 
-__global__ void k_lvl_slots(uint32_t n, const uint32_t *__restrict__ flags, const uint32_t *__restrict__ cidx,
-                            const uint32_t *__restrict__ nonleaf, const uint32_t *__restrict__ nlidx,
-                            uint32_t *__restrict__ pslot)
+/* end of every node (the next sibling's begin, else its parent's end) and,
+ * from each last child, its parent's number of daughters */
+__global__ void k_node_end(NodeTop *__restrict__ ntop, const uint32_t *__restrict__ par, uint32_t nn, uint32_t n,
+                           uint32_t *__restrict__ nd)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (pslot[i] == PLAN_NONE) return;
-    uint32_t r = cidx[i] + flags[i] - 1;
-    pslot[i] = nonleaf[r] ? nlidx[r] : PLAN_NONE;
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nn) return;
+    uint32_t x = v, e = n;
+    for (int k = 0; k < 40 && x != 0; ++k) {
+        if (x + 1 < nn && par[x + 1] == par[x]) {
+            e = ntop[x + 1].begin;
+            break;
+        }
+        x = par[x];
+    }
+    const uint32_t pv = par[v];
+    ntop[v].end = e;
+    ntop[v].parent = pv;
+    if (v != 0 && (v + 1 >= nn || par[v + 1] != pv)) nd[pv] = v + 1 - ntop[pv].first_daughter;
+}
+
+__global__ void k_node_daughters(const NodeTop *__restrict__ ntop, const uint32_t *__restrict__ nd, uint32_t nn,
+                                 uint32_t *__restrict__ ninfo)
+{
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nn) return;
+    if (ntop[v].first_daughter != 0) ninfo[v] |= nd[v];
 }
 
 /* skip pointer: first node after the subtree in pre-order */
@@ -427,19 +535,6 @@ __global__ void k_leaf_index(const NodeTop *__restrict__ ntop, const uint32_t *_
     uint32_t k = lidx[ntop[v].begin];
     node2leaf[v] = (int32_t)k;
     leaf_nodes[k] = (int32_t)v;
-}
-
-template <class T> static void grow_copy(Arena &a, const std::string &name, T *&p, size_t old_n, size_t new_n,
-                                         hipStream_t s)
-{
-    T *np = nullptr;
-    HIPCHK(hipMalloc(&np, new_n * sizeof(T)));
-    if (p && old_n) HIPCHK(hipMemcpyAsync(np, p, old_n * sizeof(T), hipMemcpyDeviceToDevice, s));
-    (void)a;
-    (void)name;
-    HIPCHK(hipStreamSynchronize(s));
-    if (p) HIPCHK(hipFree(p));
-    p = np;
 }
 
 void dev_build_tree(DevTree &t, const DevParticles &ps, const plan_derived &d, bool check_pos,
@@ -544,16 +639,30 @@ void dev_build_tree(DevTree &t, const DevParticles &ps, const plan_derived &d, b
         t.num_sinks = read_u32(tot + 1, s);
     }
 
-    /* level-synchronous node construction into grow-only node arrays */
-    size_t cap = t.node_cap;
-    if (cap < (size_t)n + 64) {
+    /* node array straight from the sorted keys (grow-only node arrays) */
+    const uint32_t nbn = ceil_div(n, NB_TILE);
+    uint16_t *lolam = a.get_t<uint16_t>("lolam", n);
+    uint32_t *ncnt = a.get_t<uint32_t>("node_cnt", (size_t)32 * nbn);
+    uint32_t *noff = a.get_t<uint32_t>("node_off", (size_t)32 * nbn);
+    uint32_t *lptr = a.get_t<uint32_t>("node_lptr", 40);
+    hipLaunchKernelGGL(k_node_owners, dim3(nbn), dim3(NB_TILE), 0, s, t.keys, n, lolam, ncnt, nbn);
+    LAUNCHCHK();
+    scan_exclusive_u32(a, ncnt, noff, 32 * nbn, s, lptr + 32);
+    hipLaunchKernelGGL(k_level_ptrs, dim3(1), dim3(64), 0, s, noff, nbn, lptr);
+    LAUNCHCHK();
+    HIPCHK(hipMemsetAsync(lptr + 33, 0, 4, s));
+    uint32_t hl[33];
+    HIPCHK(hipMemcpyAsync(hl, lptr, sizeof(hl), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const uint32_t nn = hl[32];
+    if (t.node_cap < (size_t)nn) {
         if (t.ngeo) HIPCHK(hipFree(t.ngeo));
         if (t.ntop) HIPCHK(hipFree(t.ntop));
         if (t.ninfo) HIPCHK(hipFree(t.ninfo));
         t.ngeo = nullptr;
         t.ntop = nullptr;
         t.ninfo = nullptr;
-        cap = (size_t)n + n / 2 + 64;
+        const size_t cap = std::max<size_t>(nn, (size_t)n + n / 2 + 64);
         HIPCHK(hipMalloc(&t.ngeo, cap * sizeof(NodeGeo)));
         HIPCHK(hipMalloc(&t.ntop, cap * sizeof(NodeTop)));
         HIPCHK(hipMalloc(&t.ninfo, cap * sizeof(uint32_t)));
@@ -562,69 +671,26 @@ void dev_build_tree(DevTree &t, const DevParticles &ps, const plan_derived &d, b
     NodeGeo *ngeo = t.ngeo;
     NodeTop *ntop = t.ntop;
     uint32_t *ninfo = t.ninfo;
-    NodeGeo rg{t.root_center[0], t.root_center[1], t.root_center[2], t.half_width};
-    NodeTop rt{0u, n, 0u, PLAN_NONE};
-    uint32_t ri = 1u << 8;
-    HIPCHK(hipMemcpyAsync(ngeo, &rg, sizeof(rg), hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(ntop, &rt, sizeof(rt), hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(ninfo, &ri, sizeof(ri), hipMemcpyHostToDevice, s));
-    uint32_t *pslot = a.get_t<uint32_t>("pslot", n);
-    uint32_t *cidx = a.get_t<uint32_t>("cidx", n + 1);
-    uint32_t *cstart = a.get_t<uint32_t>("cstart", n + 1);
-    HIPCHK(hipMemsetAsync(pslot, 0, (size_t)n * 4, s));
-    uint32_t *plist = a.get_t<uint32_t>("plist_a", n + 1);
-    uint32_t *plist_next = a.get_t<uint32_t>("plist_b", n + 1);
-    uint32_t zero = 0;
-    HIPCHK(hipMemcpyAsync(plist, &zero, 4, hipMemcpyHostToDevice, s));
-    uint32_t np = 1;
-    t.level_count[0] = 1;
-    for (int l = 1; l < 32; ++l) t.level_count[l] = 0;
-    uint32_t base = 1;
-    int L = 1;
+    const NodeGeo rg{t.root_center[0], t.root_center[1], t.root_center[2], t.half_width};
+    hipLaunchKernelGGL(k_node_emit, dim3(nbn), dim3(NB_TILE), 0, s, t.keys, n, lolam, noff, nbn, rg, ngeo, ntop,
+                       ninfo);
+    LAUNCHCHK();
+    uint32_t *npar = a.get_t<uint32_t>("node_par", nn);
+    uint32_t *ndau = a.get_t<uint32_t>("node_nd", nn);
+    hipLaunchKernelGGL(k_node_parent, dim3(ceil_div(nn, BLK)), dim3(BLK), 0, s, ntop, nn, npar, lptr + 33);
+    LAUNCHCHK();
+    hipLaunchKernelGGL(k_node_end, dim3(ceil_div(nn, BLK)), dim3(BLK), 0, s, ntop, npar, nn, n, ndau);
+    LAUNCHCHK();
+    hipLaunchKernelGGL(k_node_daughters, dim3(ceil_div(nn, BLK)), dim3(BLK), 0, s, ntop, ndau, nn, ninfo);
+    LAUNCHCHK();
     t.depth = 1;
-    while (np > 0 && L <= 31) {
-        hipLaunchKernelGGL(k_lvl_flags, dim3(ceil_div(n, BLK)), dim3(BLK), 0, s, t.keys, n, pslot, plist, ntop, L,
-                           flags);
-        LAUNCHCHK();
-        scan_exclusive_u32(a, flags, cidx, n, s, tot);
-        uint32_t C = read_u32(tot, s);
-        if (C == 0) break;
-        if (base + (size_t)C > cap) {
-            size_t ncap = std::max(cap * 2, (size_t)base + C + 64);
-            grow_copy(a, "ngeo", ngeo, base, ncap, s);
-            grow_copy(a, "ntop", ntop, base, ncap, s);
-            grow_copy(a, "ninfo", ninfo, base, ncap, s);
-            cap = ncap;
-            t.ngeo = ngeo;
-            t.ntop = ntop;
-            t.ninfo = ninfo;
-            t.node_cap = cap;
-        }
-        hipLaunchKernelGGL(k_lvl_starts, dim3(ceil_div(n, BLK)), dim3(BLK), 0, s, flags, cidx, n, cstart);
-        LAUNCHCHK();
-        uint32_t *nonleaf = a.get_t<uint32_t>("nonleaf", C + 1);
-        uint32_t *nlidx = a.get_t<uint32_t>("nlidx", C + 1);
-        hipLaunchKernelGGL(k_lvl_nodes, dim3(ceil_div(C, BLK)), dim3(BLK), 0, s, t.keys, cstart, C, pslot, plist, L,
-                           base, ngeo, ntop, ninfo, nonleaf);
-        LAUNCHCHK();
-        scan_exclusive_u32(a, nonleaf, nlidx, C, s, tot + 1);
-        uint32_t Cn = read_u32(tot + 1, s);
-        hipLaunchKernelGGL(k_lvl_next, dim3(ceil_div(C, BLK)), dim3(BLK), 0, s, nonleaf, nlidx, C, base, plist_next);
-        LAUNCHCHK();
-        hipLaunchKernelGGL(k_lvl_slots, dim3(ceil_div(n, BLK)), dim3(BLK), 0, s, n, flags, cidx, nonleaf, nlidx,
-                           pslot);
-        LAUNCHCHK();
-        t.level_count[L] = (int)C;
-        base += C;
-        np = Cn;
-        std::swap(plist, plist_next);
-        t.depth = L + 1;
-        ++L;
+    for (int l = 0; l < 32; ++l) {
+        t.level_ptr[l] = (int)hl[l];
+        t.level_count[l] = (int)((l + 1 < 32 ? hl[l + 1] : nn) - hl[l]);
+        if (t.level_count[l] > 0) t.depth = l + 1;
     }
-    t.num_nodes = base;
-    t.level_ptr[0] = 0;
-    for (int l = 1; l < 32; ++l) t.level_ptr[l] = t.level_ptr[l - 1] + t.level_count[l - 1];
-    const uint32_t nn = t.num_nodes;
+    t.num_nodes = nn;
+    uint32_t *cidx = a.get_t<uint32_t>("cidx", n + 1);
     t.skip = a.get_t<uint32_t>("skip", nn);
     t.leaf_of = a.get_t<uint32_t>("leaf_of", n);
     t.node2leaf = a.get_t<int32_t>("node2leaf", nn);
@@ -639,7 +705,7 @@ void dev_build_tree(DevTree &t, const DevParticles &ps, const plan_derived &d, b
     hipLaunchKernelGGL(k_leaf_index, dim3(ceil_div(nn, BLK)), dim3(BLK), 0, s, ntop, ninfo, nn, cidx, t.node2leaf,
                        t.leaf_nodes);
     LAUNCHCHK();
-    HIPCHK(hipStreamSynchronize(s));
+    if (read_u32(lptr + 33, s) != 0) throw std::runtime_error("BuildTree: inconsistent node array");
 }
 
 }  // namespace plan
