@@ -677,6 +677,47 @@ struct RbLocal {
     }
 };
 
+static bool cpu_has_avx512()
+{
+    static const bool v = __builtin_cpu_supports("avx512f") && __builtin_cpu_supports("avx512bw") &&
+                          __builtin_cpu_supports("avx512vl");
+    return v;
+}
+
+/* D[p] = |(X, Y, Z)[p] - o|, summed x, y, z as Ops::dist_pts; compiled for
+ * each instruction set (IEEE operations give the same bits at any width) */
+#define PLAN_DIST_BODY                                          \
+    for (size_t p = 0; p < n; ++p) {                            \
+        const double dx = X[p] - ox, dy = Y[p] - oy, dz = Z[p] - oz; \
+        double q = 0;                                           \
+        q += dx * dx;                                           \
+        q += dy * dy;                                           \
+        q += dz * dz;                                           \
+        D[p] = std::sqrt(q);                                    \
+    }
+static void dist_base(double ox, double oy, double oz, const double *X, const double *Y, const double *Z, size_t n,
+                      double *D)
+{
+    PLAN_DIST_BODY
+}
+__attribute__((target("avx2"))) static void dist_avx2(double ox, double oy, double oz, const double *X,
+                                                       const double *Y, const double *Z, size_t n, double *D)
+{
+    PLAN_DIST_BODY
+}
+__attribute__((target("avx512f,avx512bw,avx512vl"))) static void dist_avx512(double ox, double oy, double oz,
+                                                                              const double *X, const double *Y,
+                                                                              const double *Z, size_t n, double *D)
+{
+    PLAN_DIST_BODY
+}
+#undef PLAN_DIST_BODY
+typedef void (*DistFn)(double, double, double, const double *, const double *, const double *, size_t, double *);
+static DistFn dist_fn()
+{
+    return cpu_has_avx512() ? dist_avx512 : __builtin_cpu_supports("avx2") ? dist_avx2 : dist_base;
+}
+
 /* SortParticles after claims (src/tree.hpp:4431-4512 via 4705-4709):
  * kinematics summed in member order (the last sorted order, then the claims
  * in claim order), distances to the new centre, sort. */
@@ -704,17 +745,9 @@ static void rb_resort(HClump &c, RbLocal &L)
     c.vel[1] = uy / M;
     c.vel[2] = uz / M;
     L.d.resize(n);
-    const double ox = c.com[0], oy = c.com[1], oz = c.com[2];
-    const double *X = L.x.data(), *Y = L.y.data(), *Z = L.z.data();
+    static const DistFn dist_to_centre = dist_fn();
     double *D = L.d.data();
-    for (size_t p = 0; p < n; ++p) {
-        const double dx = X[p] - ox, dy = Y[p] - oy, dz = Z[p] - oz;
-        double q = 0;
-        q += dx * dx;
-        q += dy * dy;
-        q += dz * dz;
-        D[p] = std::sqrt(q);
-    }
+    dist_to_centre(c.com[0], c.com[1], c.com[2], L.x.data(), L.y.data(), L.z.data(), n, D);
     L.v.resize(n);
     for (size_t i = 0; i < ns; ++i) L.v[i] = PD(L.ord[i], D[L.ord[i]]);
     for (size_t p = ns; p < n; ++p) L.v[p] = PD((uint32_t)p, D[p]);
@@ -825,9 +858,9 @@ struct EnergyState {
 };
 
 /* rebind_energy for candidates [0, n) of a block held as arrays: the same
- * expression element by element (written once, compiled for the baseline
- * and for AVX2; IEEE operations give the same bits at any vector width and
- * nothing is fused) */
+ * expression element by element (written once, compiled for the baseline,
+ * AVX2 and AVX-512; IEEE operations give the same bits at any vector width
+ * and nothing is fused) */
 #define PLAN_ENERGY_BLOCK_BODY                                                    \
     for (size_t j = 0; j < n; ++j) {                                              \
         const double tm = S.M + m[j];                                             \
@@ -861,12 +894,70 @@ __attribute__((target("avx2"))) static void energy_block_avx2(const EnergyState 
 {
     PLAN_ENERGY_BLOCK_BODY
 }
+__attribute__((target("avx512f,avx512bw,avx512vl"))) static void
+energy_block_avx512(const EnergyState &S, const double *x, const double *y, const double *z, const double *vx,
+                    const double *vy, const double *vz, const double *m, size_t n, double *e)
+{
+    PLAN_ENERGY_BLOCK_BODY
+}
 #undef PLAN_ENERGY_BLOCK_BODY
 typedef void (*EnergyBlockFn)(const EnergyState &, const double *, const double *, const double *, const double *,
                               const double *, const double *, const double *, size_t, double *);
 static EnergyBlockFn energy_block_fn()
 {
-    static const EnergyBlockFn f = __builtin_cpu_supports("avx2") ? energy_block_avx2 : energy_block_base;
+    static const EnergyBlockFn f = cpu_has_avx512()                 ? energy_block_avx512
+                                   : __builtin_cpu_supports("avx2") ? energy_block_avx2
+                                                                    : energy_block_base;
+    return f;
+}
+
+/* One iteration's scan of the live records: squared distance s to the clump
+ * centre, bit 0 = within the ball (s <= r2) and near c0, bit 1 = in the band
+ * around the sphere; returns the number of band records.  The same
+ * expression is compiled for each instruction set, as the energies are. */
+struct LiveScan {
+    double ox, oy, oz, bound, blo, bhi, r2;
+};
+#define PLAN_LIVE_SCAN_BODY                                       \
+    size_t nband = 0;                                             \
+    for (size_t k = 0; k < nl; ++k) {                             \
+        const double t0 = q.ox - X[k], t1 = q.oy - Y[k], t2 = q.oz - Z[k]; \
+        double s = 0;                                             \
+        s += t0 * t0;                                             \
+        s += t1 * t1;                                             \
+        s += t2 * t2;                                             \
+        S[k] = s;                                                 \
+        const bool near = LS[k] <= q.bound;                       \
+        const bool band = near & (s >= q.blo) & (s <= q.bhi);     \
+        F[k] = (uint8_t)((near & (s <= q.r2)) | (band << 1));     \
+        nband += band;                                            \
+    }                                                             \
+    return nband;
+static size_t live_scan_base(const LiveScan &q, const double *X, const double *Y, const double *Z, const double *LS,
+                             size_t nl, double *S, uint8_t *F)
+{
+    PLAN_LIVE_SCAN_BODY
+}
+__attribute__((target("avx2"))) static size_t live_scan_avx2(const LiveScan &q, const double *X, const double *Y,
+                                                              const double *Z, const double *LS, size_t nl,
+                                                              double *S, uint8_t *F)
+{
+    PLAN_LIVE_SCAN_BODY
+}
+__attribute__((target("avx512f,avx512bw,avx512vl"))) static size_t
+live_scan_avx512(const LiveScan &q, const double *X, const double *Y, const double *Z, const double *LS, size_t nl,
+                 double *S, uint8_t *F)
+{
+    PLAN_LIVE_SCAN_BODY
+}
+#undef PLAN_LIVE_SCAN_BODY
+typedef size_t (*LiveScanFn)(const LiveScan &, const double *, const double *, const double *, const double *, size_t,
+                             double *, uint8_t *);
+static LiveScanFn live_scan_fn()
+{
+    static const LiveScanFn f = cpu_has_avx512()                 ? live_scan_avx512
+                                : __builtin_cpu_supports("avx2") ? live_scan_avx2
+                                                                 : live_scan_base;
     return f;
 }
 
@@ -944,6 +1035,7 @@ static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
     const double inf = std::numeric_limits<double>::infinity();
     double eb[BLK], bx[BLK], by[BLK], bz[BLK], bvx[BLK], bvy[BLK], bvz[BLK], bm[BLK];
     const EnergyBlockFn energy_block = energy_block_fn();
+    const LiveScanFn live_scan = live_scan_fn();
     size_t before;
     auto t_start = std::chrono::steady_clock::now();
     auto claim = [&](uint32_t h, const double *qp, const double *qv, double qm) {
@@ -977,19 +1069,7 @@ static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
             const double *LS = live.lsd.data();
             flag.resize(nl);
             uint8_t *F = flag.data();
-            size_t nband = 0;
-            for (size_t k = 0; k < nl; ++k) {
-                const double t0 = ox - X[k], t1 = oy - Y[k], t2 = oz - Z[k];
-                double s = 0;
-                s += t0 * t0;
-                s += t1 * t1;
-                s += t2 * t2;
-                S[k] = s;
-                const bool near = LS[k] <= bound;
-                const bool band = near & (s >= blo) & (s <= bhi);
-                F[k] = (uint8_t)((near & (s <= r2)) | (band << 1));
-                nband += band;
-            }
+            const size_t nband = live_scan(LiveScan{ox, oy, oz, bound, blo, bhi, r2}, X, Y, Z, LS, nl, S, F);
             /* a candidate within the margin of the sphere anywhere sends the
              * iteration to the exact search (its candidates are then the
              * device's), so the order of this test does not matter */
