@@ -13,6 +13,8 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <condition_variable>
+#include <exception>
 #include <mutex>
 #include <vector>
 
@@ -44,6 +46,7 @@ struct plan_ctx {
     uint32_t hop_na = 0, hop_groups = 0;
     double d2h_ms = 0;
     HostPinned hpin;
+    std::vector<uint32_t> tidx_map; /* tree index -> member record (grow-only) */
     double density_kernel_ms = 0;
     double density_radius_ms = 0;
     uint32_t density_ovf_waves = 0;
@@ -215,7 +218,19 @@ __global__ void k_split_pts(const double *__restrict__ pts, const double *__rest
 /* device services for the host clump logic */
 struct Services : ClumpServices {
     plan_ctx *c;
+    /* ball searches asked for by the parallel rebinding runs, waiting for the
+     * device (one search batch on the stream at a time) */
+    struct BallRequest {
+        const std::vector<double4> *queries;
+        double rho_min;
+        std::vector<std::vector<uint32_t>> *out;
+        bool done;
+        std::exception_ptr err;
+    };
     std::mutex mu;
+    std::condition_variable cv;
+    std::vector<BallRequest *> pending;
+    bool device_busy = false;
     explicit Services(plan_ctx *cc) : c(cc) {}
 
     void saddle_density(const std::vector<double4> &pts, std::vector<double> &rho) override
@@ -275,12 +290,53 @@ struct Services : ClumpServices {
         }
     }
 
-    /* called from the parallel rebinding runs: one search on the device at a time */
+    /* Called from the parallel rebinding runs.  A search costs about the same
+     * for one query as for many (a level-synchronous walk), so the requests
+     * that wait while the device is busy go to it together: the caller that
+     * finds the device free takes every waiting request with its threshold,
+     * runs them as one batch and hands each caller its own lists (each
+     * query's list does not depend on the others in the batch). */
     void particle_balls(const std::vector<double4> &queries, double rho_min,
                         std::vector<std::vector<uint32_t>> &out) override
     {
-        std::lock_guard<std::mutex> g(mu);
-        run_balls(c->tree, queries, c->tree.rho, rho_min, 1, out);
+        BallRequest me{&queries, rho_min, &out, false, nullptr};
+        std::unique_lock<std::mutex> lk(mu);
+        pending.push_back(&me);
+        while (!me.done) {
+            if (device_busy) {
+                cv.wait(lk);
+                continue;
+            }
+            device_busy = true;
+            std::vector<BallRequest *> batch, rest;
+            const double thr = pending.front()->rho_min;
+            for (BallRequest *r : pending) (r->rho_min == thr ? batch : rest).push_back(r);
+            pending.swap(rest);
+            lk.unlock();
+            std::exception_ptr err = nullptr;
+            try {
+                std::vector<double4> q;
+                for (BallRequest *r : batch) q.insert(q.end(), r->queries->begin(), r->queries->end());
+                std::vector<std::vector<uint32_t>> lists;
+                run_balls(c->tree, q, c->tree.rho, thr, 1, lists);
+                size_t k = 0;
+                for (BallRequest *r : batch) {
+                    r->out->resize(r->queries->size());
+                    for (auto &l : *r->out) l.swap(lists[k++]);
+                }
+            } catch (...) {
+                err = std::current_exception();
+            }
+            lk.lock();
+            for (BallRequest *r : batch) {
+                r->err = err;
+                r->done = true;
+            }
+            device_busy = false;
+            cv.notify_all();
+        }
+        lk.unlock();
+        if (me.err) std::rethrow_exception(me.err);
     }
 };
 
@@ -587,6 +643,15 @@ int plan_find_planetesimals(plan_ctx *c, plan_catalog **out)
             HIPCHK(hipStreamSynchronize(s));
         }
         hm.cstart[nc] = na;
+        if (c->tidx_map.size() < c->tree.n) c->tidx_map.resize(c->tree.n);
+        {
+            uint32_t *map = c->tidx_map.data();
+            const uint32_t *st = hm.sorted_tidx.data(), *ro = hm.rec_of_sorted.data();
+#pragma omp parallel for schedule(static)
+            for (long k = 0; k < (long)na; ++k) map[st[k]] = ro[k];
+            hm.tidx_map = map;
+            hm.tidx_map_n = c->tree.n;
+        }
         ClumpSettings cs;
         cs.K = K;
         cs.kernel = kernel;
@@ -931,7 +996,7 @@ int plan_stage_times(plan_ctx *c, double *ms, int n)
 int plan_clump_stats(plan_ctx *c, double *v, int n)
 {
     if (!c || !v) return PLAN_E_ARG;
-    double a[35] = {(double)c->hop_na, (double)c->hop_groups, (double)c->cstats.naive, (double)c->cstats.after_merge,
+    double a[41] = {(double)c->hop_na, (double)c->hop_groups, (double)c->cstats.naive, (double)c->cstats.after_merge,
                     (double)c->cstats.merge_passes, (double)c->cstats.pair_candidates,
                     (double)c->cstats.saddle_queries, (double)c->cstats.rebind_searches, c->d2h_ms,
                     (double)c->cstats.rebind_reruns};
@@ -947,7 +1012,13 @@ int plan_clump_stats(plan_ctx *c, double *v, int n)
     a[32] = c->cstats.rebind_max_exact;
     a[33] = c->cstats.rebind_max_exact_ms;
     a[34] = c->cstats.rebind_max_sort_ms;
-    for (int i = 0; i < n && i < 35; ++i) v[i] = a[i];
+    a[35] = c->cstats.rebind_work_ms;
+    a[36] = c->cstats.rebind_exact_total_ms;
+    a[37] = c->cstats.rebind_max_slot;
+    a[38] = c->cstats.rebind_last_end_ms;
+    a[39] = c->cstats.rebind_last_slot;
+    a[40] = c->cstats.rebind_last_ms;
+    for (int i = 0; i < n && i < 41; ++i) v[i] = a[i];
     return PLAN_OK;
 }
 

@@ -26,12 +26,15 @@ namespace plan {
 typedef std::pair<uint32_t, double> PD;
 static bool pd_less(const PD &a, const PD &b) { return a.second < b.second; }
 
-/* OpenMP loop over [0, n), one index at a time; the first exception thrown by
- * a body is rethrown on the calling thread after the loop. */
+/* OpenMP loop over [0, n), one index at a time and in index order (callers
+ * put the longest work first; a nonmonotonic dynamic schedule may hand a
+ * thread a block of indices up front, which starts a long one late); the
+ * first exception thrown by a body is rethrown on the calling thread after
+ * the loop. */
 template <class F> static void parallel_for(long n, F f)
 {
     std::exception_ptr err = nullptr;
-#pragma omp parallel for schedule(dynamic, 1)
+#pragma omp parallel for schedule(monotonic : dynamic, 1)
     for (long i = 0; i < n; ++i) {
         try {
             f(i);
@@ -45,6 +48,11 @@ template <class F> static void parallel_for(long n, F f)
 
 uint32_t HostMembers::rec_of_tidx(uint32_t tidx) const
 {
+    if (tidx_map) {
+        if (tidx >= tidx_map_n) return PLAN_NONE;
+        const uint32_t r = tidx_map[tidx];
+        return (r < rec.size() && rec[r].tidx == tidx) ? r : PLAN_NONE;
+    }
     auto it = std::lower_bound(sorted_tidx.begin(), sorted_tidx.end(), tidx);
     if (it == sorted_tidx.end() || *it != tidx) return PLAN_NONE;
     return rec_of_sorted[it - sorted_tidx.begin()];
@@ -638,6 +646,7 @@ struct Rebind {
     uint32_t exact_searches = 0;
     uint32_t iters = 0;
     double ms = 0, ms_exact = 0, ms_sort = 0;
+    std::chrono::steady_clock::time_point t0, t1; /* start and end of the run */
 };
 
 static void to_records(const HostMembers &hm, const std::vector<uint32_t> &tidx, std::vector<uint32_t> &out)
@@ -1013,6 +1022,8 @@ struct HandleSet {
 static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::vector<uint8_t> &flags,
                        const ClumpSettings &cs, const Ops &ops, ClumpServices &svc)
 {
+    const auto t_start = std::chrono::steady_clock::now();
+    rb.t0 = t_start;
     const double sh = cs.q * cs.Omega;
     HandleSet mine;
     LiveSet live;
@@ -1037,7 +1048,6 @@ static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
     const EnergyBlockFn energy_block = energy_block_fn();
     const LiveScanFn live_scan = live_scan_fn();
     size_t before;
-    auto t_start = std::chrono::steady_clock::now();
     auto claim = [&](uint32_t h, const double *qp, const double *qv, double qm) {
         c.total_mass = c.total_mass + qm;
         for (int k = 0; k < 3; ++k) c.com[k] = (c.com[k] * c.total_mass + qm * qp[k]) / c.total_mass;
@@ -1148,7 +1158,8 @@ static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
         ops.hill_radius(c);
     } while (L.size() > before);
     if (L.size() > n0) rb_finish(c, L);
-    rb.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    rb.t1 = std::chrono::steady_clock::now();
+    rb.ms = std::chrono::duration<double, std::milli>(rb.t1 - t_start).count();
 }
 
 /* Rebinding of all clumps in key order (src/tree.hpp:4196-4208).  Every
@@ -1214,6 +1225,20 @@ static void rebind_all(std::map<uint32_t, HClump> &cl, const HostMembers &hm, co
         rebind_one(*v[i], rb[i], hm, in_clump, cs, ops, svc);
     });
     part(1);
+    auto first = rb[order[0]].t0;
+    for (size_t k = 0; k < nc; ++k) first = std::min(first, rb[order[k]].t0);
+    for (size_t k = 0; k < nc; ++k) {
+        const Rebind &r = rb[order[k]];
+        st.rebind_work_ms += r.ms;
+        st.rebind_exact_total_ms += r.ms_exact;
+        if (r.ms > rb[order[st.rebind_max_slot]].ms) st.rebind_max_slot = (uint32_t)k;
+        const double end = std::chrono::duration<double, std::milli>(r.t1 - first).count();
+        if (end > st.rebind_last_end_ms) {
+            st.rebind_last_end_ms = end;
+            st.rebind_last_slot = (uint32_t)k;
+            st.rebind_last_ms = r.ms;
+        }
+    }
     for (size_t i = 0; i < nc; ++i) {
         bool ok = true;
         for (uint32_t h : rb[i].claimed)

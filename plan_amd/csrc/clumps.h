@@ -76,6 +76,11 @@ struct HostMembers {
     HostSpan<uint32_t> sorted_tidx;     /* tree indices ascending */
     HostSpan<uint32_t> rec_of_sorted;   /* record of sorted_tidx[i] */
     HostSpan<uint32_t> sinkcnt;         /* per record: particles merged into it */
+    /* optional dense tree index -> record table (entries of tree indices
+     * that are not records hold anything; rec_of_tidx checks the record's
+     * own tree index) */
+    const uint32_t *tidx_map = nullptr;
+    size_t tidx_map_n = 0;
     uint32_t rec_of_tidx(uint32_t tidx) const;
 };
 
@@ -89,6 +94,14 @@ struct ClumpStats {
     uint32_t rebind_reruns = 0;
     uint32_t rebind_iters = 0, rebind_max_iters = 0, rebind_max_n = 0, rebind_max_exact = 0;
     double rebind_max_ms = 0, rebind_max_exact_ms = 0, rebind_max_sort_ms = 0;
+    /* rebinding runs: summed time, summed exact-search time (waits included),
+     * schedule position of the longest run */
+    double rebind_work_ms = 0, rebind_exact_total_ms = 0;
+    uint32_t rebind_max_slot = 0;
+    /* the run that ended last: its end (ms after the first run started),
+     * schedule slot and length */
+    double rebind_last_end_ms = 0, rebind_last_ms = 0;
+    uint32_t rebind_last_slot = 0;
     /* host milliseconds: sort, merge A, merge B, re-key, unbind, rebind,
      * filters, rip, J, ranking */
     double ms[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};

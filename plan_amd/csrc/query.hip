@@ -542,7 +542,9 @@ __device__ bool walk_dense(const TreeView &tv, NodeCache &nc, DenseWalkLds &D, L
         __syncthreads();
         const uint32_t cnt = (uint32_t)__popcll(M);
         const uint32_t total = nd * cnt;
-        const float rc = 1.0f / (float)cnt;
+        /* hardware reciprocal (1 ulp): (p + 0.5) rc stays within 2^-11 of
+         * (p + 0.5) / cnt for p < 512, far inside the 0.5 / cnt margin */
+        const float rc = __builtin_amdgcn_rcpf((float)cnt);
         for (uint32_t base = 0; base < total; base += 64) {
             const uint32_t p = base + (uint32_t)lane;
             const bool pv = p < total;
@@ -606,6 +608,13 @@ __global__ void k_ovf_compact(const uint32_t *__restrict__ ovf, const uint32_t *
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n && ovf[i]) list[pos[i]] = i;
+}
+
+/* waves of f queries -> the waves of one query that cover them */
+__global__ void k_wave_expand(const uint32_t *__restrict__ waves, uint32_t n, uint32_t f, uint32_t *__restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n * f) out[i] = waves[i / f] * f + i % f;
 }
 
 /* ---------------------------------------------- wave-per-query selection */
@@ -1090,6 +1099,17 @@ static void launch_select(const TreeView &tv, Arena &a, uint32_t nb, const uint3
                 LAUNCHCHK();
             }
         }
+        if (nr && rq > 1) {
+            /* a wave that still overflows after the split covers queries deep
+             * in a dense core: each of its queries walks in a wave of its own,
+             * so the few left run side by side instead of in one long wave */
+            uint32_t *w1 = a.get_t<uint32_t>("knn_owaves1", (size_t)nr * rq);
+            hipLaunchKernelGGL(k_wave_expand, dim3(ceil_div(nr * rq, 256)), dim3(256), 0, s, waves, nr, rq, w1);
+            LAUNCHCHK();
+            waves = w1;
+            nr *= rq;
+            rq = 1;
+        }
         if (nr) {
             double *sd2 = a.get_t<double>("knn_osd2", (size_t)nr * 64 * K);
             uint32_t *sidx = a.get_t<uint32_t>("knn_osidx", (size_t)nr * 64 * K);
@@ -1267,6 +1287,49 @@ __global__ void k_bfs_task_keys(const BallTask *__restrict__ tasks, uint32_t n, 
     if (k < n) keys[k] = make_uint4(tasks[k].begin, tasks[k].q, 0u, k);
 }
 
+/* One-workgroup bitonic sort of up to SMALL_SORT_N records by the 64-bit key
+ * (y:x), for the short task lists of ball searches, where the eight passes
+ * of the radix sort cost mostly launch latency.  The keys (query, begin) of
+ * a task list are distinct, so the order is the radix sort's.  p is a power
+ * of two >= n. */
+constexpr uint32_t SMALL_SORT_N = 4096, SMALL_SORT_T = 1024;
+__global__ __launch_bounds__(SMALL_SORT_T) void k_small_sort64(uint4 *__restrict__ keys, uint32_t n, uint32_t p)
+{
+    __shared__ uint64_t sk[SMALL_SORT_N];
+    __shared__ uint32_t sv[SMALL_SORT_N];
+    const uint32_t t = threadIdx.x;
+    for (uint32_t i = t; i < p; i += SMALL_SORT_T) {
+        if (i < n) {
+            const uint4 v = keys[i];
+            sk[i] = ((uint64_t)v.y << 32) | v.x;
+            sv[i] = v.w;
+        } else {
+            sk[i] = ~0ull;
+            sv[i] = PLAN_NONE;
+        }
+    }
+    __syncthreads();
+    for (uint32_t k = 2; k <= p; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = t; i < p; i += SMALL_SORT_T) {
+                const uint32_t l = i ^ j;
+                if (l > i) {
+                    const uint64_t a = sk[i], b = sk[l];
+                    if ((a > b) == ((i & k) == 0)) {
+                        sk[i] = b;
+                        sk[l] = a;
+                        const uint32_t x = sv[i];
+                        sv[i] = sv[l];
+                        sv[l] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    for (uint32_t i = t; i < n; i += SMALL_SORT_T)
+        keys[i] = make_uint4((uint32_t)sk[i], (uint32_t)(sk[i] >> 32), 0u, sv[i]);
+}
+
 __global__ void k_bfs_task_sizes(const BallTask *__restrict__ tasks, const uint4 *__restrict__ keys, uint32_t n,
                                  BallTask *__restrict__ sorted, uint32_t *__restrict__ sizes)
 {
@@ -1373,7 +1436,14 @@ void dev_ball_lists(const DevTree &t, Arena &a, const std::vector<double4> &quer
         uint4 *k0 = a.get_t<uint4>("bl_k0", nt), *k1 = a.get_t<uint4>("bl_k1", nt);
         hipLaunchKernelGGL(k_bfs_task_keys, dim3(ceil_div(nt, 256)), dim3(256), 0, s, tasks, nt, k0);
         LAUNCHCHK();
-        radix_sort_96(a, &k0, &k1, nt, 2, s);
+        if (nt <= SMALL_SORT_N) {
+            uint32_t p = 2;
+            while (p < nt) p <<= 1;
+            hipLaunchKernelGGL(k_small_sort64, dim3(1), dim3(SMALL_SORT_T), 0, s, k0, nt, p);
+            LAUNCHCHK();
+        } else {
+            radix_sort_96(a, &k0, &k1, nt, 2, s);
+        }
         BallTask *sorted = a.get_t<BallTask>("bl_sorted", nt);
         uint32_t *sizes = a.get_t<uint32_t>("bl_sizes", nt + 1);
         uint32_t *pre = a.get_t<uint32_t>("bl_pre", nt + 1);
