@@ -1266,11 +1266,14 @@ static void rebind_all(std::map<uint32_t, HClump> &cl, const HostMembers &hm, co
     part(2);
 }
 
+/* f on every clump (independent per clump), the largest first */
 template <class F> static void for_each_clump_parallel(std::map<uint32_t, HClump> &cl, F f)
 {
     std::vector<HClump *> v;
     v.reserve(cl.size());
     for (auto &it : cl) v.push_back(&it.second);
+    std::stable_sort(v.begin(), v.end(),
+                     [](const HClump *a, const HClump *b) { return a->indices.size() > b->indices.size(); });
     parallel_for((long)v.size(), [&](long i) { f(*v[i]); });
 }
 
@@ -1373,7 +1376,12 @@ void run_clump_pipeline(HostMembers &hm, const ClumpSettings &cs, ClumpServices 
             keys.push_back(it.first);
         }
         std::vector<uint8_t> empty(v.size(), 0);
-        parallel_for((long)v.size(), [&](long i) {
+        std::vector<size_t> big(v.size());
+        for (size_t i = 0; i < v.size(); ++i) big[i] = i;
+        std::stable_sort(big.begin(), big.end(),
+                         [&](size_t x, size_t y) { return v[x]->particles.size() > v[y]->particles.size(); });
+        parallel_for((long)v.size(), [&](long k) {
+            const size_t i = big[k];
             HClump &c = *v[i];
             while (c.hill < c.particles.back().second) {
                 auto low = std::lower_bound(c.particles.begin(), c.particles.end(), PD(0, c.hill), pd_less);
