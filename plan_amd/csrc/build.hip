@@ -238,6 +238,42 @@ __global__ void k_morton(const double4 *__restrict__ pos, uint32_t T, MortonFram
     keys[i] = morton_key(ix, iy, iz, i);
 }
 
+/* Keys sorted stably by their high 64 bits (words y, z) are in input order
+ * inside each run of equal high halves; sorting every such run by the low
+ * word, stably, gives the order of a stable sort by all 96 bits.  Runs hold
+ * particles within ~1e-7 of the box width of each other and are short; a run
+ * longer than RUN_FIX_MAX is reported (err) and the caller sorts by the full
+ * key instead. */
+constexpr uint32_t RUN_FIX_MAX = 64;
+__global__ void k_high_run_starts(const uint4 *__restrict__ keys, uint32_t T, uint32_t *__restrict__ start)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= T) return;
+    start[i] = (i == 0) ? 1u : ((keys[i].y != keys[i - 1].y || keys[i].z != keys[i - 1].z) ? 1u : 0u);
+}
+__global__ void k_high_run_fix(uint4 *__restrict__ keys, uint32_t T, const uint32_t *__restrict__ start,
+                               uint32_t *__restrict__ err)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= T || !start[i]) return;
+    uint32_t e = i + 1;
+    while (e < T && !start[e] && e - i <= RUN_FIX_MAX) ++e;
+    if (e - i == 1) return;
+    if (e - i > RUN_FIX_MAX) {
+        atomicOr(err, 1u);
+        return;
+    }
+    for (uint32_t a = i + 1; a < e; ++a) {
+        const uint4 v = keys[a];
+        uint32_t b = a;
+        while (b > i && keys[b - 1].x > v.x) {
+            keys[b] = keys[b - 1];
+            --b;
+        }
+        keys[b] = v;
+    }
+}
+
 __global__ void k_run_flags(const uint4 *__restrict__ keys, uint32_t T, uint32_t *__restrict__ flags)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -601,17 +637,38 @@ void dev_build_tree(DevTree &t, const DevParticles &ps, const plan_derived &d, b
     uint4 *k1 = a.get_t<uint4>("keys_b", T);
     hipLaunchKernelGGL(k_morton, dim3(ceil_div(T, BLK)), dim3(BLK), 0, s, ps.pos, T, f, k0);
     LAUNCHCHK();
-    radix_sort_96(a, &k0, &k1, T, 3, s);
-    uint4 *skeys = k0;
-
-    /* sink runs */
+    /* sort by the high 64 bits, then each run of equal high halves by the
+     * low word */
     uint32_t *flags = a.get_t<uint32_t>("flags", T + 1);
     uint32_t *ridx = a.get_t<uint32_t>("ridx", T + 1);
     uint32_t *tot = a.get_t<uint32_t>("tot", 8);
+    HIPCHK(hipMemsetAsync(tot + 1, 0, 4, s));
+    radix_sort_96(a, &k0, &k1, T, 3, s, 4);
+    hipLaunchKernelGGL(k_high_run_starts, dim3(ceil_div(T, BLK)), dim3(BLK), 0, s, k0, T, flags);
+    LAUNCHCHK();
+    hipLaunchKernelGGL(k_high_run_fix, dim3(ceil_div(T, BLK)), dim3(BLK), 0, s, k0, T, flags, tot + 1);
+    LAUNCHCHK();
+    uint4 *skeys = k0;
+
+    /* sink runs */
     hipLaunchKernelGGL(k_run_flags, dim3(ceil_div(T, BLK)), dim3(BLK), 0, s, skeys, T, flags);
     LAUNCHCHK();
     scan_exclusive_u32(a, flags, ridx, T, s, tot);
-    const uint32_t n = read_u32(tot, s);
+    uint32_t h2[2];
+    HIPCHK(hipMemcpyAsync(h2, tot, sizeof(h2), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (h2[1]) {
+        /* a long run of equal high halves: stable sort by the full key (equal
+         * full keys are in input order already) */
+        radix_sort_96(a, &k0, &k1, T, 3, s);
+        skeys = k0;
+        hipLaunchKernelGGL(k_run_flags, dim3(ceil_div(T, BLK)), dim3(BLK), 0, s, skeys, T, flags);
+        LAUNCHCHK();
+        scan_exclusive_u32(a, flags, ridx, T, s, tot);
+        HIPCHK(hipMemcpyAsync(h2, tot, sizeof(h2), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    const uint32_t n = h2[0];
     t.n = n;
     t.num_sinks = 0;
     uint32_t *start = a.get_t<uint32_t>("rstart", n + 1);

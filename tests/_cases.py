@@ -9,6 +9,23 @@ def dense_case(n, n_clumps, seed, a=4e-4, frac=0.3, knn=32, hop=None, nx=None, d
                                nx=nx, knn=knn, hop=hop)
 
 
+def duplicates_case():
+    """A clumpy snapshot with 100 particles at one position (one sink, a run
+    of equal keys longer than the sort's run fix-up takes) and 40 particles a
+    few float32 steps apart in x (equal high 64 key bits, distinct low word)."""
+    import numpy as np
+    snap = dense_case(8192, 6, 8, a=3e-4)
+    pos = snap.pos
+    pos[100:200] = pos[100]
+    base = pos[300].copy()
+    for k in range(40):
+        pos[300 + k] = base
+        pos[300 + k, 0] = np.nextafter(base[0], np.float32(1), dtype=np.float32) if k else base[0]
+        for _ in range(k - 1):
+            pos[300 + k, 0] = np.nextafter(pos[300 + k, 0], np.float32(1), dtype=np.float32)
+    return snap
+
+
 CASES = {
     "n4k": lambda: dense_case(4096, 4, 1, a=2e-4),
     "n8k": lambda: dense_case(8192, 6, 2, a=3e-4),
@@ -22,4 +39,5 @@ CASES = {
     # the reference's CPU-runnable configuration: 262,144 particles, 16
     # Plummer spheres in 10%, K = 32, seed 0
     "c1": lambda: synth.config_c1(seed=0),
+    "dup": duplicates_case,
 }
