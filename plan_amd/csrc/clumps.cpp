@@ -866,52 +866,52 @@ struct EnergyState {
     double M, cx, cy, cz, vx, vy, vz, sh, G;
 };
 
-/* rebind_energy for candidates [0, n) of a block held as arrays: the same
+/* rebind_energy for the candidates idx[0, n) of the live arrays: the same
  * expression element by element (written once, compiled for the baseline,
  * AVX2 and AVX-512; IEEE operations give the same bits at any vector width
  * and nothing is fused) */
 #define PLAN_ENERGY_BLOCK_BODY                                                    \
     for (size_t j = 0; j < n; ++j) {                                              \
-        const double tm = S.M + m[j];                                             \
-        const double a = S.cx - x[j];                                             \
+        const uint32_t k = idx[j];                                                \
+        const double tm = S.M + m[k];                                             \
+        const double a = S.cx - x[k];                                             \
         const double svx = 0.0 * a, svy = S.sh * a, svz = 0.0 * a;                \
         double n2 = 0;                                                            \
-        const double d0 = (S.vx - svx) - vx[j];                                   \
+        const double d0 = (S.vx - svx) - vx[k];                                   \
         n2 += d0 * d0;                                                            \
-        const double d1 = (S.vy - svy) - vy[j];                                   \
+        const double d1 = (S.vy - svy) - vy[k];                                   \
         n2 += d1 * d1;                                                            \
-        const double d2 = (S.vz - svz) - vz[j];                                   \
+        const double d2 = (S.vz - svz) - vz[k];                                   \
         n2 += d2 * d2;                                                            \
         double q = 0;                                                             \
-        const double t0 = S.cx - x[j];                                            \
+        const double t0 = S.cx - x[k];                                            \
         q += t0 * t0;                                                             \
-        const double t1 = S.cy - y[j];                                            \
+        const double t1 = S.cy - y[k];                                            \
         q += t1 * t1;                                                             \
-        const double t2 = S.cz - z[j];                                            \
+        const double t2 = S.cz - z[k];                                            \
         q += t2 * t2;                                                             \
         e[j] = (0.5 / tm) * n2 - S.G / std::sqrt(q);                              \
     }
-static void energy_block_base(const EnergyState &S, const double *x, const double *y, const double *z,
-                              const double *vx, const double *vy, const double *vz, const double *m, size_t n,
-                              double *e)
+#define PLAN_ENERGY_BLOCK_ARGS                                                                                      \
+    const EnergyState &S, const double *__restrict x, const double *__restrict y, const double *__restrict z,    \
+        const double *__restrict vx, const double *__restrict vy, const double *__restrict vz,                    \
+        const double *__restrict m, const uint32_t *__restrict idx, size_t n, double *__restrict e
+static void energy_block_base(PLAN_ENERGY_BLOCK_ARGS)
 {
     PLAN_ENERGY_BLOCK_BODY
 }
-__attribute__((target("avx2"))) static void energy_block_avx2(const EnergyState &S, const double *x, const double *y,
-                                                               const double *z, const double *vx, const double *vy,
-                                                               const double *vz, const double *m, size_t n, double *e)
+__attribute__((target("avx2"))) static void energy_block_avx2(PLAN_ENERGY_BLOCK_ARGS)
 {
     PLAN_ENERGY_BLOCK_BODY
 }
-__attribute__((target("avx512f,avx512bw,avx512vl"))) static void
-energy_block_avx512(const EnergyState &S, const double *x, const double *y, const double *z, const double *vx,
-                    const double *vy, const double *vz, const double *m, size_t n, double *e)
+__attribute__((target("avx512f,avx512bw,avx512vl"))) static void energy_block_avx512(PLAN_ENERGY_BLOCK_ARGS)
 {
     PLAN_ENERGY_BLOCK_BODY
 }
 #undef PLAN_ENERGY_BLOCK_BODY
+#undef PLAN_ENERGY_BLOCK_ARGS
 typedef void (*EnergyBlockFn)(const EnergyState &, const double *, const double *, const double *, const double *,
-                              const double *, const double *, const double *, size_t, double *);
+                              const double *, const double *, const double *, const uint32_t *, size_t, double *);
 static EnergyBlockFn energy_block_fn()
 {
     static const EnergyBlockFn f = cpu_has_avx512()                 ? energy_block_avx512
@@ -1044,7 +1044,7 @@ static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
     size_t ndead = 0;
     constexpr size_t BLK = 32;
     const double inf = std::numeric_limits<double>::infinity();
-    double eb[BLK], bx[BLK], by[BLK], bz[BLK], bvx[BLK], bvy[BLK], bvz[BLK], bm[BLK];
+    double eb[BLK];
     const EnergyBlockFn energy_block = energy_block_fn();
     const LiveScanFn live_scan = live_scan_fn();
     size_t before;
@@ -1115,18 +1115,9 @@ static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
             const size_t nc = cpos.size();
             while (i < nc) {
                 const size_t e = std::min(nc, i + BLK);
-                for (size_t j = i; j < e; ++j) {
-                    const uint32_t k = cpos[j];
-                    bx[j - i] = live.x[k];
-                    by[j - i] = live.y[k];
-                    bz[j - i] = live.z[k];
-                    bvx[j - i] = live.vx[k];
-                    bvy[j - i] = live.vy[k];
-                    bvz[j - i] = live.vz[k];
-                    bm[j - i] = live.m[k];
-                }
                 const EnergyState es{c.total_mass, c.com[0], c.com[1], c.com[2], c.vel[0], c.vel[1], c.vel[2], sh, cs.G};
-                energy_block(es, bx, by, bz, bvx, bvy, bvz, bm, e - i, eb);
+                energy_block(es, live.x.data(), live.y.data(), live.z.data(), live.vx.data(), live.vy.data(),
+                             live.vz.data(), live.m.data(), cpos.data() + i, e - i, eb);
                 size_t f = e;
                 for (size_t j = i; j < e; ++j)
                     if (eb[j - i] < 0) {
