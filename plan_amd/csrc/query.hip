@@ -473,13 +473,15 @@ __device__ __forceinline__ bool walk_fill(const TreeView &tv, NodeCache &nc, Wal
 /* LDS of the group-wise walk: a stack of sibling groups still to visit
  * (first node, count, mask of the queries that descended into their
  * parent), the rank -> query table of the group being evaluated and the
- * per-child result masks. */
-constexpr int WALK_STACK = 160;
+ * per-child result masks.  The stack depth is sized so that a K <= 32
+ * select wave fits 6 KB of LDS (26 waves per CU instead of 22 at 7 KB); a
+ * walk that needs more is treated as a list overflow. */
+constexpr int WALK_STACK = 108;
 struct DenseWalkLds {
     uint32_t sfd[WALK_STACK];
     uint64_t smask[WALK_STACK];
     uint8_t snd[WALK_STACK];
-    uint32_t rk[64];
+    uint8_t rk[64];
     uint64_t inc[8];
     uint64_t desc[8];
 };
@@ -534,7 +536,7 @@ __device__ bool walk_dense(const TreeView &tv, NodeCache &nc, DenseWalkLds &D, L
             nc.t[lane] = tv.ntop[fd + lane];
             nc.info[lane] = tv.ninfo[fd + lane];
         }
-        if ((M >> lane) & 1ull) D.rk[mbcnt64(M)] = (uint32_t)lane;
+        if ((M >> lane) & 1ull) D.rk[mbcnt64(M)] = (uint8_t)lane;
         if (lane < 8) {
             D.inc[lane] = 0;
             D.desc[lane] = 0;
@@ -716,21 +718,23 @@ __device__ __forceinline__ void merge64(double &bd, uint32_t &bi, double cd, uin
 }
 
 /* LDS of one select wave: node cache, the include list of the walk and the
- * compacted entries of the query being selected */
+ * compacted entries (list index, candidate prefix) of the query being
+ * selected */
 template <int CAP> struct SelectLds {
     NodeCache nc;
     ListEnt lst[CAP];
     union {
         struct {
-            uint32_t sbeg[CAP];
+            uint16_t sent[CAP];
             uint32_t spref[CAP + 1];
         };
         DenseWalkLds dw;
     };
 };
+static_assert(sizeof(SelectLds<256>) <= 6144, "a K <= 32 select wave should fit 6 KB of LDS");
 
 /* Feed the candidates of query bit j of the recorded list to the running
- * best-64 (bd, bi): the entries of the query are compacted into sbeg/spref,
+ * best-64 (bd, bi): the entries of the query are compacted into sent/spref,
  * then candidates are taken 64 at a time (lane c of a chunk locates its
  * particle by binary search over the prefix), measured, filtered by T,
  * sorted and merged. */
@@ -755,7 +759,7 @@ __device__ __forceinline__ void select_list(const TreeView &tv, LDS &L, uint32_t
         for (uint32_t e = lane; e < nent; e += 64) {
             const ListEnt le = L.lst[e];
             if ((le.mask >> j) & 1ull) {
-                L.sbeg[slot] = le.begin;
+                L.sent[slot] = (uint16_t)e;
                 L.spref[slot] = pref;
                 ++slot;
                 pref += le.end - le.begin;
@@ -778,7 +782,7 @@ __device__ __forceinline__ void select_list(const TreeView &tv, LDS &L, uint32_t
                 const uint32_t t = lo + step;
                 lo = (t < M && L.spref[t] <= c) ? t : lo;
             }
-            const uint32_t p = L.sbeg[lo] + (c - L.spref[lo]);
+            const uint32_t p = L.lst[L.sent[lo]].begin + (c - L.spref[lo]);
             const double4 x = tv.pm[p];
             const double dd = dist2(qx, qy, qz, x.x, x.y, x.z);
             if (dd <= T) {
