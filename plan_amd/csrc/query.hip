@@ -412,9 +412,12 @@ struct ListEnt {
     uint32_t begin, end;
     uint64_t mask;
 };
-/* include-list capacity (entries) of one wave: larger K walks larger spheres */
+/* include-list capacity (entries) of one wave: larger K walks larger spheres.
+ * The list is most of a wave's LDS, so the capacity trades occupancy against
+ * the waves that overflow it and are redone in smaller pieces (at 16.8M
+ * particles, 384 entries for K = 64 beat 256, 320, 448 and 512). */
 template <int K> struct ListCap {
-    static constexpr int value = K >= 64 ? 512 : 256;
+    static constexpr int value = K >= 64 ? 384 : 256;
 };
 
 struct WalkState {
