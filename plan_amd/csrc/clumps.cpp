@@ -182,6 +182,7 @@ struct Ops {
     void sort_particles(HClump &c, bool nearly_sorted = false) const
     {
         kinematics(c);
+        c.particles.reserve(c.particles.size() + c.indices.size());
         for (uint32_t h : c.indices) c.particles.push_back(PD(h, dist_to(h, c.com)));
         sort_by_distance(c.particles, nearly_sorted);
         for (size_t i = 0; i < c.particles.size(); ++i) c.indices[i] = c.particles[i].first;
@@ -470,6 +471,27 @@ double rho_of_key(const HostMembers &hm, uint32_t tidx)
     return hm.rec[r].rho;
 }
 
+/* Keys, in key order, of the clumps (the masked ones only, if asked) for
+ * which IsPositionDispersion2Large holds; the checks, a sequential sum over
+ * each clump's members, run in parallel, the largest clumps first. */
+static std::vector<uint32_t> dispersed_keys(std::map<uint32_t, HClump> &cl, double tol, bool masked_only)
+{
+    std::vector<std::pair<uint32_t, const HClump *>> v;
+    for (auto &it : cl)
+        if (!masked_only || it.second.mask) v.push_back(std::make_pair(it.first, &it.second));
+    std::vector<size_t> order(v.size());
+    for (size_t i = 0; i < v.size(); ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+        return v[x].second->particles.size() > v[y].second->particles.size();
+    });
+    std::vector<uint8_t> f(v.size(), 0);
+    parallel_for((long)v.size(), [&](long k) { f[order[k]] = Ops::dispersion_too_large(*v[order[k]].second, tol); });
+    std::vector<uint32_t> out;
+    for (size_t i = 0; i < v.size(); ++i)
+        if (f[i]) out.push_back(v[i].first);
+    return out;
+}
+
 /* one merge loop (src/tree.hpp:3863-4006 with one10th radii, 4011-4152 with
  * Hill radii), serial semantics: predators in key order; a clump eaten in a
  * pass cannot eat in it; merges applied after the pass in pair order */
@@ -623,11 +645,9 @@ void merge_loop(std::map<uint32_t, HClump> &cl, const HostMembers &hm, const Clu
             }
         }
         st.mparts[2] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t2).count();
-        for (auto &it : cl) {
-            if (it.second.mask && Ops::dispersion_too_large(it.second)) {
-                it.second.mask = false;
-                deleted.push_back(it.first);
-            }
+        for (uint32_t k : dispersed_keys(cl, 0.55, true)) {
+            cl[k].mask = false;
+            deleted.push_back(k);
         }
     }
     for (auto k : deleted) cl.erase(k);
@@ -672,6 +692,12 @@ struct RbLocal {
     std::vector<uint32_t> cnt;
     size_t n_sorted = 0;
     size_t size() const { return h.size(); }
+    /* room for the members plus the superset they are claimed from */
+    void reserve(size_t n)
+    {
+        for (auto *a : {&x, &y, &z, &m, &vx, &vy, &vz, &rho}) a->reserve(n);
+        h.reserve(n);
+    }
     void push(const MemberRec &r, uint32_t hh)
     {
         x.push_back(r.x);
@@ -809,6 +835,11 @@ struct LiveSet {
     {
         for (auto *a : {&x, &y, &z, &m, &vx, &vy, &vz, &lsd}) a->clear();
         h.clear();
+    }
+    void reserve(size_t n)
+    {
+        for (auto *a : {&x, &y, &z, &m, &vx, &vy, &vz, &lsd}) a->reserve(n);
+        h.reserve(n);
     }
     void push(const MemberRec &q, uint32_t hh, double l)
     {
@@ -1029,11 +1060,13 @@ static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
     LiveSet live;
     auto rebuild = [&]() {
         live.clear();
+        live.reserve(rb.sup.size());
         for (uint32_t h : rb.sup)
             if (!flags[h] && !mine.count(h)) live.push(hm.rec[h], h, Ops::dist_pts(rb.c0, &hm.rec[h].x));
     };
     rebuild();
     RbLocal L;
+    L.reserve(c.indices.size() + rb.sup.size());
     for (uint32_t h : c.indices) L.push(hm.rec[h], h);
     L.ord.resize(L.size());
     for (size_t i = 0; i < L.size(); ++i) L.ord[i] = (uint32_t)i;
@@ -1293,12 +1326,8 @@ void run_clump_pipeline(HostMembers &hm, const ClumpSettings &cs, ClumpServices 
     }
     for_each_clump_parallel(cl, [&](HClump &h) { ops.sort_particles(h); });
     std::vector<uint32_t> del;
-    if (cs.diffuse < 0.525) {
-        for (auto &it : cl)
-            if (Ops::dispersion_too_large(it.second, cs.diffuse)) del.push_back(it.first);
-        for (auto k : del) cl.erase(k);
-        del.clear();
-    }
+    if (cs.diffuse < 0.525)
+        for (uint32_t k : dispersed_keys(cl, cs.diffuse, false)) cl.erase(k);
     if (!cs.enough_resolution) remove_small_mass_low_peak(cl, hm, cs);
     st.naive = (uint32_t)cl.size();
     lap(0);
@@ -1351,11 +1380,7 @@ void run_clump_pipeline(HostMembers &hm, const ClumpSettings &cs, ClumpServices 
     lap(5);
     /* 6: filters */
     remove_small_mass_low_peak(cl, hm, cs);
-    for (auto &it : cl)
-        if (it.second.mask && Ops::dispersion_too_large(it.second, std::min(cs.diffuse + 0.1, 0.475)))
-            del.push_back(it.first);
-    for (auto k : del) cl.erase(k);
-    del.clear();
+    for (uint32_t k : dispersed_keys(cl, std::min(cs.diffuse + 0.1, 0.475), true)) cl.erase(k);
 
     lap(6);
     /* 7: envelope ripping (src/tree.hpp:4229-4249) */
