@@ -110,8 +110,8 @@ __device__ __forceinline__ bool sphere_intersect(const NodeGeo &g, double qx, do
 }
 
 /* The reference's node decision for one (node, query) pair, evaluated by
- * all lanes of a wave: 1 = take the node's particles (contained node, or
- * intersecting leaf), 2 = descend, 3 = reject.  Same binary64 operations as
+ * all lanes of a wave: take the node's particles (0 = the sphere contains
+ * the node, 1 = intersecting leaf), 2 = descend, 3 = reject.  Same binary64 operations as
  * sphere_contain and sphere_intersect.  The bounding-sphere rejection, the
  * containment test and the squared-vs-linear comparisons decide nearly every
  * pair; the box distance is computed only when some lane of the wave needs
@@ -142,7 +142,7 @@ __device__ __forceinline__ int classify_sphere(const NodeGeo &g, bool leaf, doub
         const double md = __dadd_rn(__dadd_rn(__dmul_rn(ex, ex), __dmul_rn(ey, ey)), __dmul_rn(ez, ez));
         inter = inter || (!far && md <= r2);
     }
-    return contain ? 1 : (inter ? (leaf ? 1 : 2) : 3);
+    return contain ? 0 : (inter ? (leaf ? 1 : 2) : 3);
 }
 
 /* Geometric node tests for exact distance queries (a particle lies within
@@ -480,14 +480,19 @@ __device__ __forceinline__ bool walk_fill(const TreeView &tv, NodeCache &nc, Wal
  * select wave fits 6 KB of LDS (26 waves per CU instead of 22 at 7 KB); a
  * walk that needs more is treated as a list overflow. */
 constexpr int WALK_STACK = 108;
-struct DenseWalkLds {
-    uint32_t sfd[WALK_STACK];
-    uint64_t smask[WALK_STACK];
-    uint8_t snd[WALK_STACK];
+/* A depth-first walk holds at most 7 pending sibling groups per level of
+ * its current path (32 levels) plus the root: 256 groups never overflow. */
+constexpr int WALK_STACK_FULL = 256;
+static_assert(WALK_STACK_FULL >= 1 + 7 * 31, "a full-depth walk fits the stack");
+template <int STACK> struct DenseWalkLdsT {
+    uint32_t sfd[STACK];
+    uint64_t smask[STACK];
+    uint8_t snd[STACK];
     uint8_t rk[64];
     uint64_t inc[8];
     uint64_t desc[8];
 };
+using DenseWalkLds = DenseWalkLdsT<WALK_STACK>;
 
 /* Workgroups are dispatched round-robin over the 8 XCDs (block b runs on
  * XCD b % 8), each with its own L2.  Neighbouring waves of Morton-ordered
@@ -506,27 +511,25 @@ __device__ __forceinline__ uint32_t mbcnt64(uint64_t m)
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+enum class WalkEnd { done, list_full, stack_full };
+
 /* The walk of RecursiveKNN_OpenMP (src/tree.hpp:3177-3193) for the queries
- * of mask M0 (lane = query, radius r per lane), one sibling group per step:
- * the (child, query) pairs of a group whose parent some queries descended
- * into are packed densely over the lanes and classified, then the children
- * some query takes go to the include list (with the mask of those queries)
- * and the children some query descends into push their own groups.  The
- * set of (node, query) decisions is exactly the recursive walk's.  Returns
- * false when the list or the group stack is full. */
-template <int CAP>
-__device__ bool walk_dense(const TreeView &tv, NodeCache &nc, DenseWalkLds &D, ListEnt *lst, uint32_t &nent, double qx,
-                           double qy, double qz, double r, uint64_t M0)
+ * of the groups on the LDS stack (lane = query, radius r per lane), one
+ * sibling group per step: the (child, query) pairs of a group whose parent
+ * some queries descended into are packed densely over the lanes and
+ * classified, then the children some query takes go to the include list
+ * (with the mask of those queries) and the children some query descends
+ * into push their own groups.  The set of (node, query) decisions is exactly
+ * the recursive walk's.  COUNT walks keep no list: each lane adds the sizes
+ * of the nodes it takes (sc) and of those its sphere contains (cc).  When
+ * the list is full the group being decided goes back on the stack, so a
+ * later call continues the walk (sp is the stack height). */
+template <int CAP, int STACK, bool COUNT>
+__device__ WalkEnd walk_dense_run(const TreeView &tv, NodeCache &nc, DenseWalkLdsT<STACK> &D, uint64_t *cont,
+                                  ListEnt *lst, uint32_t &nent, double qx, double qy, double qz, double r,
+                                  uint32_t &sp, uint32_t &sc, uint32_t &cc)
 {
     const int lane = threadIdx.x;
-    if (M0 == 0) return true;
-    __syncthreads();
-    if (lane == 0) {
-        D.sfd[0] = 0;
-        D.snd[0] = 1;
-        D.smask[0] = M0;
-    }
-    uint32_t sp = 1;
     while (sp > 0) {
         __syncthreads();
         --sp;
@@ -543,6 +546,7 @@ __device__ bool walk_dense(const TreeView &tv, NodeCache &nc, DenseWalkLds &D, L
         if (lane < 8) {
             D.inc[lane] = 0;
             D.desc[lane] = 0;
+            if (COUNT) cont[lane] = 0;
         }
         __syncthreads();
         const uint32_t cnt = (uint32_t)__popcll(M);
@@ -562,10 +566,18 @@ __device__ bool walk_dense(const TreeView &tv, NodeCache &nc, DenseWalkLds &D, L
             const NodeGeo g = nc.g[c];
             const bool leaf = (nc.info[c] & 0xffu) == 0u;
             const int act = classify_sphere(g, leaf, xj, yj, zj, rj, tv.to_diag);
-            if (pv && act == 1) atomicOr((unsigned long long *)&D.inc[c], 1ull << j);
+            if (pv && act <= 1) atomicOr((unsigned long long *)&D.inc[c], 1ull << j);
             if (pv && act == 2) atomicOr((unsigned long long *)&D.desc[c], 1ull << j);
+            if (COUNT && pv && act == 0) atomicOr((unsigned long long *)&cont[c], 1ull << j);
         }
         __syncthreads();
+        if (COUNT) {
+            for (uint32_t c = 0; c < nd; ++c) {
+                const uint32_t sz = nc.t[c].end - nc.t[c].begin;
+                if ((D.inc[c] >> lane) & 1ull) sc += sz;
+                if ((cont[c] >> lane) & 1ull) cc += sz;
+            }
+        }
         uint64_t I = 0, Dm = 0;
         NodeTop t{0u, 0u, 0u, 0u};
         uint32_t cnd = 0;
@@ -575,14 +587,19 @@ __device__ bool walk_dense(const TreeView &tv, NodeCache &nc, DenseWalkLds &D, L
             t = nc.t[lane];
             cnd = nc.info[lane] & 0xffu;
         }
-        const uint64_t mi = __ballot(I != 0ull);
-        const uint32_t ni = (uint32_t)__popcll(mi);
-        if (nent + ni > min((uint32_t)CAP, tv.list_limit)) return false;
-        if (I != 0ull) lst[nent + mbcnt64(mi)] = ListEnt{t.begin, t.end, I};
-        nent += ni;
+        if (!COUNT) {
+            const uint64_t mi = __ballot(I != 0ull);
+            const uint32_t ni = (uint32_t)__popcll(mi);
+            if (nent + ni > min((uint32_t)CAP, tv.list_limit)) {
+                ++sp; /* the group is decided again by the next call */
+                return WalkEnd::list_full;
+            }
+            if (I != 0ull) lst[nent + mbcnt64(mi)] = ListEnt{t.begin, t.end, I};
+            nent += ni;
+        }
         const uint64_t md = __ballot(Dm != 0ull);
         const uint32_t nD = (uint32_t)__popcll(md);
-        if (sp + nD > (uint32_t)WALK_STACK) return false;
+        if (sp + nD > (uint32_t)STACK) return WalkEnd::stack_full;
         __syncthreads();
         if (Dm != 0ull) {
             /* reversed, so the first child's group is popped first */
@@ -593,7 +610,29 @@ __device__ bool walk_dense(const TreeView &tv, NodeCache &nc, DenseWalkLds &D, L
         }
         sp += nD;
     }
-    return true;
+    return WalkEnd::done;
+}
+
+/* start a walk at the root for the queries of mask M0 */
+template <int STACK> __device__ __forceinline__ uint32_t walk_dense_start(DenseWalkLdsT<STACK> &D, uint64_t M0)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        D.sfd[0] = 0;
+        D.snd[0] = 1;
+        D.smask[0] = M0;
+    }
+    return M0 ? 1u : 0u;
+}
+
+/* the whole walk into one list; false when the list or the stack is full */
+template <int CAP>
+__device__ bool walk_dense(const TreeView &tv, NodeCache &nc, DenseWalkLds &D, ListEnt *lst, uint32_t &nent, double qx,
+                           double qy, double qz, double r, uint64_t M0)
+{
+    uint32_t sp = walk_dense_start(D, M0), sc = 0, cc = 0;
+    return walk_dense_run<CAP, WALK_STACK, false>(tv, nc, D, nullptr, lst, nent, qx, qy, qz, r, sp, sc, cc) ==
+           WalkEnd::done;
 }
 
 /* |S(r)| of each query from the include-list entries [e0, e1) */
@@ -735,6 +774,16 @@ template <int CAP> struct SelectLds {
     };
 };
 static_assert(sizeof(SelectLds<256>) <= 6144, "a K <= 32 select wave should fit 6 KB of LDS");
+/* the few waves that walk in several fills keep the walk's stack beside the
+ * selection's arrays, so the walk resumes after each selection */
+template <int CAP> struct ResumeLds {
+    NodeCache nc;
+    ListEnt lst[CAP];
+    uint16_t sent[CAP];
+    uint32_t spref[CAP + 1];
+    DenseWalkLdsT<WALK_STACK_FULL> dw;
+    uint64_t cont[8];
+};
 
 /* Feed the candidates of query bit j of the recorded list to the running
  * best-64 (bd, bi): the entries of the query are compacted into sent/spref,
@@ -910,7 +959,8 @@ __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, con
                                                    double *__restrict__ d2_out, uint32_t qpw)
 {
     constexpr int CAP = ListCap<K>::value;
-    __shared__ SelectLds<CAP> L;
+    using Lds = std::conditional_t<RESUME, ResumeLds<CAP>, SelectLds<CAP>>;
+    __shared__ Lds L;
     const int lane = threadIdx.x;
     /* RESUME: waves[] lists the waves to redo.  Otherwise waves[] (when
      * given) lists 64-query waves to split into 64 / qpw waves each. */
@@ -933,7 +983,7 @@ __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, con
     WalkState w{0u, valid ? 0u : 0xffffffffu, 1u, 0u};
     uint32_t nent = 0;
     bool filled = false;
-    if (FUSE && !RESUME) {
+    if constexpr (FUSE && !RESUME) {
         /* the radius schedule r0, 2 r0, ... until |S(r)| >= K
          * (src/tree.hpp:3206-3223): each round walks every query at its
          * current radius; the queries short of K candidates double theirs.
@@ -957,36 +1007,28 @@ __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, con
             c_io[q] = 0u;
         }
         w = WalkState{0u, valid ? 0u : 0xffffffffu, 1u, 0u};
-    } else if (FUSE) {
-        /* the radius schedule r0, 2 r0, ... until |S(r)| >= K
-         * (src/tree.hpp:3206-3223), each round a recording walk; the list of
-         * the last round is the one selected from */
-        bool need = valid;
+    } else if constexpr (FUSE) {
+        /* waves that walk in several fills: the radius schedule r0, 2 r0, ...
+         * until |S(r)| >= K (src/tree.hpp:3206-3223) by counting walks, which
+         * also sum the particles of the nodes each sphere contains */
+        const uint64_t V = __ballot(valid);
         for (int round = 0; round < 2200; ++round) {
             uint32_t sc = 0, cc = 0;
-            w = WalkState{0u, valid ? 0u : 0xffffffffu, 1u, 0u};
-            nent = 0;
-            bool fin = true;
-            if (!RESUME) fin = walk_fill<CAP>(tv, L.nc, w, qx, qy, qz, r, false, L.lst, nent, &sc, &cc);
-            else walk_count(tv, L.nc, qx, qy, qz, r, valid, sc, cc);
-            if (!fin) {
-                if (lane == 0) ovf[wv] = 1u;
-                return;
-            }
-            need = valid && sc < (uint32_t)K;
+            uint32_t sp = walk_dense_start(L.dw, V);
+            walk_dense_run<CAP, WALK_STACK_FULL, true>(tv, L.nc, L.dw, L.cont, L.lst, nent, qx, qy, qz, r, sp, sc,
+                                                       cc);
+            const bool need = valid && sc < (uint32_t)K;
             if (!need) cin = cc;
             if (!__ballot(need)) break;
             __syncthreads();
             if (need) r = __dmul_rn(r, 2.0);
         }
-        filled = !RESUME;
         if (valid) {
             r_io[q] = r;
             c_io[q] = cin;
         }
-        w = WalkState{0u, valid ? 0u : 0xffffffffu, 1u, 0u};
-    } else if (valid) {
-        cin = c_io[q];
+    } else {
+        if (valid) cin = c_io[q];
     }
     /* Candidates in nodes the sphere contains lie within r* (a contained
      * node's box is inside the sphere with a margin far above the Morton
@@ -1004,7 +1046,7 @@ __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, con
         r = rt_in[q];
         T = r * r;
     }
-    if (!RESUME) {
+    if constexpr (!RESUME) {
         if (!filled) {
             nent = 0;
             const bool fin = walk_fill<CAP>(tv, L.nc, w, qx, qy, qz, r, geo, L.lst, nent);
@@ -1032,9 +1074,18 @@ __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, con
                 bix[j * K + lane] = PLAN_NONE;
             }
         bool fin = false;
+        uint32_t sp = 0;
+        if constexpr (FUSE) sp = walk_dense_start(L.dw, __ballot(valid));
         while (!fin) {
             nent = 0;
-            fin = walk_fill<CAP>(tv, L.nc, w, qx, qy, qz, r, geo, L.lst, nent);
+            if constexpr (FUSE) {
+                /* the group walk resumes where the full list stopped it */
+                uint32_t sc = 0, cc = 0;
+                fin = walk_dense_run<CAP, WALK_STACK_FULL, false>(tv, L.nc, L.dw, nullptr, L.lst, nent, qx, qy, qz,
+                                                                   r, sp, sc, cc) != WalkEnd::list_full;
+            } else {
+                fin = walk_fill<CAP>(tv, L.nc, w, qx, qy, qz, r, geo, L.lst, nent);
+            }
             __syncthreads();
             for (uint32_t j = 0; j < nvalid; ++j) {
                 double bd = lane < K ? bd2[j * K + lane] : inf;
