@@ -230,8 +230,9 @@ struct Ops {
         }
     }
     /* std::sort of v = A ++ B by distance, where A (the first na entries) is
-     * usually nearly sorted: A by insertion (by std::sort when that takes too
-     * many moves), B by std::sort, then a linear merge.  Without distance
+     * usually nearly sorted: A by insertion (by the distance-bucket sort when
+     * that takes too many moves), B by the bucket sort, then a linear merge.
+     * Without distance
      * ties the sorted order is unique and equals std::sort's; on a tie
      * std::sort runs on the original v, or, when strict, v is left unsorted
      * and false is returned. */
@@ -241,7 +242,8 @@ struct Ops {
             std::sort(v.begin(), v.end(), pd_less);
             return true;
         }
-        static thread_local std::vector<PD> a, b, m;
+        static thread_local std::vector<PD> a, b, m, tmp;
+        static thread_local std::vector<uint32_t> cnt;
         a.assign(v.begin(), v.begin() + na);
         b.assign(v.begin() + na, v.end());
         size_t budget = 16 * a.size() + 1024, moves = 0;
@@ -259,10 +261,10 @@ struct Ops {
             }
             a[j] = x;
         }
-        if (far) std::sort(a.begin(), a.end(), pd_less);
+        if (far) sort_by_distance_buckets(a, tmp, cnt);
         bool ok = true;
         {
-            std::sort(b.begin(), b.end(), pd_less);
+            sort_by_distance_buckets(b, tmp, cnt);
             m.resize(v.size());
             std::merge(a.begin(), a.end(), b.begin(), b.end(), m.begin(), pd_less);
             for (size_t i = 1; i < m.size(); ++i)
