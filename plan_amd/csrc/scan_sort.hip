@@ -224,9 +224,54 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const uint4 *__restrict
     }
 }
 
+/* One-workgroup bitonic sort of up to SMALL96_N records by (w2, w1, w0,
+ * payload).  Every caller's payload is the record's input position, so this
+ * order is the stable LSD sort's, for any key words the caller sorts on:
+ * ordering by the further words too only fixes what a later pass or fix-up
+ * would.  Small sorts (clump trees, small snapshots) then cost one launch
+ * instead of three per digit. */
+constexpr uint32_t SMALL96_N = 2048, SMALL96_T = 1024;
+__device__ __forceinline__ bool key96_less(const uint4 &a, const uint4 &b)
+{
+    if (a.z != b.z) return a.z < b.z;
+    if (a.y != b.y) return a.y < b.y;
+    if (a.x != b.x) return a.x < b.x;
+    return a.w < b.w;
+}
+__global__ __launch_bounds__(SMALL96_T) void k_small_sort96(uint4 *__restrict__ keys, uint32_t n, uint32_t p)
+{
+    __shared__ uint4 sk[SMALL96_N];
+    const uint32_t t = threadIdx.x;
+    for (uint32_t i = t; i < p; i += SMALL96_T)
+        sk[i] = i < n ? keys[i] : make_uint4(0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu);
+    __syncthreads();
+    for (uint32_t k = 2; k <= p; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = t; i < p; i += SMALL96_T) {
+                const uint32_t l = i ^ j;
+                if (l > i) {
+                    const uint4 x = sk[i], y = sk[l];
+                    if (key96_less(y, x) == ((i & k) == 0)) {
+                        sk[i] = y;
+                        sk[l] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    for (uint32_t i = t; i < n; i += SMALL96_T) keys[i] = sk[i];
+}
+
 void radix_sort_96(Arena &a, uint4 **keys, uint4 **tmp, uint32_t n, int nwords, hipStream_t s, int first_digit)
 {
     if (n <= 1) return;
+    if (n <= SMALL96_N) {
+        uint32_t p = 2;
+        while (p < n) p <<= 1;
+        hipLaunchKernelGGL(k_small_sort96, dim3(1), dim3(SMALL96_T), 0, s, *keys, n, p);
+        LAUNCHCHK();
+        return;
+    }
     unsigned nb = ceil_div(n, RS_TILE);
     uint32_t *counts = a.get_t<uint32_t>("rs_counts", (size_t)256 * nb);
     uint32_t *offs = a.get_t<uint32_t>("rs_offs", (size_t)256 * nb);
