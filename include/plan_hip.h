@@ -138,13 +138,15 @@ int plan_stage_times(plan_ctx *ctx, double *ms, int n);
  *  24-26 merge ms in clump-tree searches, saddle densities, applying merges,
  *  27-29 rebinding ms in superset searches, parallel runs, in-order validation,
  *  30-31 waves of the density and densest-neighbour passes whose node list
- *  overflowed the on-chip capacity (walked in several fills),
+ *  overflowed the on-chip capacity (redone in smaller waves),
  *  32-34 the slowest clump's rebinding: exact device searches, their ms, ms
  *  re-sorting members,
  *  35-37 all rebinding runs: summed ms, summed ms in exact device searches
  *  (waits for the device included), schedule slot of the slowest run,
  *  38-40 the run that ended last: its end (ms after the first run started),
- *  schedule slot and ms */
+ *  schedule slot and ms,
+ *  41-42 queries of the density and densest-neighbour passes that walked in
+ *  several list fills (one query per wave) */
 int plan_clump_stats(plan_ctx *ctx, double *values, int n);
 /* device milliseconds of the last density pass's gather kernel (the K-nearest
  * selection and density, one launch) and of its radius-schedule kernel */

@@ -50,6 +50,7 @@ struct plan_ctx {
     double density_kernel_ms = 0;
     double density_radius_ms = 0;
     uint32_t density_ovf_waves = 0;
+    uint32_t density_res_queries = 0; /* queries walked in several fills */
     hipEvent_t ev[8];
 };
 
@@ -548,7 +549,9 @@ static int run_density(plan_ctx *c, uint32_t K, int kernel)
     ko.rho = c->tree.rho;
     ko.ev = &c->ev[5];
     c->density_ovf_waves = 0;
+    c->density_res_queries = 0;
     ko.novf = &c->density_ovf_waves;
+    ko.nres = &c->density_res_queries;
     HIPCHK(hipEventRecord(c->ev[2], c->stream));
     dev_knn(c->tree, K, 0, kernel, c->tree.n, nullptr, nullptr, nullptr, nullptr, c->d, ko, c->stream);
     HIPCHK(hipEventRecord(c->ev[3], c->stream));
@@ -996,7 +999,7 @@ int plan_stage_times(plan_ctx *c, double *ms, int n)
 int plan_clump_stats(plan_ctx *c, double *v, int n)
 {
     if (!c || !v) return PLAN_E_ARG;
-    double a[41] = {(double)c->hop_na, (double)c->hop_groups, (double)c->cstats.naive, (double)c->cstats.after_merge,
+    double a[43] = {(double)c->hop_na, (double)c->hop_groups, (double)c->cstats.naive, (double)c->cstats.after_merge,
                     (double)c->cstats.merge_passes, (double)c->cstats.pair_candidates,
                     (double)c->cstats.saddle_queries, (double)c->cstats.rebind_searches, c->d2h_ms,
                     (double)c->cstats.rebind_reruns};
@@ -1018,7 +1021,9 @@ int plan_clump_stats(plan_ctx *c, double *v, int n)
     a[38] = c->cstats.rebind_last_end_ms;
     a[39] = c->cstats.rebind_last_slot;
     a[40] = c->cstats.rebind_last_ms;
-    for (int i = 0; i < n && i < 41; ++i) v[i] = a[i];
+    a[41] = c->density_res_queries;
+    a[42] = c->hop.knn_res_queries;
+    for (int i = 0; i < n && i < 43; ++i) v[i] = a[i];
     return PLAN_OK;
 }
 

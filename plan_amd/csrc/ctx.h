@@ -72,7 +72,7 @@ struct KnnOut {
     double *rstop = nullptr;   /* nq final radius (optional) */
     hipEvent_t *ev = nullptr;  /* 3 events: before the radius pass, around the first select launch */
     uint32_t *novf = nullptr;  /* host counter: waves whose include list overflowed */
-    uint32_t *nres = nullptr;  /* host counter: waves that took the resumable path */
+    uint32_t *nres = nullptr;  /* host counter: queries walked in several fills (one per wave) */
 };
 /* mode: 0 density, 1 hop (densest neighbour), 2 lists only.
  * Queries are tree particles qidx[i] (or i when qidx is null) or explicit

@@ -29,7 +29,8 @@ struct HopResult {
     uint32_t *cstart = nullptr;   /* first member of each naive clump in order[] */
     MemberRec *members = nullptr; /* member data in grouped order */
     uint32_t *msink = nullptr;    /* per member: particles merged into it (sinks) */
-    uint32_t knn_overflow_waves = 0; /* densest-neighbour waves that took the resumable path */
+    uint32_t knn_overflow_waves = 0; /* densest-neighbour waves whose include list overflowed */
+    uint32_t knn_res_queries = 0;    /* of their queries, those walked in several fills */
 };
 
 void dev_hop(const DevTree &t, Arena &a, const plan_derived &d, double thr, uint32_t H, HopResult &res,

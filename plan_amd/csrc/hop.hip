@@ -161,7 +161,9 @@ void dev_hop(const DevTree &t, Arena &a, const plan_derived &d, double thr, uint
     KnnOut ko;
     ko.densest = dn;
     res.knn_overflow_waves = 0;
+    res.knn_res_queries = 0;
     ko.novf = &res.knn_overflow_waves;
+    ko.nres = &res.knn_res_queries;
     dev_knn(t, H, 1, 0, na, res.a_idx, nullptr, nullptr, t.rho, d, ko, s);
     res.densest = dn;
     /* pointer jumping */

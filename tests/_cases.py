@@ -27,6 +27,8 @@ def duplicates_case():
 
 
 CASES = {
+    # tree small enough for the one-workgroup sorts of BuildTree and HOP
+    "n1k": lambda: dense_case(900, 2, 9, a=2e-4),
     "n4k": lambda: dense_case(4096, 4, 1, a=2e-4),
     "n8k": lambda: dense_case(8192, 6, 2, a=3e-4),
     "n32k": lambda: dense_case(32768, 10, 3, a=4e-4),

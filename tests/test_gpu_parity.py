@@ -161,7 +161,7 @@ def test_ball_queries(name):
         got = ctx.ball(c, r)
         assert np.array_equal(got, want)
         total += len(want)
-    assert total > 1000
+    assert total > min(1000, ot.n // 4)
 
 
 @pytest.mark.parametrize("name", NAMES)
@@ -210,6 +210,7 @@ def test_overflow_paths(limit):
     finally:
         ctx.set_knn_list_limit(0)
     assert st["density_overflow_waves"] > 0 and st["hop_overflow_waves"] > 0, st
+    assert st["density_resumed_queries"] > 0 and st["hop_resumed_queries"] > 0, st
     orho = ot.densities()
     rel = np.abs(rho - orho) / np.maximum(np.abs(orho), 1e-300)
     assert rel.max() <= 1e-5 and np.mean(rho == orho) > 0.999
