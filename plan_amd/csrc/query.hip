@@ -180,63 +180,14 @@ __device__ __forceinline__ double pow15(double x)
     return __dadd_rn(ph, __dadd_rn(pl, c));
 }
 
-/* Up to 8 consecutive nodes (a sibling group: siblings are contiguous in the
- * level-major array) staged in LDS by one vector load and then read by the
- * whole wave; pre-order walks visit siblings back to back, so one fetch
- * serves a group instead of one dependent scalar load per node. */
+/* The nodes of one sibling group (siblings are contiguous in the
+ * level-major array), staged in LDS by one vector load and then read by the
+ * whole wave. */
 struct NodeCache {
     NodeGeo g[8];
     NodeTop t[8];
     uint32_t info[8];
-    uint32_t skip[8];
 };
-
-__device__ __forceinline__ void cache_fetch(const TreeView &tv, NodeCache &c, uint32_t v, uint32_t &lo, uint32_t &hi)
-{
-    const int lane = threadIdx.x;
-    __syncthreads();
-    if (lane < 8 && v + lane < tv.nn) {
-        c.g[lane] = tv.ngeo[v + lane];
-        c.t[lane] = tv.ntop[v + lane];
-        c.info[lane] = tv.ninfo[v + lane];
-        c.skip[lane] = tv.skip[v + lane];
-    }
-    __syncthreads();
-    lo = v;
-    hi = min(v + 8u, tv.nn);
-}
-
-/* Lock-step pre-order walk for the 64 lanes of a wave at per-lane radius r:
- * count = |S_q(r)| (whole nodes and intersecting leaves, by size) and
- * contained = the part of it in nodes the sphere contains. */
-__device__ __forceinline__ void walk_count(const TreeView &tv, NodeCache &nc, double qx, double qy, double qz,
-                                           double r, bool active, uint32_t &count, uint32_t &contained)
-{
-    uint32_t cov_end = active ? 0u : 0xffffffffu;
-    uint32_t nd = 0, lo = 1, hi = 0;
-    uint32_t steps = 0;
-    while (nd != PLAN_NONE && steps++ <= tv.nn) { /* pre-order visits each node at most once */
-        nd = uniform_u32(nd);
-        if (nd < lo || nd >= hi) cache_fetch(tv, nc, nd, lo, hi);
-        const uint32_t s = nd - lo;
-        const NodeGeo g = nc.g[s];
-        const NodeTop t = nc.t[s];
-        const bool leaf = (nc.info[s] & 0xffu) == 0u;
-        int act = 0; /* 1 include, 2 descend, 3 reject */
-        bool whole = false;
-        if (t.begin >= cov_end) {
-            if (sphere_contain(g, qx, qy, qz, r, tv.to_diag)) {
-                act = 1;
-                whole = true;
-            } else if (sphere_intersect(g, qx, qy, qz, r, tv.to_diag)) act = leaf ? 1 : 2;
-            else act = 3;
-            if (act != 2) cov_end = t.end;
-        }
-        if (whole) contained += t.end - t.begin;
-        if (act == 1) count += t.end - t.begin;
-        nd = __ballot(act == 2) ? t.first_daughter : nc.skip[s];
-    }
-}
 
 /* query position and the reference's start radius: hw of the first node on
  * the leaf-to-root path holding >= K/4 particles (src/tree.hpp:3206-3219),
@@ -269,143 +220,6 @@ __device__ __forceinline__ void query_setup(const TreeView &tv, uint32_t q, cons
     }
 }
 
-/* Pass 1: radius schedule r, 2r, 4r ... until |S(r)| >= K (node sizes only,
- * no heap; K <= n guarantees an end, 2200 doublings exceed any finite start). */
-template <int K>
-__global__ __launch_bounds__(64) void k_knn_radius(TreeView tv, uint32_t nq, const uint32_t *__restrict__ qidx,
-                                                   const double4 *__restrict__ qpts, const double *__restrict__ est,
-                                                   const double *__restrict__ rt, double *__restrict__ r_out,
-                                                   uint32_t *__restrict__ c_out, uint32_t qpw)
-{
-    __shared__ NodeCache nc;
-    const int lane = threadIdx.x;
-    const uint32_t q = blockIdx.x * qpw + lane;
-    const bool valid = (uint32_t)lane < qpw && q < nq;
-    double qx = 0, qy = 0, qz = 0, r = 0;
-    uint32_t self = 0;
-    if (valid) query_setup<K>(tv, q, qidx, qpts, est, qx, qy, qz, r, self);
-    /* a point query resolved by k_point_tight: S(est) holds >= K particles,
-     * so the schedule stops at est */
-    const bool tight = valid && rt && rt[q] > 0;
-    bool done = !valid || tight;
-    int rounds = 0;
-    uint32_t inside = tight ? (uint32_t)K : 0u;
-    while (__ballot(!done) && rounds++ < 2200) {
-        uint32_t cnt = 0, cin = 0;
-        walk_count(tv, nc, qx, qy, qz, r, !done, cnt, cin);
-        if (!done) {
-            if (cnt >= (uint32_t)K) {
-                done = true;
-                inside = cin;
-            } else {
-                r = __dmul_rn(r, 2.0);
-            }
-        }
-    }
-    if (valid) {
-        r_out[q] = r;
-        c_out[q] = inside;
-    }
-}
-
-/* Exact count of particles with d^2 <= r^2 from q, by a lock-step walk with
- * the geometric node tests (whole nodes counted by size, boundary leaves
- * particle by particle). */
-__device__ __forceinline__ uint32_t walk_geo_count(const TreeView &tv, NodeCache &nc, double qx, double qy, double qz,
-                                                   double r, bool active)
-{
-    uint32_t cov_end = active ? 0u : 0xffffffffu;
-    uint32_t nd = 0, lo = 1, hi = 0, steps = 0, count = 0;
-    const double r2 = r * r, m = tv.margin_abs;
-    while (nd != PLAN_NONE && steps++ <= tv.nn) {
-        nd = uniform_u32(nd);
-        if (nd < lo || nd >= hi) cache_fetch(tv, nc, nd, lo, hi);
-        const uint32_t s = nd - lo;
-        const NodeGeo g = nc.g[s];
-        const NodeTop t = nc.t[s];
-        const bool leaf = (nc.info[s] & 0xffu) == 0u;
-        int act = 0; /* 1 whole, 4 boundary leaf, 2 descend, 3 reject */
-        if (t.begin >= cov_end) {
-            if (geo_contain(g, qx, qy, qz, r, m)) act = 1;
-            else if (geo_intersect(g, qx, qy, qz, r, m)) act = leaf ? 4 : 2;
-            else act = 3;
-            if (act != 2) cov_end = t.end;
-        }
-        if (act == 1) count += t.end - t.begin;
-        if (act == 4)
-            for (uint32_t p = t.begin; p < t.end; ++p) {
-                const double4 x = tv.pm[p];
-                if (dist2(qx, qy, qz, x.x, x.y, x.z) <= r2) ++count;
-            }
-        nd = __ballot(act == 2) ? t.first_daughter : nc.skip[s];
-    }
-    return count;
-}
-
-/* Point queries (saddle points, user points) with a start radius est: when
- * some radius r_t <= est - margin already holds >= K particles within
- * distance r_t, every particle within r_t belongs to the reference's
- * candidate set S(est) (src/tree.hpp:3160-3230), so the K smallest
- * (d^2, index) of S(est) are the K nearest overall and can be gathered from
- * the ball of radius r_t instead of all of S(est).  r_t starts at the half
- * width of the deepest node holding q with >= K/4 particles and doubles.
- * rt_out[q] = r_t, or 0 when no such radius exists (the query then takes
- * the candidate-set path). */
-template <int K>
-__global__ __launch_bounds__(64) void k_point_tight(TreeView tv, uint32_t nq, const double4 *__restrict__ qpts,
-                                                    const double *__restrict__ est, double *__restrict__ rt_out,
-                                                    uint32_t qpw)
-{
-    __shared__ NodeCache nc;
-    const int lane = threadIdx.x;
-    const uint32_t q = blockIdx.x * qpw + lane;
-    const bool valid = (uint32_t)lane < qpw && q < nq;
-    double qx = 0, qy = 0, qz = 0, e = 0, r = 0, lim = 0;
-    if (valid) {
-        const double4 p = qpts[q];
-        qx = p.x;
-        qy = p.y;
-        qz = p.z;
-        e = est[q];
-        lim = e - (tv.margin_abs + 1e-12 * e);
-        uint32_t node = 0;
-        for (int lvl = 0; lvl < 32; ++lvl) {
-            const uint32_t nd = tv.ninfo[node] & 0xffu;
-            if (nd == 0) break;
-            const uint32_t fd = tv.ntop[node].first_daughter;
-            uint32_t next = PLAN_NONE;
-            for (uint32_t c = 0; c < nd; ++c) {
-                const NodeGeo g = tv.ngeo[fd + c];
-                if (fabs(qx - g.cx) <= g.hw && fabs(qy - g.cy) <= g.hw && fabs(qz - g.cz) <= g.hw) {
-                    next = fd + c;
-                    break;
-                }
-            }
-            if (next == PLAN_NONE) break;
-            const NodeTop t = tv.ntop[next];
-            if (t.end - t.begin < (uint32_t)(K / 4 > 0 ? K / 4 : 1)) break;
-            node = next;
-        }
-        r = tv.ngeo[node].hw;
-    }
-    bool done = !valid || !(r > 0) || !(r <= lim);
-    bool ok = false;
-    int rounds = 0;
-    while (__ballot(!done) && rounds++ < 64) {
-        const uint32_t cnt = walk_geo_count(tv, nc, qx, qy, qz, r, !done);
-        if (!done) {
-            if (cnt >= (uint32_t)K) {
-                done = true;
-                ok = true;
-            } else {
-                r = r * 2.0;
-                if (!(r <= lim)) done = true;
-            }
-        }
-    }
-    if (valid) rt_out[q] = ok ? r : 0.0;
-}
-
 /* One entry of a wave's include list: a node whose particles some lanes take
  * whole (contained node or intersecting leaf), with the mask of those lanes. */
 struct ListEnt {
@@ -419,59 +233,6 @@ struct ListEnt {
 template <int K> struct ListCap {
     static constexpr int value = K >= 64 ? 384 : 256;
 };
-
-struct WalkState {
-    uint32_t nd, cov_end, lo, hi;
-};
-
-/* Resumable lock-step walk that records the include list.  Returns true when
- * the walk is complete, false when the list is full; the node that did not
- * fit is re-evaluated on resume (lane state is only updated after it is
- * recorded). */
-template <int CAP>
-__device__ __forceinline__ bool walk_fill(const TreeView &tv, NodeCache &nc, WalkState &w, double qx, double qy,
-                                          double qz, double r, bool geo, ListEnt *lst, uint32_t &nent,
-                                          uint32_t *scnt = nullptr, uint32_t *ccnt = nullptr)
-{
-    const int lane = threadIdx.x;
-    uint32_t guard = 0;
-    while (w.nd != PLAN_NONE && guard++ <= tv.nn) {
-        const uint32_t nd = uniform_u32(w.nd);
-        if (nd < w.lo || nd >= w.hi) cache_fetch(tv, nc, nd, w.lo, w.hi);
-        const uint32_t s = nd - w.lo;
-        const NodeGeo g = nc.g[s];
-        const NodeTop t = nc.t[s];
-        const bool leaf = (nc.info[s] & 0xffu) == 0u;
-        int act = 0;
-        bool whole = false;
-        if (t.begin >= w.cov_end) {
-            if (geo) {
-                if (geo_contain(g, qx, qy, qz, r, tv.margin_abs)) act = 1;
-                else if (geo_intersect(g, qx, qy, qz, r, tv.margin_abs)) act = leaf ? 1 : 2;
-                else act = 3;
-            } else {
-                if (sphere_contain(g, qx, qy, qz, r, tv.to_diag)) {
-                    act = 1;
-                    whole = true;
-                } else if (sphere_intersect(g, qx, qy, qz, r, tv.to_diag)) act = leaf ? 1 : 2;
-                else act = 3;
-            }
-        }
-        const uint64_t inc = __ballot(act == 1);
-        if (inc) {
-            if (nent >= min((uint32_t)CAP, tv.list_limit)) return false;
-            if (lane == 0) lst[nent] = ListEnt{t.begin, t.end, inc};
-            ++nent;
-            if (scnt && act == 1) {
-                *scnt += t.end - t.begin;
-                if (whole) *ccnt += t.end - t.begin;
-            }
-        }
-        if (act == 1 || act == 3) w.cov_end = t.end;
-        w.nd = __ballot(act == 2) ? t.first_daughter : nc.skip[s];
-    }
-    return true;
-}
 
 /* LDS of the group-wise walk: a stack of sibling groups still to visit
  * (first node, count, mask of the queries that descended into their
@@ -513,6 +274,24 @@ __device__ __forceinline__ uint32_t mbcnt64(uint64_t m)
 
 enum class WalkEnd { done, list_full, stack_full };
 
+/* What a group-wise walk records: the include list with the reference's
+ * sphere tests (LIST) or with the exact-ball tests of point queries with a
+ * tight radius (LIST_GEO); per query the sizes of the nodes taken and of
+ * those the sphere contains (COUNT); per query the exact number of particles
+ * within r (GEOCOUNT: whole nodes inside the ball by size, boundary leaves
+ * particle by particle). */
+enum WalkMode { WALK_LIST, WALK_LIST_GEO, WALK_COUNT, WALK_GEOCOUNT };
+
+/* The geometric decision for one (node, query) pair of the exact ball:
+ * 0 = the node lies inside, 1 = boundary leaf, 2 = descend, 3 = outside. */
+__device__ __forceinline__ int classify_geo(const NodeGeo &g, bool leaf, double qx, double qy, double qz, double r,
+                                            double m)
+{
+    if (geo_contain(g, qx, qy, qz, r, m)) return 0;
+    if (geo_intersect(g, qx, qy, qz, r, m)) return leaf ? 1 : 2;
+    return 3;
+}
+
 /* The walk of RecursiveKNN_OpenMP (src/tree.hpp:3177-3193) for the queries
  * of the groups on the LDS stack (lane = query, radius r per lane), one
  * sibling group per step: the (child, query) pairs of a group whose parent
@@ -520,15 +299,18 @@ enum class WalkEnd { done, list_full, stack_full };
  * classified, then the children some query takes go to the include list
  * (with the mask of those queries) and the children some query descends
  * into push their own groups.  The set of (node, query) decisions is exactly
- * the recursive walk's.  COUNT walks keep no list: each lane adds the sizes
- * of the nodes it takes (sc) and of those its sphere contains (cc).  When
- * the list is full the group being decided goes back on the stack, so a
- * later call continues the walk (sp is the stack height). */
-template <int CAP, int STACK, bool COUNT>
+ * the recursive walk's.  The counting modes keep no list: COUNT adds per
+ * lane the sizes of the nodes it takes (sc) and of those its sphere contains
+ * (cc); GEOCOUNT adds to qcnt[query] the particles within r.  When the list
+ * is full the group being decided goes back on the stack, so a later call
+ * continues the walk (sp is the stack height). */
+template <int CAP, int STACK, int MODE>
 __device__ WalkEnd walk_dense_run(const TreeView &tv, NodeCache &nc, DenseWalkLdsT<STACK> &D, uint64_t *cont,
                                   ListEnt *lst, uint32_t &nent, double qx, double qy, double qz, double r,
-                                  uint32_t &sp, uint32_t &sc, uint32_t &cc)
+                                  uint32_t &sp, uint32_t &sc, uint32_t &cc, uint32_t *qcnt = nullptr)
 {
+    constexpr bool LIST = MODE == WALK_LIST || MODE == WALK_LIST_GEO;
+    constexpr bool GEO = MODE == WALK_LIST_GEO || MODE == WALK_GEOCOUNT;
     const int lane = threadIdx.x;
     while (sp > 0) {
         __syncthreads();
@@ -546,7 +328,7 @@ __device__ WalkEnd walk_dense_run(const TreeView &tv, NodeCache &nc, DenseWalkLd
         if (lane < 8) {
             D.inc[lane] = 0;
             D.desc[lane] = 0;
-            if (COUNT) cont[lane] = 0;
+            if (MODE == WALK_COUNT) cont[lane] = 0;
         }
         __syncthreads();
         const uint32_t cnt = (uint32_t)__popcll(M);
@@ -565,13 +347,29 @@ __device__ WalkEnd walk_dense_run(const TreeView &tv, NodeCache &nc, DenseWalkLd
             const double xj = __shfl(qx, j, 64), yj = __shfl(qy, j, 64), zj = __shfl(qz, j, 64), rj = __shfl(r, j, 64);
             const NodeGeo g = nc.g[c];
             const bool leaf = (nc.info[c] & 0xffu) == 0u;
-            const int act = classify_sphere(g, leaf, xj, yj, zj, rj, tv.to_diag);
-            if (pv && act <= 1) atomicOr((unsigned long long *)&D.inc[c], 1ull << j);
+            int act;
+            if constexpr (GEO) act = classify_geo(g, leaf, xj, yj, zj, rj, tv.margin_abs);
+            else act = classify_sphere(g, leaf, xj, yj, zj, rj, tv.to_diag);
+            if constexpr (MODE == WALK_GEOCOUNT) {
+                if (pv && act == 0) atomicAdd(&qcnt[j], nc.t[c].end - nc.t[c].begin);
+                if (pv && act == 1) {
+                    const NodeTop t = nc.t[c];
+                    const double r2 = rj * rj;
+                    uint32_t in = 0;
+                    for (uint32_t q = t.begin; q < t.end; ++q) {
+                        const double4 x = tv.pm[q];
+                        if (dist2(xj, yj, zj, x.x, x.y, x.z) <= r2) ++in;
+                    }
+                    atomicAdd(&qcnt[j], in);
+                }
+            } else {
+                if (pv && act <= 1) atomicOr((unsigned long long *)&D.inc[c], 1ull << j);
+                if (MODE == WALK_COUNT && pv && act == 0) atomicOr((unsigned long long *)&cont[c], 1ull << j);
+            }
             if (pv && act == 2) atomicOr((unsigned long long *)&D.desc[c], 1ull << j);
-            if (COUNT && pv && act == 0) atomicOr((unsigned long long *)&cont[c], 1ull << j);
         }
         __syncthreads();
-        if (COUNT) {
+        if constexpr (MODE == WALK_COUNT) {
             for (uint32_t c = 0; c < nd; ++c) {
                 const uint32_t sz = nc.t[c].end - nc.t[c].begin;
                 if ((D.inc[c] >> lane) & 1ull) sc += sz;
@@ -587,7 +385,7 @@ __device__ WalkEnd walk_dense_run(const TreeView &tv, NodeCache &nc, DenseWalkLd
             t = nc.t[lane];
             cnd = nc.info[lane] & 0xffu;
         }
-        if (!COUNT) {
+        if constexpr (LIST) {
             const uint64_t mi = __ballot(I != 0ull);
             const uint32_t ni = (uint32_t)__popcll(mi);
             if (nent + ni > min((uint32_t)CAP, tv.list_limit)) {
@@ -631,8 +429,128 @@ __device__ bool walk_dense(const TreeView &tv, NodeCache &nc, DenseWalkLds &D, L
                            double qy, double qz, double r, uint64_t M0)
 {
     uint32_t sp = walk_dense_start(D, M0), sc = 0, cc = 0;
-    return walk_dense_run<CAP, WALK_STACK, false>(tv, nc, D, nullptr, lst, nent, qx, qy, qz, r, sp, sc, cc) ==
+    return walk_dense_run<CAP, WALK_STACK, WALK_LIST>(tv, nc, D, nullptr, lst, nent, qx, qy, qz, r, sp, sc, cc) ==
            WalkEnd::done;
+}
+
+/* LDS of the radius-schedule and tight-radius kernels */
+struct CountLds {
+    NodeCache nc;
+    DenseWalkLdsT<WALK_STACK_FULL> dw;
+    uint64_t cont[8];
+    uint32_t qcnt[64];
+};
+
+/* Pass 1: radius schedule r, 2r, 4r ... until |S(r)| >= K (node sizes only,
+ * no heap; K <= n guarantees an end, 2200 doublings exceed any finite start). */
+template <int K>
+__global__ __launch_bounds__(64) void k_knn_radius(TreeView tv, uint32_t nq, const uint32_t *__restrict__ qidx,
+                                                   const double4 *__restrict__ qpts, const double *__restrict__ est,
+                                                   const double *__restrict__ rt, double *__restrict__ r_out,
+                                                   uint32_t *__restrict__ c_out, uint32_t qpw)
+{
+    __shared__ CountLds L;
+    const int lane = threadIdx.x;
+    const uint32_t q = blockIdx.x * qpw + lane;
+    const bool valid = (uint32_t)lane < qpw && q < nq;
+    double qx = 0, qy = 0, qz = 0, r = 0;
+    uint32_t self = 0;
+    if (valid) query_setup<K>(tv, q, qidx, qpts, est, qx, qy, qz, r, self);
+    /* a point query resolved by k_point_tight: S(est) holds >= K particles,
+     * so the schedule stops at est */
+    const bool tight = valid && rt && rt[q] > 0;
+    bool done = !valid || tight;
+    int rounds = 0;
+    uint32_t inside = tight ? (uint32_t)K : 0u;
+    while (__ballot(!done) && rounds++ < 2200) {
+        uint32_t cnt = 0, cin = 0, nent = 0;
+        uint32_t sp = walk_dense_start(L.dw, __ballot(!done));
+        walk_dense_run<0, WALK_STACK_FULL, WALK_COUNT>(tv, L.nc, L.dw, L.cont, nullptr, nent, qx, qy, qz, r, sp, cnt,
+                                                      cin);
+        if (!done) {
+            if (cnt >= (uint32_t)K) {
+                done = true;
+                inside = cin;
+            } else {
+                r = __dmul_rn(r, 2.0);
+            }
+        }
+    }
+    if (valid) {
+        r_out[q] = r;
+        c_out[q] = inside;
+    }
+}
+
+/* Point queries (saddle points, user points) with a start radius est: when
+ * some radius r_t <= est - margin already holds >= K particles within
+ * distance r_t, every particle within r_t belongs to the reference's
+ * candidate set S(est) (src/tree.hpp:3160-3230), so the K smallest
+ * (d^2, index) of S(est) are the K nearest overall and can be gathered from
+ * the ball of radius r_t instead of all of S(est).  r_t starts at the half
+ * width of the deepest node holding q with >= K/4 particles and doubles.
+ * rt_out[q] = r_t, or 0 when no such radius exists (the query then takes
+ * the candidate-set path).  The counts are exact (boundary leaves particle
+ * by particle) and come from the group-wise walk with the ball tests. */
+template <int K>
+__global__ __launch_bounds__(64) void k_point_tight(TreeView tv, uint32_t nq, const double4 *__restrict__ qpts,
+                                                    const double *__restrict__ est, double *__restrict__ rt_out,
+                                                    uint32_t qpw)
+{
+    __shared__ CountLds L;
+    const int lane = threadIdx.x;
+    const uint32_t q = blockIdx.x * qpw + lane;
+    const bool valid = (uint32_t)lane < qpw && q < nq;
+    double qx = 0, qy = 0, qz = 0, e = 0, r = 0, lim = 0;
+    if (valid) {
+        const double4 p = qpts[q];
+        qx = p.x;
+        qy = p.y;
+        qz = p.z;
+        e = est[q];
+        lim = e - (tv.margin_abs + 1e-12 * e);
+        uint32_t node = 0;
+        for (int lvl = 0; lvl < 32; ++lvl) {
+            const uint32_t nd = tv.ninfo[node] & 0xffu;
+            if (nd == 0) break;
+            const uint32_t fd = tv.ntop[node].first_daughter;
+            uint32_t next = PLAN_NONE;
+            for (uint32_t c = 0; c < nd; ++c) {
+                const NodeGeo g = tv.ngeo[fd + c];
+                if (fabs(qx - g.cx) <= g.hw && fabs(qy - g.cy) <= g.hw && fabs(qz - g.cz) <= g.hw) {
+                    next = fd + c;
+                    break;
+                }
+            }
+            if (next == PLAN_NONE) break;
+            const NodeTop t = tv.ntop[next];
+            if (t.end - t.begin < (uint32_t)(K / 4 > 0 ? K / 4 : 1)) break;
+            node = next;
+        }
+        r = tv.ngeo[node].hw;
+    }
+    bool done = !valid || !(r > 0) || !(r <= lim);
+    bool ok = false;
+    int rounds = 0;
+    while (__ballot(!done) && rounds++ < 64) {
+        L.qcnt[lane] = 0;
+        uint32_t sc = 0, cc = 0, nent = 0;
+        uint32_t sp = walk_dense_start(L.dw, __ballot(!done));
+        walk_dense_run<0, WALK_STACK_FULL, WALK_GEOCOUNT>(tv, L.nc, L.dw, nullptr, nullptr, nent, qx, qy, qz, r, sp,
+                                                         sc, cc, L.qcnt);
+        __syncthreads();
+        const uint32_t cnt = L.qcnt[lane];
+        if (!done) {
+            if (cnt >= (uint32_t)K) {
+                done = true;
+                ok = true;
+            } else {
+                r = r * 2.0;
+                if (!(r <= lim)) done = true;
+            }
+        }
+    }
+    if (valid) rt_out[q] = ok ? r : 0.0;
 }
 
 /* |S(r)| of each query from the include-list entries [e0, e1) */
@@ -980,7 +898,6 @@ __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, con
     const double inf = __longlong_as_double(0x7ff0000000000000ll);
     const uint32_t nvalid = (uint32_t)__popcll(__ballot(valid));
     uint32_t cin = 0;
-    WalkState w{0u, valid ? 0u : 0xffffffffu, 1u, 0u};
     uint32_t nent = 0;
     bool filled = false;
     if constexpr (FUSE && !RESUME) {
@@ -1006,7 +923,6 @@ __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, con
             r_io[q] = r;
             c_io[q] = 0u;
         }
-        w = WalkState{0u, valid ? 0u : 0xffffffffu, 1u, 0u};
     } else if constexpr (FUSE) {
         /* waves that walk in several fills: the radius schedule r0, 2 r0, ...
          * until |S(r)| >= K (src/tree.hpp:3206-3223) by counting walks, which
@@ -1015,8 +931,8 @@ __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, con
         for (int round = 0; round < 2200; ++round) {
             uint32_t sc = 0, cc = 0;
             uint32_t sp = walk_dense_start(L.dw, V);
-            walk_dense_run<CAP, WALK_STACK_FULL, true>(tv, L.nc, L.dw, L.cont, L.lst, nent, qx, qy, qz, r, sp, sc,
-                                                       cc);
+            walk_dense_run<CAP, WALK_STACK_FULL, WALK_COUNT>(tv, L.nc, L.dw, L.cont, L.lst, nent, qx, qy, qz, r, sp,
+                                                             sc, cc);
             const bool need = valid && sc < (uint32_t)K;
             if (!need) cin = cc;
             if (!__ballot(need)) break;
@@ -1041,16 +957,24 @@ __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, con
         const double rr = __dadd_rn(r, 1e-9);
         T = __dmul_rn(rr, rr);
     }
+    /* point queries with a tight radius take the exact-ball tests; they run
+     * one query per wave, so the choice is the wave's */
     const bool geo = valid && rt_in && rt_in[q] > 0;
     if (geo) {
         r = rt_in[q];
         T = r * r;
     }
+    const bool wgeo = __ballot(geo) != 0ull;
     if constexpr (!RESUME) {
         if (!filled) {
             nent = 0;
-            const bool fin = walk_fill<CAP>(tv, L.nc, w, qx, qy, qz, r, geo, L.lst, nent);
-            if (!fin) {
+            uint32_t sp = walk_dense_start(L.dw, __ballot(valid)), sc = 0, cc = 0;
+            const WalkEnd e =
+                wgeo ? walk_dense_run<CAP, WALK_STACK, WALK_LIST_GEO>(tv, L.nc, L.dw, nullptr, L.lst, nent, qx, qy, qz,
+                                                                       r, sp, sc, cc)
+                     : walk_dense_run<CAP, WALK_STACK, WALK_LIST>(tv, L.nc, L.dw, nullptr, L.lst, nent, qx, qy, qz, r,
+                                                                  sp, sc, cc);
+            if (e != WalkEnd::done) {
                 if (lane == 0) ovf[wv] = 1u;
                 return;
             }
@@ -1074,18 +998,17 @@ __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, con
                 bix[j * K + lane] = PLAN_NONE;
             }
         bool fin = false;
-        uint32_t sp = 0;
-        if constexpr (FUSE) sp = walk_dense_start(L.dw, __ballot(valid));
+        uint32_t sp = walk_dense_start(L.dw, __ballot(valid));
         while (!fin) {
+            /* the group walk resumes where the full list stopped it */
             nent = 0;
-            if constexpr (FUSE) {
-                /* the group walk resumes where the full list stopped it */
-                uint32_t sc = 0, cc = 0;
-                fin = walk_dense_run<CAP, WALK_STACK_FULL, false>(tv, L.nc, L.dw, nullptr, L.lst, nent, qx, qy, qz,
-                                                                   r, sp, sc, cc) != WalkEnd::list_full;
-            } else {
-                fin = walk_fill<CAP>(tv, L.nc, w, qx, qy, qz, r, geo, L.lst, nent);
-            }
+            uint32_t sc = 0, cc = 0;
+            const WalkEnd e =
+                wgeo ? walk_dense_run<CAP, WALK_STACK_FULL, WALK_LIST_GEO>(tv, L.nc, L.dw, nullptr, L.lst, nent, qx, qy,
+                                                                            qz, r, sp, sc, cc)
+                     : walk_dense_run<CAP, WALK_STACK_FULL, WALK_LIST>(tv, L.nc, L.dw, nullptr, L.lst, nent, qx, qy,
+                                                                       qz, r, sp, sc, cc);
+            fin = e != WalkEnd::list_full;
             __syncthreads();
             for (uint32_t j = 0; j < nvalid; ++j) {
                 double bd = lane < K ? bd2[j * K + lane] : inf;
