@@ -54,6 +54,7 @@ inline int exit_code_of(int status)
     case PLAN_E_ARG: return 4;
     case PLAN_E_STATE: return 3;
     case PLAN_E_DEVICE: return 5;
+    case PLAN_E_IO: return 3;
     default: return 2;
     }
 }
@@ -109,11 +110,34 @@ public:
         num_particles = num_total_particles = N;
     }
 
-    /* MakeGhostParticles (src/tree.hpp:1626-1754): uploads the particles and
-     * generates the ghost copies on the device */
+    double time = 0, dt = 0;
+
+    /* ReadLisFile (src/tree.hpp:1540-1583) straight into device memory: one
+     * combined file or one file per process, in process order.  paras: in,
+     * the parameter-file keys; out, the header's box limits, time and species
+     * weights.  The particles stay on the device (`particles` is left empty)
+     * until BuildTree consumes them. */
+    void ReadLisFile(const std::vector<std::string> &paths, plan_params &paras)
+    {
+        std::vector<const char *> p;
+        for (const auto &f : paths) p.push_back(f.c_str());
+        uint32_t N = 0;
+        ctx->check(plan_read_lis(ctx->get(), &paras, p.data(), (uint32_t)p.size(), &dt, &N), "ReadLisFile");
+        particles.clear();
+        num_particles = num_total_particles = N;
+        num_ghost_particles = 0;
+        num_types = paras.num_types;
+        time = paras.time;
+        params_ = paras;
+        uploaded_ = from_lis_ = true;
+    }
+
+    /* MakeGhostParticles (src/tree.hpp:1626-1754): uploads the particles
+     * (unless ReadLisFile put them on the device) and generates the ghost
+     * copies on the device */
     void MakeGhostParticles(const plan_params &paras)
     {
-        upload(paras);
+        if (!from_lis_) upload(paras);
         uint32_t T = 0;
         ctx->check(plan_make_ghost_particles(ctx->get(), &T), "MakeGhostParticles");
         num_total_particles = T;
@@ -140,6 +164,7 @@ public:
                    "set particles");
         params_ = paras;
         uploaded_ = true;
+        from_lis_ = false;
     }
     bool uploaded() const { return uploaded_; }
     const plan_params &params() const { return params_; }
@@ -147,6 +172,7 @@ public:
 private:
     plan_params params_{};
     bool uploaded_ = false;
+    bool from_lis_ = false;
 };
 
 /* InternalParticle (src/tree.hpp:2367-2410), fetched on demand */

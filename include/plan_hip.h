@@ -29,7 +29,9 @@ enum {
     PLAN_E_ARG = 1,      /* invalid argument (reference: exit(4), src/global.cpp:204,215) */
     PLAN_E_STATE = 2,    /* call out of order (no particles / no tree yet) */
     PLAN_E_DEVICE = 3,   /* HIP runtime error or out of device memory */
-    PLAN_E_INTERNAL = 4  /* unexpected condition inside the pipeline */
+    PLAN_E_INTERNAL = 4, /* unexpected condition inside the pipeline */
+    PLAN_E_IO = 5        /* a file cannot be opened or is shorter than its header states
+                            (reference: exit(3), src/tree.hpp:1341, 1354, 1529) */
 };
 
 typedef struct plan_ctx plan_ctx;
@@ -65,6 +67,26 @@ int plan_set_particles(plan_ctx *ctx, const plan_params *params, uint32_t N, con
 int plan_set_particles_f32(plan_ctx *ctx, const plan_params *params, uint32_t N, const float *pos, const float *vel,
                            const float *density, const int32_t *property_index, const uint32_t *id);
 
+/* ParticleSet<3>::ReadLisFile (src/tree.hpp:1540-1583) for one snapshot:
+ * one combined file (num_paths = 1, ReadSingleLisFile, src/tree.hpp:1434-1529)
+ * or one file per process of the run, in process order
+ * (ReadMultipleLisFile, src/tree.hpp:1302-1430).  The records travel to the
+ * device unchanged and are put in (cpu_id, id_in_run) order there, id =
+ * index (src/tree.hpp:1552-1560); records with equal (cpu_id, id_in_run),
+ * which the reference's std::sort leaves in unspecified order, keep their
+ * file order.  params: in, the parameter-file keys; out, box_min/box_max
+ * (the header's domain limits), time, num_types and type_info from the first
+ * file's header.  dt (may be NULL) receives the header's dt, N_out the
+ * particle count.  A missing file or a short header or body is PLAN_E_IO;
+ * more than 16 species or a species index outside them is PLAN_E_ARG. */
+int plan_read_lis(plan_ctx *ctx, plan_params *params, const char *const *paths, uint32_t num_paths, double *dt,
+                  uint32_t *N_out);
+/* The same from records already in host memory: N records of 44 bytes in
+ * .lis layout (float pos[3], float vel[3], float density, int
+ * property_index, long id_in_run, int cpu_id), in file order; params
+ * complete, as for plan_set_particles. */
+int plan_set_particles_lis(plan_ctx *ctx, const plan_params *params, uint32_t N, const void *records);
+
 /* ParticleSet<3>::MakeGhostParticles (src/tree.hpp:1626-1754), on the device.
  * T_out receives N + number of ghosts.  With params->no_ghost set it is a no-op. */
 int plan_make_ghost_particles(plan_ctx *ctx, uint32_t *T_out);
@@ -92,6 +114,12 @@ void plan_catalog_free(plan_catalog *cat);
 
 /* ---- inspection (tests, tools) ------------------------------------------------ */
 
+/* the loaded particle set (ParticleSet<3>::particles, src/tree.hpp:1200-1298),
+ * with ghosts after plan_make_ghost_particles: the first min(cap, count)
+ * particles; *count receives the set's size.  Any pointer may be NULL.
+ * pos/vel: n x 3; density: the .lis grid density. */
+int plan_get_particle_set(plan_ctx *ctx, uint32_t cap, double *pos, double *vel, double *density, int32_t *prop,
+                          uint32_t *id, uint32_t *count);
 /* tree particles in tree order; any pointer may be NULL.  pos/vel: n x 3. */
 int plan_get_tree_particles(plan_ctx *ctx, double *pos, double *vel, double *mass, double *ath_density,
                             double *new_density, uint32_t *original_id, uint32_t *sink_count);

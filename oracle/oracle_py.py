@@ -329,3 +329,51 @@ def rank_clumps(keys, mass, num_peaks):
     out = np.zeros(2)
     lib().orc_rank_clumps(len(keys), keys, np.ascontiguousarray(mass, np.float64), int(num_peaks), keep, out)
     return keep.astype(bool), out[0], out[1]
+
+
+# ---------------------------------------------------------------- .lis ingest
+
+def read_lis(paths):
+    """ParticleSet<3>::ReadLisFile restated with numpy (src/tree.hpp:1540-1583).
+
+    Header of the first file (ReadSingleLisFile src/tree.hpp:1439-1466,
+    ReadMultipleLisFile 1306-1337): float coor_lim[12], int num_types, float
+    type_info[num_types], float time, float dt, long N.  The other files'
+    counts sit at byte (14 + num_types) * 4 + 4 (1344-1350).  Records of 44
+    bytes follow every header (1384-1417, 1488-1512): float pos[3], float
+    vel[3], float density, int property_index, long id_in_run (kept as its
+    low 32 bits), int cpu_id (kept as its low 16 bits); files are read in the
+    order given.  Particles are then sorted by (cpu_id, id_in_run) (1552-1557,
+    a stable sort here: the reference leaves equal keys in unspecified order)
+    and id = index (1558-1560); the box is the header's domain limits
+    (1565-1566).  Returns a dict of float64 pos/vel/density, int32
+    property_index, uint32 id, and the header values."""
+    if isinstance(paths, (str, os.PathLike)):
+        paths = [paths]
+    bodies = []
+    head = None
+    for k, p in enumerate(paths):
+        raw = np.fromfile(p, dtype=np.uint8)
+        if k == 0:
+            cl = raw[:48].view("<f4").astype(np.float64)
+            nt = int(raw[48:52].view("<i4")[0])
+            ti = raw[52:52 + 4 * nt].view("<f4").astype(np.float64)
+            time, dt = raw[52 + 4 * nt:60 + 4 * nt].view("<f4").astype(np.float64)
+            head = dict(coor_lim=cl, num_types=nt, type_info=ti, time=float(time), dt=float(dt))
+        nt = head["num_types"]
+        off = (14 + nt) * 4 + 4
+        n = int(raw[off:off + 8].view("<i8")[0])
+        body = raw[off + 8:off + 8 + 44 * n]
+        if body.size != 44 * n:
+            raise ValueError(f"{p}: fewer records than the header states")
+        bodies.append(body.reshape(n, 44))
+    rec = np.concatenate(bodies)
+    f32 = lambda a, b: np.ascontiguousarray(rec[:, a:b]).view("<f4").astype(np.float64)
+    pos, vel, dens = f32(0, 12), f32(12, 24), f32(24, 28)[:, 0]
+    prop = np.ascontiguousarray(rec[:, 28:32]).view("<i4")[:, 0]
+    idr = np.ascontiguousarray(rec[:, 32:36]).view("<u4")[:, 0]
+    cpu = np.ascontiguousarray(rec[:, 40:42]).view("<u2")[:, 0]
+    order = np.lexsort((idr, cpu))
+    cl = head["coor_lim"]
+    return dict(pos=pos[order], vel=vel[order], density=dens[order], property_index=prop[order].astype(np.int32),
+                id=np.arange(len(order), dtype=np.uint32), box_min=cl[[6, 8, 10]], box_max=cl[[7, 9, 11]], **head)

@@ -50,7 +50,8 @@ EXPORTS = [
     "plan_knn_query", "plan_density_at", "plan_ball_query", "plan_stage_times", "plan_last_density_kernel_ms",
     "plan_keep_input", "plan_restore_input", "plan_device_memory", "plan_clump_stats",
     "plan_last_density_radius_ms", "plan_boundness_load", "plan_boundness_load_catalog", "plan_boundness_run",
-    "plan_boundness_result", "plan_set_knn_list_limit",
+    "plan_boundness_result", "plan_set_knn_list_limit", "plan_read_lis", "plan_set_particles_lis",
+    "plan_get_particle_set",
 ]
 
 
@@ -97,6 +98,10 @@ def lib():
     L.plan_boundness_run.argtypes = [vp, C.c_int, C.c_uint32, C.POINTER(PlanBoundnessStats)]
     L.plan_boundness_result.argtypes = [vp, vp, C.POINTER(PlanBoundnessClump)]
     L.plan_set_knn_list_limit.argtypes = [vp, C.c_uint32]
+    L.plan_read_lis.argtypes = [vp, C.POINTER(PlanParams), C.POINTER(C.c_char_p), C.c_uint32,
+                                C.POINTER(C.c_double), C.POINTER(C.c_uint32)]
+    L.plan_set_particles_lis.argtypes = [vp, C.POINTER(PlanParams), C.c_uint32, vp]
+    L.plan_get_particle_set.argtypes = [vp, C.c_uint32, vp, vp, vp, vp, vp, C.POINTER(C.c_uint32)]
     for f in EXPORTS:
         if f in ("plan_ctx_destroy", "plan_catalog_free", "plan_last_error"):
             continue
@@ -155,6 +160,53 @@ class Context:
                                                np.ascontiguousarray(density, np.float64), prop, ids),
                       "plan_set_particles")
         return n
+
+    def read_lis(self, params: dict, paths) -> tuple:
+        """ReadLisFile: one combined .lis file or one per process (in process
+        order).  params holds the parameter-file keys; returns (params with the
+        header's box, time and species weights, dt, N)."""
+        if isinstance(paths, (str, os.PathLike)):
+            paths = [paths]
+        enc = [os.fsencode(p) for p in paths]
+        arr = (C.c_char_p * len(enc))(*enc)
+        self._p = make_params(params)
+        dt = C.c_double()
+        n = C.c_uint32()
+        self._chk(lib().plan_read_lis(self._h, C.byref(self._p), arr, len(enc), C.byref(dt), C.byref(n)),
+                  "plan_read_lis")
+        out = dict(params)
+        p = self._p
+        out["box_min"] = [p.box_min[k] for k in range(3)]
+        out["box_max"] = [p.box_max[k] for k in range(3)]
+        out["time"] = p.time
+        out["type_info"] = [p.type_info[k] for k in range(p.num_types)]
+        out["n"] = n.value
+        return out, dt.value, n.value
+
+    def set_particles_lis(self, params: dict, records) -> int:
+        """Particles from .lis records already in memory (44 bytes each, file order)."""
+        buf = np.ascontiguousarray(np.frombuffer(records, dtype=np.uint8) if isinstance(records, (bytes, bytearray))
+                                   else records.view(np.uint8).ravel())
+        if buf.size % 44:
+            raise PlanError("records must be a whole number of 44-byte .lis records")
+        n = buf.size // 44
+        self._p = make_params(params)
+        self._chk(lib().plan_set_particles_lis(self._h, C.byref(self._p), n, buf.ctypes.data),
+                  "plan_set_particles_lis")
+        return n
+
+    def particle_set(self, cap: int | None = None) -> dict:
+        """The loaded particle set (with ghosts after make_ghosts)."""
+        cnt = C.c_uint32()
+        self._chk(lib().plan_get_particle_set(self._h, 0, None, None, None, None, None, C.byref(cnt)),
+                  "plan_get_particle_set")
+        n = cnt.value if cap is None else min(cap, cnt.value)
+        out = dict(pos=np.empty((n, 3)), vel=np.empty((n, 3)), density=np.empty(n), property_index=np.empty(n, np.int32),
+                   id=np.empty(n, np.uint32))
+        self._chk(lib().plan_get_particle_set(self._h, n, *[out[k].ctypes.data for k in
+                                                            ("pos", "vel", "density", "property_index", "id")],
+                                              C.byref(cnt)), "plan_get_particle_set")
+        return out
 
     def make_ghosts(self) -> int:
         t = C.c_uint32()
@@ -355,7 +407,7 @@ class ParticleSet:
         self.property_index = property_index
         self.ids = ids
         self.params = params
-        self.num_particles = int(np.asarray(pos).shape[0])
+        self.num_particles = 0 if pos is None else int(np.asarray(pos).shape[0])
         self.num_total_particles = self.num_particles
         self._ctx = None
 
@@ -363,8 +415,23 @@ class ParticleSet:
     def from_snapshot(cls, snap):
         return cls(snap.pos, snap.vel, snap.density, snap.property_index, snap.id, snap.params)
 
+    @classmethod
+    def ReadLisFile(cls, paths, params: dict, ctx: Context):
+        """ReadLisFile (src/tree.hpp:1540-1583) straight into ctx's device
+        memory: one combined file or one per process, in process order.  The
+        header's box, time and species weights update params; the particle
+        arrays stay on the device."""
+        params, dt, n = ctx.read_lis(params, paths)
+        ps = cls(None, None, None, params=params)
+        ps.num_particles = ps.num_total_particles = n
+        ps.time, ps.dt = params["time"], dt
+        ps._ctx = ctx
+        return ps
+
     def _bind(self, ctx: Context):
         if self._ctx is not ctx:
+            if self.pos is None:
+                raise PlanError("this particle set was read into another context; read it again")
             ctx.set_particles(self.params, self.pos, self.vel, self.density, self.property_index, self.ids)
             self._ctx = ctx
 

@@ -493,6 +493,107 @@ int plan_set_particles_f32(plan_ctx *c, const plan_params *params, uint32_t N, c
     });
 }
 
+/* the header's fields of plan_params (ReadLisFile, src/tree.hpp:1564-1583):
+ * domain limits, time and species weights */
+static void params_from_lis(const LisHeader &h, plan_params &p)
+{
+    for (int k = 0; k < 3; ++k) {
+        p.box_min[k] = h.coor_lim[6 + 2 * k];
+        p.box_max[k] = h.coor_lim[7 + 2 * k];
+    }
+    p.time = h.time;
+    p.num_types = h.num_types;
+    for (int k = 0; k < PLAN_MAX_TYPES; ++k) p.type_info[k] = k < (int)h.num_types ? h.type_info[k] : 0.0;
+}
+
+static int lis_finish(plan_ctx *c, const uint32_t *rec, uint32_t N, uint32_t num_types)
+{
+    if (!dev_lis_to_particles(c->work, rec, N, num_types, c->ps, c->stream)) {
+        c->have_particles = false;
+        return fail(c, PLAN_E_ARG, "property_index outside the species table");
+    }
+    return PLAN_OK;
+}
+
+int plan_read_lis(plan_ctx *c, plan_params *params, const char *const *paths, uint32_t num_paths, double *dt,
+                  uint32_t *N_out)
+{
+    return guarded(c, [&]() -> int {
+        if (!params) return fail(c, PLAN_E_ARG, "params is NULL");
+        LisHeader h;
+        std::vector<uint64_t> counts;
+        bool io = false;
+        std::string e = lis_read_headers(paths, num_paths, h, counts, io);
+        if (!e.empty()) return fail(c, io ? PLAN_E_IO : PLAN_E_ARG, e);
+        plan_params p = *params;
+        params_from_lis(h, p);
+        const uint32_t N = h.num_particles;
+        int r = set_common(c, &p, N);
+        if (r) return r;
+        c->have_particles = false;
+        ensure_particles(c->ps, N);
+        DevBuf rec((size_t)N * 44);
+        e = lis_upload_records(paths, num_paths, h, counts, static_cast<uint8_t *>(rec.p), c->stream);
+        if (!e.empty()) return fail(c, PLAN_E_IO, e);
+        r = lis_finish(c, static_cast<const uint32_t *>(rec.p), N, p.num_types);
+        if (r) return r;
+        c->have_particles = true;
+        *params = p;
+        if (dt) *dt = h.dt;
+        if (N_out) *N_out = N;
+        return PLAN_OK;
+    });
+}
+
+int plan_set_particles_lis(plan_ctx *c, const plan_params *params, uint32_t N, const void *records)
+{
+    return guarded(c, [&]() -> int {
+        if (!records) return fail(c, PLAN_E_ARG, "NULL records");
+        int r = set_common(c, params, N);
+        if (r) return r;
+        c->have_particles = false;
+        ensure_particles(c->ps, N);
+        DevBuf rec((size_t)N * 44);
+        HIPCHK(hipMemcpyAsync(rec.p, records, (size_t)N * 44, hipMemcpyHostToDevice, c->stream));
+        r = lis_finish(c, static_cast<const uint32_t *>(rec.p), N, params->num_types);
+        if (r) return r;
+        c->have_particles = true;
+        return PLAN_OK;
+    });
+}
+
+int plan_get_particle_set(plan_ctx *c, uint32_t cap, double *pos, double *vel, double *density, int32_t *prop,
+                          uint32_t *id, uint32_t *count)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_particles) return fail(c, PLAN_E_STATE, "no particle set loaded");
+        const uint32_t T = c->ps.T, n = cap < T ? cap : T;
+        if (count) *count = T;
+        if (n == 0) return PLAN_OK;
+        hipStream_t s = c->stream;
+        std::vector<double4> P(n), V(n);
+        HIPCHK(hipMemcpyAsync(P.data(), c->ps.pos, (size_t)n * 32, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(V.data(), c->ps.vel, (size_t)n * 32, hipMemcpyDeviceToHost, s));
+        if (prop) HIPCHK(hipMemcpyAsync(prop, c->ps.prop, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+        if (id) HIPCHK(hipMemcpyAsync(id, c->ps.id, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        for (uint32_t i = 0; i < n; ++i) {
+            if (pos) {
+                pos[3 * i] = P[i].x;
+                pos[3 * i + 1] = P[i].y;
+                pos[3 * i + 2] = P[i].z;
+            }
+            if (vel) {
+                vel[3 * i] = V[i].x;
+                vel[3 * i + 1] = V[i].y;
+                vel[3 * i + 2] = V[i].z;
+            }
+            if (density) density[i] = P[i].w;
+        }
+        return PLAN_OK;
+    });
+}
+
 int plan_make_ghost_particles(plan_ctx *c, uint32_t *T_out)
 {
     return guarded(c, [&]() -> int {

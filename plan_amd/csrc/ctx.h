@@ -58,6 +58,28 @@ struct DevParticles {
 /* derived parameters: the reference's NumericalParameters after ReadLisFile */
 void derive_params(const plan_params *p, uint32_t N, plan_derived *d);
 
+/* lis.hip: ParticleSet<3>::ReadLisFile with the particle work on the device */
+struct LisHeader {
+    double coor_lim[12] = {};   /* grid lo/hi x3, then domain lo/hi x3 */
+    uint32_t num_types = 0;
+    double type_info[PLAN_MAX_TYPES] = {};
+    double time = 0, dt = 0;
+    uint64_t body_offset = 0;   /* first record's byte offset in every file */
+    uint32_t num_particles = 0; /* over all files */
+};
+/* headers of the files (error message, or "" on success; io: the message is
+ * about a file that cannot be opened or read, not about its contents) */
+std::string lis_read_headers(const char *const *paths, uint32_t num_paths, LisHeader &h,
+                             std::vector<uint64_t> &counts, bool &io);
+/* all files' records, in file order, to consecutive device bytes at dst
+ * (error message, always about reading a file, or "" on success) */
+std::string lis_upload_records(const char *const *paths, uint32_t num_paths, const LisHeader &h,
+                               const std::vector<uint64_t> &counts, uint8_t *dst, hipStream_t s);
+/* n device records -> ps (capacity >= n) in (cpu_id, id_in_run) order with
+ * id = index; false if a species lies outside [0, num_types) */
+bool dev_lis_to_particles(Arena &a, const uint32_t *rec, uint32_t n, uint32_t num_types, DevParticles &ps,
+                          hipStream_t s);
+
 /* build.hip */
 void dev_make_ghosts(DevParticles &ps, const plan_params &pp, const plan_derived &d, hipStream_t s);
 void dev_build_tree(DevTree &t, const DevParticles &ps, const plan_derived &d, bool check_pos,

@@ -50,6 +50,19 @@ private:
     std::unordered_map<std::string, Buf> bufs_;
 };
 
+/* device bytes owned by one call (freed when it returns) */
+struct DevBuf {
+    void *p = nullptr;
+    explicit DevBuf(size_t bytes) { HIPCHK(hipMalloc(&p, bytes ? bytes : 16)); }
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
 /* pinned host staging buffer */
 class HostPinned {
 public:

@@ -178,12 +178,16 @@ void dev_hop(const DevTree &t, Arena &a, const plan_derived &d, double thr, uint
                        fmin);
     LAUNCHCHK();
     HIPCHK(hipMemcpyAsync(up, par, (size_t)na * 4, hipMemcpyDeviceToDevice, s));
-    for (int it = 0; it < 40; ++it) {
+    /* two jumps per check of the change flag: a jump after convergence
+     * changes nothing, so the extra one is harmless and halves the syncs */
+    for (int it = 0; it < 20; ++it) {
         HIPCHK(hipMemsetAsync(chg, 0, 4, s));
-        hipLaunchKernelGGL(k_jump, dim3(ceil_div(na, BLK)), dim3(BLK), 0, s, par, dep, na, par2, dep2, chg);
-        LAUNCHCHK();
-        std::swap(par, par2);
-        std::swap(dep, dep2);
+        for (int h = 0; h < 2; ++h) {
+            hipLaunchKernelGGL(k_jump, dim3(ceil_div(na, BLK)), dim3(BLK), 0, s, par, dep, na, par2, dep2, chg);
+            LAUNCHCHK();
+            std::swap(par, par2);
+            std::swap(dep, dep2);
+        }
         if (!read_u32(chg, s)) break;
     }
     hipLaunchKernelGGL(k_subtree_min, dim3(ceil_div(na, BLK)), dim3(BLK), 0, s, up, na, fmin);

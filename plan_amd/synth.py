@@ -206,6 +206,75 @@ def to_plan_params_dict(snap: Snapshot) -> dict:
     return dict(snap.params)
 
 
+# ---------------------------------------------------------------- .lis files
+
+# One particle record of an Athena .lis file (src/tree.hpp:1384-1417): 44
+# bytes, little-endian, no padding.
+LIS_RECORD = np.dtype([("pos", "<f4", (3,)), ("vel", "<f4", (3,)), ("density", "<f4"),
+                       ("property_index", "<i4"), ("id_in_run", "<i8"), ("cpu_id", "<i4")])
+
+
+def lis_header(snap: Snapshot, n: int, dt: float = 1e-3) -> bytes:
+    """float coor_lim[12] (grid lo/hi x3, then domain lo/hi x3), int
+    num_types, float type_info[num_types], float time, float dt, long N."""
+    p = snap.params
+    lim = []
+    for k in range(3):
+        lim += [p["box_min"][k], p["box_max"][k]]
+    ti = p.get("type_info", [1.0])
+    return (np.array(lim + lim, "<f4").tobytes() + np.array([len(ti)], "<i4").tobytes()
+            + np.array(ti, "<f4").tobytes() + np.array([p.get("time", 0.0), dt], "<f4").tobytes()
+            + np.array([n], "<i8").tobytes())
+
+
+def lis_records(snap: Snapshot, num_cpus: int = 4, seed: int = 0, high_bits: bool = False):
+    """Records whose (cpu_id, id_in_run) order is the snapshot's order:
+    particle i belongs to process i * num_cpus // n, with ids in the run
+    increasing by 1-3 inside a process.  With high_bits the bits the reader
+    drops (above 16 of cpu_id, above 32 of id_in_run) are filled at random.
+    Returns (records in random order, process of each record)."""
+    rng = np.random.default_rng(seed)
+    n = snap.n
+    cpu = (np.arange(n, dtype=np.int64) * num_cpus) // n
+    step = rng.integers(1, 4, n)
+    idr = np.empty(n, np.int64)
+    for c in range(num_cpus):
+        m = cpu == c
+        idr[m] = np.cumsum(step[m])
+    rec = np.zeros(n, LIS_RECORD)
+    rec["pos"] = snap.pos
+    rec["vel"] = snap.vel
+    rec["density"] = snap.density
+    rec["property_index"] = snap.property_index
+    rec["id_in_run"] = idr + (rng.integers(0, 1 << 30, n) << 32 if high_bits else 0)
+    rec["cpu_id"] = cpu + (rng.integers(0, 1 << 14, n) << 16 if high_bits else 0)
+    order = rng.permutation(n)
+    return rec[order], cpu[order]
+
+
+def write_lis(snap: Snapshot, base: str, num_files: int = 1, seed: int = 0, dt: float = 1e-3,
+              high_bits: bool = False) -> list:
+    """The snapshot as one combined .lis file (num_files = 1, four processes'
+    records mixed) or as one file per process; returns the paths in process
+    order."""
+    rec, cpu = lis_records(snap, num_cpus=4 if num_files == 1 else num_files, seed=seed, high_bits=high_bits)
+    if num_files == 1:
+        path = f"{base}.lis"
+        with open(path, "wb") as f:
+            f.write(lis_header(snap, len(rec), dt))
+            f.write(rec.tobytes())
+        return [path]
+    paths = []
+    for c in range(num_files):
+        part = rec[cpu == c]
+        path = f"{base}.{c:04d}.lis"
+        with open(path, "wb") as f:
+            f.write(lis_header(snap, len(part), dt))
+            f.write(part.tobytes())
+        paths.append(path)
+    return paths
+
+
 # ---------------------------------------------------------------- boundness batches
 
 @dataclass

@@ -10,12 +10,60 @@
 #include "plan_facade.hpp"
 
 #include <cstdio>
+#include <cstring>
+#include <string>
 #include <vector>
+
+static int write_catalog(const plan_amd::DataSet<float, 3> &ds, const char *path)
+{
+    std::FILE *o = std::fopen(path, "w");
+    if (!o) return 2;
+    const auto &pl = ds.planetesimal_list;
+    std::fprintf(o, "clumps %u Mp_tot %.17g\n", pl.num_planetesimals, pl.Mp_tot);
+    for (const auto &it : pl.planetesimals) {
+        const auto &p = it.second;
+        std::fprintf(o, "%u %zu %.17g %.17g\n", p.peak_index, p.indices.size(), p.total_mass, p.Hill_radius);
+        for (size_t k = 0; k < p.original_ids.size(); ++k) std::fprintf(o, k ? " %u" : "%u", p.original_ids[k]);
+        std::fprintf(o, "\n");
+    }
+    std::fclose(o);
+    std::printf("facade: %u particles, %u tree particles, %u clumps\n", ds.particle_set.num_particles,
+                ds.tree.num_particles, pl.num_planetesimals);
+    return 0;
+}
+
+/* --lis params.bin catalog.txt file.lis [file.lis ...]: the reference's
+ * ReadLisFile, MakeGhostParticles, BuildTree, FindPlanetesimals
+ * (src/main.cpp:71-78) with the parameter-file keys given as a raw
+ * plan_params */
+static int run_lis(int argc, char **argv)
+{
+    plan_params paras{};
+    std::FILE *f = std::fopen(argv[2], "rb");
+    if (!f) return 2;
+    const bool ok = std::fread(&paras, sizeof(paras), 1, f) == 1;
+    std::fclose(f);
+    if (!ok) return 2;
+    std::vector<std::string> files(argv + 4, argv + argc);
+    try {
+        plan_amd::DataSet<float, 3> ds;
+        ds.particle_set.ReadLisFile(files, paras);
+        ds.particle_set.MakeGhostParticles(paras);
+        ds.tree.BuildTree(paras, ds.particle_set);
+        ds.tree.FindPlanetesimals(ds, -1);
+        return write_catalog(ds, argv[3]);
+    } catch (const plan_amd::Error &e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return plan_amd::exit_code_of(e.code);
+    }
+}
 
 int main(int argc, char **argv)
 {
+    if (argc >= 5 && std::strcmp(argv[1], "--lis") == 0) return run_lis(argc, argv);
     if (argc < 3) {
-        std::fprintf(stderr, "usage: %s snapshot.bin catalog.txt\n", argv[0]);
+        std::fprintf(stderr, "usage: %s snapshot.bin catalog.txt | --lis params.bin catalog.txt file.lis...\n",
+                     argv[0]);
         return 2;
     }
     std::FILE *f = std::fopen(argv[1], "rb");
@@ -54,20 +102,7 @@ int main(int argc, char **argv)
         ds.particle_set.MakeGhostParticles(paras);
         ds.tree.BuildTree(paras, ds.particle_set);
         ds.tree.FindPlanetesimals(ds, -1);
-        std::FILE *o = std::fopen(argv[2], "w");
-        if (!o) return 2;
-        const auto &pl = ds.planetesimal_list;
-        std::fprintf(o, "clumps %u Mp_tot %.17g\n", pl.num_planetesimals, pl.Mp_tot);
-        for (const auto &it : pl.planetesimals) {
-            const auto &p = it.second;
-            std::fprintf(o, "%u %zu %.17g %.17g\n", p.peak_index, p.indices.size(), p.total_mass, p.Hill_radius);
-            for (size_t k = 0; k < p.original_ids.size(); ++k)
-                std::fprintf(o, k ? " %u" : "%u", p.original_ids[k]);
-            std::fprintf(o, "\n");
-        }
-        std::fclose(o);
-        std::printf("facade: %u particles, %u tree particles, %u clumps\n", N, ds.tree.num_particles,
-                    pl.num_planetesimals);
+        return write_catalog(ds, argv[2]);
     } catch (const plan_amd::Error &e) {
         std::fprintf(stderr, "error: %s\n", e.what());
         return plan_amd::exit_code_of(e.code);

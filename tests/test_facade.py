@@ -41,3 +41,34 @@ def test_facade_catalog_matches_abi(tmp_path):
         assert a["total_mass"] == b["total_mass"]
         assert np.array_equal(a["ids"], ref["member_original_ids"][b["member_offset"]:b["member_offset"] + b["num_members"]])
     assert got["Mp_tot"] == ref["Mp_tot"]
+
+
+@pytest.mark.gpu
+def test_facade_read_lis(tmp_path):
+    """ReadLisFile through the facade (two per-process files) gives the
+    catalog of the same snapshot loaded as arrays."""
+    import plan_amd
+    from _cases import CASES
+    from plan_amd import synth
+    from plan_amd.abi import make_params
+
+    snap = CASES["n32k"]()
+    exe = build_facade_run(str(tmp_path))
+    paths = synth.write_lis(snap, str(tmp_path / "snap"), num_files=2, seed=4)
+    keys = dict(snap.params, box_min=[0.0] * 3, box_max=[1.0] * 3)  # the header's limits replace these
+    par_path = str(tmp_path / "params.bin")
+    with open(par_path, "wb") as f:
+        f.write(bytes(make_params(keys)))
+    cat_path = str(tmp_path / "cat.txt")
+    print(f"[facade lis] n={snap.n}", flush=True)
+    r = subprocess.run([exe, "--lis", par_path, cat_path, *paths], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = read_catalog(cat_path)
+    ref = plan_amd.find_clumps(snap, device=0)
+    assert len(got["clumps"]) == len(ref["clumps"]) > 0
+    for a, b in zip(got["clumps"], ref["clumps"]):
+        assert a["peak_index"] == b["peak_index"] and a["total_mass"] == b["total_mass"]
+        assert np.array_equal(a["ids"], ref["member_original_ids"][b["member_offset"]:b["member_offset"] + b["num_members"]])
+    r = subprocess.run([exe, "--lis", par_path, cat_path, str(tmp_path / "missing.lis")], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 3 and "Failed to open" in r.stderr, (r.returncode, r.stderr)  # src/tree.hpp:1529
