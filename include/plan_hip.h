@@ -112,6 +112,28 @@ int plan_hop(plan_ctx *ctx, uint32_t H, double threshold, uint32_t *num_above, u
 int plan_find_planetesimals(plan_ctx *ctx, plan_catalog **out);
 void plan_catalog_free(plan_catalog *cat);
 
+/* ---- catalog files ---------------------------------------------------------- */
+
+/* The files PlanetesimalList<3> writes after FindPlanetesimals, byte for byte,
+ * for a catalog of this context's current tree (before the next
+ * plan_build_tree or particle load).  File errors are PLAN_E_IO.
+ * plan_write_peaks: the peaks table of OutputPlanetesimalsInfo
+ * (src/tree.hpp:4958-5150): clumps by ascending mass, Npar with sinks
+ * expanded, mass, Hill radius, centre, velocity, J, and the geometric-mean
+ * and median member offsets after rotating J onto z.  The reference names it
+ * peaks_at_<time as %07.3f>.txt in the output directory (src/tree.hpp:4960-4968). */
+int plan_write_peaks(plan_ctx *ctx, const plan_catalog *cat, const char *path);
+/* OutputSinglePlanetesimal (src/tree.hpp:5191-5222) for every clump:
+ * dir/<peak_index>.txt, every sampling_rate-th member in distance order
+ * (original id, position, velocity, mass).  The reference's dir is
+ * ParList.<NNNN> (src/tree.hpp:4976-4985); the caller creates it. */
+int plan_write_clump_particles(plan_ctx *ctx, const plan_catalog *cat, const char *dir, uint32_t sampling_rate);
+/* WriteBasicResults (src/tree.hpp:4935-4953): one line (time, number of
+ * clumps, largest and total clump mass in code units, as fractions of the
+ * total, in Ceres masses) appended to path, after the column header when
+ * header != 0 (the reference writes it for the first snapshot). */
+int plan_write_basic_results(plan_ctx *ctx, const plan_catalog *cat, const char *path, int header);
+
 /* ---- inspection (tests, tools) ------------------------------------------------ */
 
 /* the loaded particle set (ParticleSet<3>::particles, src/tree.hpp:1200-1298),

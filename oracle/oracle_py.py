@@ -7,6 +7,7 @@ product under plan_amd/ never imports it.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -57,6 +58,8 @@ def lib():
         L.orc_ball.restype = C.c_uint32
         L.orc_ball.argtypes = [vp, _d, C.c_double, _u32, C.c_uint32]
         L.orc_find_planetesimals.argtypes = [vp]
+        L.orc_tree_members.restype = C.c_uint32
+        L.orc_tree_members.argtypes = [vp, C.c_uint32, _d, C.c_uint32]
         L.orc_densities.argtypes = [vp, _d]
         L.orc_hop.argtypes = [vp, _u32, _u32]
         L.orc_stage_info.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
@@ -206,6 +209,13 @@ class OracleTree:
         k = C.c_int(); h = C.c_uint32(); a = C.c_uint32(); b = C.c_uint32()
         lib().orc_stage_info(self.h, C.byref(k), C.byref(h), C.byref(a), C.byref(b))
         return dict(kernel=k.value, hop=h.value, naive_clumps=a.value, after_merge=b.value)
+
+    def members(self, i: int):
+        """Positions of the particles merged into tree particle i, in sorted order."""
+        n = lib().orc_tree_members(self.h, int(i), np.empty(3), 0)
+        out = np.empty((n, 3))
+        lib().orc_tree_members(self.h, int(i), out.ravel(), n)
+        return out
 
     def catalog(self):
         L = lib()
@@ -377,3 +387,118 @@ def read_lis(paths):
     cl = head["coor_lim"]
     return dict(pos=pos[order], vel=vel[order], density=dens[order], property_index=prop[order].astype(np.int32),
                 id=np.arange(len(order), dtype=np.uint32), box_min=cl[[6, 8, 10]], box_max=cl[[7, 9, 11]], **head)
+
+
+# ---------------------------------------------------------------- catalog files
+
+def _c_div(a, b):
+    """IEEE division (0/0 = nan, x/0 = inf), as C does it."""
+    with np.errstate(all="ignore"):
+        return float(np.float64(a) / np.float64(b))
+
+
+def _c_acos(x):
+    return math.nan if math.isnan(x) or abs(x) > 1 else math.acos(x)
+
+
+def peak_offsets(com, J, pos):
+    """geo_mean_offset and median_offset (src/tree.hpp:5027-5104): offsets of
+    the member positions pos (sinks expanded) from com, rotated by
+    R = Rz(phi) Ry(theta) with theta = acos(J_z / |J|), phi = atan2(J_y, J_x)
+    (offset'_d = sum_a offset_a R[a][d]), then per axis the geometric mean of
+    |offset'| as 10^(sum log10 / n) with values below 1e-32 counted as -32,
+    and the median (mean of the two middle values for even n)."""
+    n2 = 0.0
+    for k in range(3):
+        n2 += J[k] * J[k]
+    theta = _c_acos(_c_div(J[2], math.sqrt(n2)))
+    phi = math.atan2(J[1], J[0])
+    rz = [[math.cos(phi), -math.sin(phi), 0.0], [math.sin(phi), math.cos(phi), 0.0], [0.0, 0.0, 1.0]]
+    ry = [[math.cos(theta), 0.0, math.sin(theta)], [0.0, 1.0, 0.0], [-math.sin(theta), 0.0, math.cos(theta)]]
+    rot = [[0.0] * 3 for _ in range(3)]
+    for a in range(3):
+        for b in range(3):
+            acc = 0.0
+            for d in range(3):
+                acc += rz[a][d] * ry[d][b]
+            rot[a][b] = acc
+    npar = len(pos)
+    off = [[0.0] * npar for _ in range(3)]
+    for i in range(npar):
+        t = [float(pos[i][k]) - com[k] for k in range(3)]
+        for d in range(3):
+            acc = 0.0
+            for a in range(3):
+                acc += t[a] * rot[a][d]
+            off[d][i] = acc
+    geo, med = [0.0] * 3, [0.0] * 3
+    half = npar // 2
+    for d in range(3):
+        acc = 0.0
+        for i in range(npar):
+            v = abs(off[d][i])
+            acc = acc - 32 if v < 1e-32 else acc + math.log10(v)
+        geo[d] = math.pow(10.0, acc / npar)
+        srt = sorted(abs(v) for v in off[d])
+        med[d] = srt[half]
+        if npar % 2 == 0:
+            med[d] = (med[d] + srt[half - 1]) / 2.0
+    return geo, med
+
+
+PEAKS_HEADER = ("%12s%12s" % ("#peak_id", "Npar") + "".join("%24s" % h for h in (
+    "total_mass", "Hill_radius", "center_of_mass[x]", "center_of_mass[y]", "center_of_mass[z]", "vel_COM[x]",
+    "vel_COM[y]", "vel_COM[z]", "J[x]", "J[y]", "J[z]/(M R_H^2 Omega)", "geo_mean_offset[x]", "geo_mean_offset[y]",
+    "geo_mean_offset[z]", "median_offset[x]", "median_offset[y]", "median_offset[z]")) + "\n")
+
+
+def peaks_text(cat: dict, members_of) -> str:
+    """OutputPlanetesimalsInfo's peaks table (src/tree.hpp:5004-5150) for a
+    catalog dict: clumps in peaks_and_masses order (ascending mass, then peak
+    index, src/tree.hpp:4291-4299); setw(12) integers, then setprecision(16)
+    setw(24) values in the default float format (= printf %24.16g).
+    members_of(tree_index) gives the positions a tree particle stands for."""
+    order = sorted(range(len(cat["clumps"])), key=lambda i: (cat["clumps"][i]["total_mass"],
+                                                              cat["clumps"][i]["peak_index"]))
+    out = [PEAKS_HEADER]
+    for i in order:
+        c = cat["clumps"][i]
+        mem = cat["members"][c["member_offset"]:c["member_offset"] + c["num_members"]]
+        pos = [p for t in mem for p in members_of(int(t))]
+        geo, med = peak_offsets(c["center_of_mass"], c["J"], pos)
+        vals = [c["total_mass"], c["Hill_radius"], *c["center_of_mass"], *c["vel_com"], *c["J"], *geo, *med]
+        out.append("%12u%12u" % (c["peak_index"], len(pos)) + "".join("%24.16g" % v for v in vals) + "\n")
+    return "".join(out)
+
+
+def clump_particles_text(cat: dict, i: int, tree_particles: dict, rate: int = 1) -> str:
+    """OutputSinglePlanetesimal (src/tree.hpp:5191-5222) for clump i: every
+    rate-th member in distance order, setw(16) original id, then scientific
+    setprecision(16) setw(24) position, velocity, mass (= %24.16e)."""
+    c = cat["clumps"][i]
+    lines = []
+    rate = max(rate, 1)
+    for j in range(0, c["num_members"], rate):
+        t = int(cat["members"][c["member_offset"] + j])
+        oid = int(cat["member_original_ids"][c["member_offset"] + j])
+        vals = [*tree_particles["pos"][t], *tree_particles["vel"][t], tree_particles["mass"][t]]
+        lines.append("%16u" % oid + "".join("%24.16e" % float(v) for v in vals) + "\n")
+    return "".join(lines)
+
+
+def basic_results_text(cat: dict, time: float, mass_total: float, G_tilde: float, header: bool) -> str:
+    """WriteBasicResults (src/tree.hpp:4935-4953): width 15, scientific with
+    the default precision 6 (= %15.6e); largest and total clump mass summed
+    in peak-index order; mass_physical = (G_tilde / 0.1) 1.3e24 and the Ceres
+    mass 9.4e20 (src/global.cpp:40-42, src/global.hpp:803-809)."""
+    out = ""
+    if header:
+        out = "".join("%15s" % h for h in ("#time", "N_peak", "M_p,max,code", "M_p,tot,code", "M_p,max,frac",
+                                            "M_p,tot,frac", "M_p,max,Ceres", "M_p,tot,Ceres")) + "\n"
+    mmax = mtot = 0.0
+    for c in cat["clumps"]:
+        mmax = max(mmax, c["total_mass"])
+        mtot += c["total_mass"]
+    mp = (G_tilde / 0.1) * 1.3e24
+    vals = [mmax, mtot, mmax / mass_total, mtot / mass_total, mmax * mp / 9.4e20, mtot * mp / 9.4e20]
+    return out + "%15.6e%15u" % (time, len(cat["clumps"])) + "".join("%15.6e" % v for v in vals) + "\n"

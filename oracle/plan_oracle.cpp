@@ -1315,6 +1315,24 @@ void orc_tree_particles(const orc_tree *h, double *pos, double *vel, double *mas
     }
 }
 
+/* positions of the particles merged into tree particle i (sink_particle_indices,
+ * src/tree.hpp:2608-2651), in sorted order; a plain particle gives its own.
+ * Writes at most cap positions; returns the count. */
+uint32_t orc_tree_members(const orc_tree *h, uint32_t i, double *pos, uint32_t cap)
+{
+    const Tree &T = h->T;
+    auto it = T.sinks.find(i);
+    if (it == T.sinks.end()) {
+        if (cap >= 1)
+            for (int k = 0; k < 3; ++k) pos[k] = T.p[i].pos[k];
+        return 1;
+    }
+    const uint32_t n = (uint32_t)it->second.size();
+    for (uint32_t j = 0; j < n && j < cap; ++j)
+        for (int k = 0; k < 3; ++k) pos[3 * j + k] = it->second[j].pos[k];
+    return n;
+}
+
 void orc_tree_nodes(const orc_tree *h, double *center, double *hw, uint32_t *begin, uint32_t *end,
                     uint32_t *parent, uint32_t *first_daughter, uint8_t *num_daughters, uint8_t *level,
                     uint16_t *orthant, int32_t *node2leaf, int32_t *leaf_nodes)

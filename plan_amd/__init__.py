@@ -21,7 +21,7 @@ import os
 import numpy as np
 
 from .abi import (PlanBoundnessClump, PlanBoundnessIn, PlanBoundnessStats, PlanCatalog, PlanParams, catalog_to_py,
-                  make_params)
+                  make_params, py_to_catalog)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libplan_hip.so")
@@ -51,7 +51,7 @@ EXPORTS = [
     "plan_keep_input", "plan_restore_input", "plan_device_memory", "plan_clump_stats",
     "plan_last_density_radius_ms", "plan_boundness_load", "plan_boundness_load_catalog", "plan_boundness_run",
     "plan_boundness_result", "plan_set_knn_list_limit", "plan_read_lis", "plan_set_particles_lis",
-    "plan_get_particle_set",
+    "plan_get_particle_set", "plan_write_peaks", "plan_write_clump_particles", "plan_write_basic_results",
 ]
 
 
@@ -102,6 +102,9 @@ def lib():
                                 C.POINTER(C.c_double), C.POINTER(C.c_uint32)]
     L.plan_set_particles_lis.argtypes = [vp, C.POINTER(PlanParams), C.c_uint32, vp]
     L.plan_get_particle_set.argtypes = [vp, C.c_uint32, vp, vp, vp, vp, vp, C.POINTER(C.c_uint32)]
+    L.plan_write_peaks.argtypes = [vp, C.POINTER(PlanCatalog), C.c_char_p]
+    L.plan_write_clump_particles.argtypes = [vp, C.POINTER(PlanCatalog), C.c_char_p, C.c_uint32]
+    L.plan_write_basic_results.argtypes = [vp, C.POINTER(PlanCatalog), C.c_char_p, C.c_int]
     for f in EXPORTS:
         if f in ("plan_ctx_destroy", "plan_catalog_free", "plan_last_error"):
             continue
@@ -241,6 +244,26 @@ class Context:
             return catalog_to_py(out.contents)
         finally:
             lib().plan_catalog_free(out)
+
+    # --- catalog files -----------------------------------------------------------
+
+    def write_peaks(self, cat: dict, path) -> None:
+        """The peaks table of OutputPlanetesimalsInfo for a catalog of this
+        context's current tree."""
+        c, keep = py_to_catalog(cat)
+        self._chk(lib().plan_write_peaks(self._h, C.byref(c), os.fsencode(path)), "plan_write_peaks")
+
+    def write_clump_particles(self, cat: dict, directory, sampling_rate: int = 1) -> None:
+        """OutputSinglePlanetesimal for every clump: directory/<peak_index>.txt."""
+        c, keep = py_to_catalog(cat)
+        self._chk(lib().plan_write_clump_particles(self._h, C.byref(c), os.fsencode(directory), int(sampling_rate)),
+                  "plan_write_clump_particles")
+
+    def write_basic_results(self, cat: dict, path, header: bool = False) -> None:
+        """WriteBasicResults: one summary line appended to path."""
+        c, keep = py_to_catalog(cat)
+        self._chk(lib().plan_write_basic_results(self._h, C.byref(c), os.fsencode(path), 1 if header else 0),
+                  "plan_write_basic_results")
 
     # --- pairwise boundness (extension) -----------------------------------------
     FP64, FP32 = 0, 1

@@ -130,6 +130,34 @@ def catalog_to_py(cat: PlanCatalog) -> dict:
                 Mp_max=cat.Mp_max, Mp_tot=cat.Mp_tot)
 
 
+def py_to_catalog(d: dict):
+    """A PlanCatalog over a dict from catalog_to_py; returns (catalog, keep),
+    keep holding the arrays the catalog points into."""
+    import numpy as np
+
+    n = len(d["clumps"])
+    arr = (PlanClump * max(n, 1))()
+    for i, c in enumerate(d["clumps"]):
+        o = arr[i]
+        for k in ("peak_index", "num_members", "num_particles", "member_offset", "total_mass", "Hill_radius",
+                  "one10th_radius", "two_sigma_mass_radius", "max_distance", "peak_density"):
+            setattr(o, k, c[k])
+        for k in ("center_of_mass", "vel_com", "J", "J_quarter_Hill"):
+            for j in range(3):
+                getattr(o, k)[j] = c[k][j]
+    mem = np.ascontiguousarray(d["members"], np.uint32)
+    oid = np.ascontiguousarray(d["member_original_ids"], np.uint32)
+    cat = PlanCatalog()
+    cat.num_clumps = n
+    cat.num_members_total = len(mem)
+    cat.clumps = C.cast(arr, C.POINTER(PlanClump))
+    cat.members = mem.ctypes.data_as(C.POINTER(C.c_uint32))
+    cat.member_original_ids = oid.ctypes.data_as(C.POINTER(C.c_uint32))
+    for k in ("density_threshold", "peak_density_threshold", "clump_mass_threshold", "Mp_max", "Mp_tot"):
+        setattr(cat, k, d[k])
+    return cat, (arr, mem, oid)
+
+
 class PlanBoundnessIn(C.Structure):
     _fields_ = [("num_clumps", C.c_uint32), ("offsets", C.c_void_p), ("pos", C.c_void_p), ("vel", C.c_void_p),
                 ("mass", C.c_void_p), ("G", C.c_double), ("Omega", C.c_double), ("q", C.c_double)]

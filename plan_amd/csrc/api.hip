@@ -851,6 +851,51 @@ int plan_find_planetesimals(plan_ctx *c, plan_catalog **out)
     });
 }
 
+/* members of cat on the device tree they index (the context's current one) */
+static int load_members(plan_ctx *c, const plan_catalog *cat, CatalogMembers &mem)
+{
+    if (!cat) return fail(c, PLAN_E_ARG, "catalog is NULL");
+    if (!c->have_tree) return fail(c, PLAN_E_STATE, "no tree built");
+    catalog_members(c->tree, c->ps, *cat, c->work, c->stream, mem);
+    return PLAN_OK;
+}
+
+int plan_write_peaks(plan_ctx *c, const plan_catalog *cat, const char *path)
+{
+    return guarded(c, [&]() -> int {
+        if (!path) return fail(c, PLAN_E_ARG, "path is NULL");
+        CatalogMembers mem;
+        int r = load_members(c, cat, mem);
+        if (r) return r;
+        const std::string e = write_peaks(*cat, mem, path);
+        return e.empty() ? PLAN_OK : fail(c, PLAN_E_IO, e);
+    });
+}
+
+int plan_write_clump_particles(plan_ctx *c, const plan_catalog *cat, const char *dir, uint32_t sampling_rate)
+{
+    return guarded(c, [&]() -> int {
+        if (!dir) return fail(c, PLAN_E_ARG, "dir is NULL");
+        CatalogMembers mem;
+        int r = load_members(c, cat, mem);
+        if (r) return r;
+        const std::string e = write_clump_particles(*cat, mem, dir, sampling_rate);
+        return e.empty() ? PLAN_OK : fail(c, PLAN_E_IO, e);
+    });
+}
+
+int plan_write_basic_results(plan_ctx *c, const plan_catalog *cat, const char *path, int header)
+{
+    return guarded(c, [&]() -> int {
+        if (!cat || !path) return fail(c, PLAN_E_ARG, "catalog or path is NULL");
+        if (!c->have_particles) return fail(c, PLAN_E_STATE, "no particle set loaded");
+        const double G_tilde = c->pp.four_pi_G / c->d.Omega / c->d.Omega; /* src/global.cpp:40 */
+        const std::string e =
+            write_basic_results(*cat, c->d.mass_total_code_units, G_tilde, c->pp.time, path, header != 0);
+        return e.empty() ? PLAN_OK : fail(c, PLAN_E_IO, e);
+    });
+}
+
 void plan_catalog_free(plan_catalog *cat)
 {
     if (!cat) return;

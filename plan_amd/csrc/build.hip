@@ -670,6 +670,7 @@ void dev_build_tree(DevTree &t, const DevParticles &ps, const plan_derived &d, b
     }
     const uint32_t n = h2[0];
     t.n = n;
+    t.skeys = skeys;
     t.num_sinks = 0;
     uint32_t *start = a.get_t<uint32_t>("rstart", n + 1);
     hipLaunchKernelGGL(k_run_starts, dim3(ceil_div(T, BLK)), dim3(BLK), 0, s, flags, ridx, T, start);

@@ -30,6 +30,7 @@ struct DevTree {
     uint32_t *oid = nullptr;      /* original_id              */
     uint32_t *sinkcnt = nullptr;  /* members merged into this particle (1 = not a sink) */
     uint32_t *sinkoff = nullptr;  /* first member in sorted order (for sink expansion) */
+    uint4 *skeys = nullptr;       /* all T keys in sorted order, payload = particle-set index (sink members) */
     uint4 *keys = nullptr;        /* sorted unique keys {w0, w1, w2, first sorted slot} */
     size_t node_cap = 0;          /* capacity of the node arrays (grow-only) */
     uint32_t list_limit = 0xffffffffu; /* KNN include-list entries per wave (test hook) */
@@ -79,6 +80,23 @@ std::string lis_upload_records(const char *const *paths, uint32_t num_paths, con
  * id = index; false if a species lies outside [0, num_types) */
 bool dev_lis_to_particles(Arena &a, const uint32_t *rec, uint32_t n, uint32_t num_types, DevParticles &ps,
                           hipStream_t s);
+
+/* writers.hip: the reference's catalog files */
+struct CatalogMembers {
+    std::vector<double4> pm, va;  /* tree particle of each catalog member: pos + mass, vel + ath_density */
+    std::vector<uint32_t> off;    /* member i's particles (sinks expanded) are exp[off[i], off[i+1]) */
+    std::vector<double4> exp;     /* their positions, sink members in sorted order */
+};
+void catalog_members(const DevTree &t, const DevParticles &ps, const plan_catalog &cat, Arena &a, hipStream_t s,
+                     CatalogMembers &out);
+/* the peaks table of OutputPlanetesimalsInfo ("" or an error message) */
+std::string write_peaks(const plan_catalog &cat, const CatalogMembers &mem, const char *path);
+/* OutputSinglePlanetesimal for every clump: dir/<peak_index>.txt */
+std::string write_clump_particles(const plan_catalog &cat, const CatalogMembers &mem, const char *dir,
+                                  uint32_t rate);
+/* WriteBasicResults: one line (after the header line if header) appended to path */
+std::string write_basic_results(const plan_catalog &cat, double mass_total, double G_tilde, double time,
+                                const char *path, bool header);
 
 /* build.hip */
 void dev_make_ghosts(DevParticles &ps, const plan_params &pp, const plan_derived &d, hipStream_t s);

@@ -191,6 +191,52 @@ def test_catalog(name):
     assert gc["Mp_tot"] == pytest.approx(oc["Mp_tot"], rel=1e-12)
 
 
+@pytest.mark.parametrize("name", NAMES)
+def test_catalog_files(name, tmp_path):
+    """The peaks table, the per-clump particle lists and the summary line
+    (src/tree.hpp:4935-5222) written by the library equal the CPU
+    restatement's text for the same catalog, byte for byte.  Where a clump
+    holds a sink, its members' order follows the sort's order of equal keys
+    (unspecified in the reference), which can move the last digit of the
+    offset statistics: those tables are compared as numbers."""
+    import oracle_py as O
+
+    snap, ot, g = run_case(name)
+    ctx, cat = g["ctx"], g["cat"]
+    ctx.write_peaks(cat, tmp_path / "peaks.txt")
+    got = (tmp_path / "peaks.txt").read_text()
+    want = O.peaks_text(cat, ot.members)
+    sinks = g["parts"]["sinkcnt"][cat["members"]] > 1 if len(cat["members"]) else np.zeros(0, bool)
+    if not sinks.any():
+        assert got == want
+    else:
+        gl, wl = got.splitlines(), want.splitlines()
+        assert gl[0] == wl[0] and len(gl) == len(wl)
+        for a, b in zip(gl[1:], wl[1:]):
+            assert a[:24] == b[:24]
+            np.testing.assert_allclose([float(a[24 + 24 * k:48 + 24 * k]) for k in range(17)],
+                                       [float(b[24 + 24 * k:48 + 24 * k]) for k in range(17)], rtol=1e-12)
+    d = tmp_path / "ParList"
+    d.mkdir()
+    ctx.write_clump_particles(cat, d, sampling_rate=2)
+    op = ot.particles()
+    for i, c in enumerate(cat["clumps"]):
+        got = (d / f"{c['peak_index']}.txt").read_text()
+        want = O.clump_particles_text(cat, i, op, rate=2)
+        if not sinks.any():
+            assert got == want
+        else:  # a sink's velocity sums its members in the sort's order of equal keys
+            assert [r.split()[0] for r in got.splitlines()] == [r.split()[0] for r in want.splitlines()]
+            np.testing.assert_allclose(np.loadtxt(got.splitlines(), ndmin=2)[:, 1:],
+                                       np.loadtxt(want.splitlines(), ndmin=2)[:, 1:], rtol=1e-12, atol=1e-15)
+    ctx.write_basic_results(cat, tmp_path / "planetesimals.txt", header=True)
+    dv = O.derive(snap.params, snap.n)
+    G_tilde = snap.params["four_pi_G"] / dv.Omega / dv.Omega
+    assert (tmp_path / "planetesimals.txt").read_text() == O.basic_results_text(
+        cat, snap.params.get("time", 0.0), dv.mass_total_code_units, G_tilde, True)
+    print(f"{name}: {len(cat['clumps'])} clumps, sinks in clumps: {int(sinks.sum())}")
+
+
 @pytest.mark.parametrize("limit", [32, 8])
 def test_overflow_paths(limit):
     """Waves whose include list overflows are redone as 16-query waves and,
