@@ -28,6 +28,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -212,6 +213,76 @@ public:
     double Mp_max = 0, Mp_tot = 0;
     std::vector<std::pair<uint32_t, double>> peaks_and_masses;
     std::map<uint32_t, Planetesimal<D>> planetesimals;
+    std::shared_ptr<Context> ctx; /* the context whose tree the indices refer to */
+
+    /* WriteBasicResults (src/tree.hpp:4935-4953): the summary line appended
+     * to path, after the column header for the first snapshot */
+    void WriteBasicResults(const std::string &path, bool first_snapshot) const
+    {
+        Catalog c(*this);
+        ctx->check(plan_write_basic_results(ctx->get(), &c.cat, path.c_str(), first_snapshot ? 1 : 0),
+                   "WriteBasicResults");
+    }
+
+    /* OutputPlanetesimalsInfo (src/tree.hpp:4958-5150): peaks_at_<time>.txt
+     * in dir and, with parlist_dir non-empty (save_clumps), one particle list
+     * per clump there (the caller creates ParList.<NNNN>) */
+    void OutputPlanetesimalsInfo(const std::string &dir, double time, const std::string &parlist_dir = "",
+                                 uint32_t sampling_rate = 1) const
+    {
+        Catalog c(*this);
+        char name[64];
+        std::snprintf(name, sizeof(name), "peaks_at_%07.3f.txt", time);
+        const std::string path = dir.empty() ? std::string(name) : dir + "/" + name;
+        ctx->check(plan_write_peaks(ctx->get(), &c.cat, path.c_str()), "OutputPlanetesimalsInfo");
+        if (!parlist_dir.empty())
+            ctx->check(plan_write_clump_particles(ctx->get(), &c.cat, parlist_dir.c_str(), sampling_rate),
+                       "OutputSinglePlanetesimal");
+    }
+
+private:
+    /* a plan_catalog view of the list, clumps in map order */
+    struct Catalog {
+        std::vector<plan_clump> clumps;
+        std::vector<uint32_t> members, oids;
+        plan_catalog cat{};
+        explicit Catalog(const PlanetesimalList &l)
+        {
+            for (const auto &it : l.planetesimals) {
+                const Planetesimal<D> &p = it.second;
+                plan_clump k{};
+                k.peak_index = p.peak_index;
+                k.num_members = (uint32_t)p.indices.size();
+                k.num_particles = p.num_particles;
+                k.member_offset = (uint32_t)members.size();
+                k.total_mass = p.total_mass;
+                for (int d = 0; d < D; ++d) {
+                    k.center_of_mass[d] = p.center_of_mass[d];
+                    k.vel_com[d] = p.vel_com[d];
+                    k.J[d] = p.J[d];
+                    k.J_quarter_Hill[d] = p.accumulated_J_in_quarter_Hill_radius[d];
+                }
+                k.Hill_radius = p.Hill_radius;
+                k.one10th_radius = p.one10th_radius;
+                k.two_sigma_mass_radius = p.two_sigma_mass_radius;
+                k.max_distance = p.max_distance;
+                k.peak_density = p.peak_density;
+                clumps.push_back(k);
+                members.insert(members.end(), p.indices.begin(), p.indices.end());
+                oids.insert(oids.end(), p.original_ids.begin(), p.original_ids.end());
+            }
+            cat.num_clumps = (uint32_t)clumps.size();
+            cat.num_members_total = (uint32_t)members.size();
+            cat.clumps = clumps.data();
+            cat.members = members.data();
+            cat.member_original_ids = oids.data();
+            cat.density_threshold = l.density_threshold;
+            cat.peak_density_threshold = l.peak_density_threshold;
+            cat.clump_mass_threshold = l.clump_mass_threshold;
+            cat.Mp_max = l.Mp_max;
+            cat.Mp_tot = l.Mp_tot;
+        }
+    };
 };
 
 template <int D> class BHtree;
@@ -271,6 +342,7 @@ public:
         ctx->check(plan_find_planetesimals(ctx->get(), &cat), "FindPlanetesimals");
         std::unique_ptr<plan_catalog, void (*)(plan_catalog *)> hold(cat, plan_catalog_free);
         PlanetesimalList<D> &pl = ds.planetesimal_list;
+        pl.ctx = ctx;
         pl.planetesimals.clear();
         pl.peaks_and_masses.clear();
         pl.density_threshold = cat->density_threshold;

@@ -129,7 +129,10 @@ class OracleTree:
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().orc_tree_free(self.h)
+            try:
+                lib().orc_tree_free(self.h)
+            except TypeError:  # interpreter shutdown: module globals already cleared
+                pass
             self.h = None
 
     @property

@@ -51,6 +51,11 @@ static int run_lis(int argc, char **argv)
         ds.particle_set.MakeGhostParticles(paras);
         ds.tree.BuildTree(paras, ds.particle_set);
         ds.tree.FindPlanetesimals(ds, -1);
+        /* the reference's own files next to the catalog text */
+        const std::string out(argv[3]);
+        const std::string dir = out.substr(0, out.find_last_of('/'));
+        ds.planetesimal_list.OutputPlanetesimalsInfo(dir, ds.particle_set.time);
+        ds.planetesimal_list.WriteBasicResults(dir + "/planetesimals.txt", true);
         return write_catalog(ds, argv[3]);
     } catch (const plan_amd::Error &e) {
         std::fprintf(stderr, "error: %s\n", e.what());

@@ -69,6 +69,16 @@ def test_facade_read_lis(tmp_path):
     for a, b in zip(got["clumps"], ref["clumps"]):
         assert a["peak_index"] == b["peak_index"] and a["total_mass"] == b["total_mass"]
         assert np.array_equal(a["ids"], ref["member_original_ids"][b["member_offset"]:b["member_offset"] + b["num_members"]])
+    # the reference's files, written through the facade, equal the library's for the same snapshot
+    ctx = plan_amd.Context(0)
+    ctx.set_particles(snap.params, snap.pos, snap.vel, snap.density, snap.property_index, snap.id)
+    ctx.make_ghosts()
+    ctx.build_tree()
+    cat = ctx.find_planetesimals()
+    ctx.write_peaks(cat, tmp_path / "want_peaks.txt")
+    assert (tmp_path / "peaks_at_000.000.txt").read_text() == (tmp_path / "want_peaks.txt").read_text()
+    ctx.write_basic_results(cat, tmp_path / "want_basic.txt", header=True)
+    assert (tmp_path / "planetesimals.txt").read_text() == (tmp_path / "want_basic.txt").read_text()
     r = subprocess.run([exe, "--lis", par_path, cat_path, str(tmp_path / "missing.lis")], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 3 and "Failed to open" in r.stderr, (r.returncode, r.stderr)  # src/tree.hpp:1529
