@@ -134,6 +134,21 @@ int plan_write_clump_particles(plan_ctx *ctx, const plan_catalog *cat, const cha
  * header != 0 (the reference writes it for the first snapshot). */
 int plan_write_basic_results(plan_ctx *ctx, const plan_catalog *cat, const char *path, int header);
 
+/* ---- analyses on the tree --------------------------------------------------- */
+
+/* BasicAnalysesWithTree's density against scale (src/analyses.hpp:196-333)
+ * on the built tree: for radii r_i = (cell/2) 2^i, i < num_levels, the
+ * largest sheared-periodic ball count (RecursiveBallSearchCountWithShear,
+ * src/tree.hpp:3383-3402) over the cell-centre grid from z_min to the top
+ * particle (levels 0-2 also at half-cell offsets, level 0 at quarter-cell
+ * offsets), as max_rhop[i] = count x m_particle / (4/3 pi r_i^3).  sums[i]
+ * is the sum of the counts over the level's searches (each distinct search
+ * once).  Writes at most cap entries of each array (any may be NULL).  With
+ * path non-NULL the reference's max_rhop_vs_scale lines are appended there
+ * (the radii header line first when header != 0). */
+int plan_density_vs_scale(plan_ctx *ctx, uint32_t cap, double *radii, double *max_rhop, uint64_t *sums,
+                          uint32_t *num_levels, const char *path, int header);
+
 /* ---- inspection (tests, tools) ------------------------------------------------ */
 
 /* the loaded particle set (ParticleSet<3>::particles, src/tree.hpp:1200-1298),

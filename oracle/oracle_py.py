@@ -58,6 +58,10 @@ def lib():
         L.orc_ball.restype = C.c_uint32
         L.orc_ball.argtypes = [vp, _d, C.c_double, _u32, C.c_uint32]
         L.orc_find_planetesimals.argtypes = [vp]
+        L.orc_density_vs_scale.restype = C.c_uint32
+        L.orc_density_vs_scale.argtypes = [vp, C.c_uint32, _d, _d, P(dtype=np.uint64, flags="C_CONTIGUOUS"), _d]
+        L.orc_sheared_count.restype = C.c_uint32
+        L.orc_sheared_count.argtypes = [vp, _d, C.c_double, _d, C.c_double]
         L.orc_tree_members.restype = C.c_uint32
         L.orc_tree_members.argtypes = [vp, C.c_uint32, _d, C.c_uint32]
         L.orc_densities.argtypes = [vp, _d]
@@ -213,6 +217,18 @@ class OracleTree:
         lib().orc_stage_info(self.h, C.byref(k), C.byref(h), C.byref(a), C.byref(b))
         return dict(kernel=k.value, hop=h.value, naive_clumps=a.value, after_merge=b.value)
 
+    def density_vs_scale(self):
+        """BasicAnalysesWithTree's density-vs-scale pass (src/analyses.hpp:196-333)."""
+        L = lib()
+        cap = 64
+        radii, rhop, sums, zr = np.empty(cap), np.empty(cap), np.empty(cap, np.uint64), np.empty(2)
+        n = L.orc_density_vs_scale(self.h, cap, radii, rhop, sums, zr)
+        return dict(radii=radii[:n].copy(), max_rhop=rhop[:n].copy(), sums=sums[:n].copy(), z_min=zr[0], z_max=zr[1])
+
+    def sheared_count(self, c, r, maxd, shear):
+        return lib().orc_sheared_count(self.h, np.ascontiguousarray(c, np.float64), float(r),
+                                       np.ascontiguousarray(maxd, np.float64), float(shear))
+
     def members(self, i: int):
         """Positions of the particles merged into tree particle i, in sorted order."""
         n = lib().orc_tree_members(self.h, int(i), np.empty(3), 0)
@@ -227,6 +243,14 @@ class OracleTree:
             return catalog_to_py(c.contents)
         finally:
             L.orc_catalog_free(c)
+
+
+def build_tree(snap, params=None):
+    """Ghosts -> tree on a synth.Snapshot (no FindPlanetesimals)."""
+    params = params or snap.params
+    gp, gv, gd, gpr, gid = make_ghosts(params, snap.pos.astype(np.float64), snap.vel.astype(np.float64),
+                                       snap.density.astype(np.float64), snap.property_index, snap.id)
+    return OracleTree(params, snap.n, gp, gv, gd, gpr, gid)
 
 
 def run_pipeline(snap, params=None, threads: int | None = None):

@@ -26,6 +26,7 @@
 #include "../include/plan_params.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
@@ -1313,6 +1314,188 @@ void orc_tree_particles(const orc_tree *h, double *pos, double *vel, double *mas
         oid[i] = q.original_id;
         sink[i] = q.sink_flag;
     }
+}
+
+/* ---- density against scale (BasicAnalysesWithTree, src/analyses.hpp:196-333) ---- */
+
+struct ScaleQuery {
+    const Tree &T;
+    double boxmin[3], boxmax[3], boxlen[3];
+    double maxd[3], shear;
+};
+
+/* ShearedCenter2Center (src/tree.hpp:3307-3330): the node centre's y moves
+ * with the shear once the x offset wraps, and the moved y feeds the y offset */
+static void sheared_c2c(const ScaleQuery &q, const double *c, const Node &nd, double *c2c)
+{
+    double nc[3] = {nd.center[0], nd.center[1], nd.center[2]};
+    for (int i = 0; i < 3; ++i) {
+        c2c[i] = c[i] - nc[i];
+        if (std::fabs(c2c[i]) > q.maxd[i]) {
+            if (c2c[i] > 0) c2c[i] -= q.boxlen[i];
+            else c2c[i] += q.boxlen[i];
+            if (i == 0) {
+                if (c[0] < 0) {
+                    nc[1] -= q.shear;
+                    nc[1] += static_cast<int>((q.boxmax[1] - nc[1]) / q.boxlen[1]) * q.boxlen[1];
+                } else {
+                    nc[1] += q.shear;
+                    nc[1] -= static_cast<int>((nc[1] - q.boxmin[1]) / q.boxlen[1]) * q.boxlen[1];
+                }
+            }
+        }
+    }
+}
+
+/* RecursiveBallSearchCountWithShear with SphereContainNodeWithShear and
+ * SphereNodeIntersectWithShear (src/tree.hpp:3334-3402); leaf particles are
+ * tested at their own (unshifted) positions */
+static void count_sheared(const ScaleQuery &q, const double *c, uint32_t node, double r, uint32_t &count)
+{
+    const Node &nd = q.T.nodes[node];
+    double c2c[3];
+    sheared_c2c(q, c, nd, c2c);
+    const double diag = nd.half_width * std::sqrt(3.0);
+    double t[3];
+    for (int i = 0; i < 3; ++i) t[i] = std::fabs(c2c[i]) + diag;
+    double n2 = 0;
+    for (int i = 0; i < 3; ++i) n2 += t[i] * t[i];
+    if (n2 < r * r) {
+        count += nd.end - nd.begin;
+        return;
+    }
+    double c2 = 0;
+    for (int i = 0; i < 3; ++i) c2 += c2c[i] * c2c[i];
+    const double td = diag + r;
+    if (c2 > td * td) return;
+    bool hit = c2 < r || c2 < nd.half_width;
+    if (!hit) {
+        double md = 0;
+        for (int i = 0; i < 3; ++i) {
+            const double lo = (c[i] - c2c[i]) - nd.half_width, hi = (c[i] - c2c[i]) + nd.half_width;
+            if (c[i] < lo) md += (c[i] - lo) * (c[i] - lo);
+            else if (c[i] > hi) md += (c[i] - hi) * (c[i] - hi);
+        }
+        hit = md <= r * r;
+    }
+    if (!hit) return;
+    if (q.T.is_leaf(node)) {
+        for (uint32_t p = nd.begin; p != nd.end; ++p)
+            if (norm2_diff(c, q.T.p[p].pos) <= r * r) ++count;
+    } else {
+        for (uint32_t dd = nd.first_daughter; dd != nd.first_daughter + nd.num_daughters; ++dd)
+            count_sheared(q, c, dd, r, count);
+    }
+}
+
+/* one sheared ball count from the root (for tests) */
+uint32_t orc_sheared_count(const orc_tree *h, const double *c, double r, const double *maxd, double shear)
+{
+    ScaleQuery q{h->T, {}, {}, {}, {}, shear};
+    for (int k = 0; k < 3; ++k) {
+        q.boxmin[k] = h->pp.box_min[k];
+        q.boxmax[k] = h->pp.box_max[k];
+        q.boxlen[k] = h->d.box_length[k];
+        q.maxd[k] = maxd[k];
+    }
+    uint32_t cnt = 0;
+    count_sheared(q, c, 0, r, cnt);
+    return cnt;
+}
+
+/* RecursiveFindExtreme along z (src/analyses.hpp:171-191) */
+static void find_extreme(const Tree &T, uint32_t node, uint32_t &ext, bool smaller)
+{
+    auto comp = [smaller](double a, double b) { return smaller ? a < b : a > b; };
+    const Node &nd = T.nodes[node];
+    if (T.is_leaf(node)) {
+        for (uint32_t p = nd.begin; p != nd.end; ++p)
+            if (comp(T.p[p].pos[2], T.p[ext].pos[2])) ext = p;
+        return;
+    }
+    for (uint32_t dd = nd.first_daughter; dd != nd.first_daughter + nd.num_daughters; ++dd) {
+        if (comp(T.nodes[dd].center[2], nd.center[2])) find_extreme(T, dd, ext, smaller);
+        else if (comp(nd.center[2], T.p[ext].pos[2])) find_extreme(T, dd, ext, smaller);
+    }
+}
+
+/* the density-vs-scale pass: per level i, radius r_i = (cell/2) 2^i and
+ * the largest sheared ball count over the cell-centre grid (levels 0-2 also
+ * at the 8 half-cell offsets, level 0 at the quarter-cell offsets the
+ * reference's loop visits), as max count x m_particle / (4/3 pi r^3).
+ * out_sums[i] = sum of the counts of level i over every search made (the
+ * reference repeats each level-0 quarter-cell search 8 times; counted once
+ * here).  Returns the number of levels (at most cap written). */
+uint32_t orc_density_vs_scale(const orc_tree *h, uint32_t cap, double *radii, double *rhop, uint64_t *sums,
+                              double *zrange)
+{
+    const Tree &T = h->T;
+    const plan_derived &d = h->d;
+    const plan_params &pp = h->pp;
+    uint32_t emin = 0, emax = 0;
+    find_extreme(T, 0, emin, true);
+    find_extreme(T, 0, emax, false);
+    double z_min = d.box_center[2] - d.cell_length[2] / 2.0;
+    const double z_max = T.p[emax].pos[2];
+    while (z_min - d.cell_length[2] > T.p[emin].pos[2]) z_min -= d.cell_length[2];
+    zrange[0] = z_min;
+    zrange[1] = z_max;
+    const int L = (int)std::round(std::log10(d.box_half_width[0] * 2 / d.cell_length[0]) / std::log10(2.0)) + 1;
+    std::vector<double> rad(L, d.cell_length[0] / 2.0), best(L, 0.0);
+    std::vector<uint64_t> sum(L, 0);
+    ScaleQuery q{T, {}, {}, {}, {}, d.shear_speed * pp.time};
+    for (int k = 0; k < 3; ++k) {
+        q.boxmin[k] = pp.box_min[k];
+        q.boxmax[k] = pp.box_max[k];
+        q.boxlen[k] = d.box_length[k];
+    }
+    std::vector<std::array<double, 3>> maxd(L);
+    for (int i = 0; i < L; ++i) {
+        rad[i] *= std::pow(2.0, i);
+        for (int k = 0; k < 3; ++k) maxd[i][k] = d.box_length[k] - rad[i];
+        maxd[i][2] = 100 * d.max_half_width;
+    }
+    static const int orth[8][3] = {{-1, -1, -1}, {1, -1, -1}, {-1, 1, -1}, {1, 1, -1},
+                                   {-1, -1, 1},  {1, -1, 1},  {-1, 1, 1},  {1, 1, 1}};
+    double cmin[3], step[3], sub[3], subsub[3];
+    for (int k = 0; k < 3; ++k) {
+        cmin[k] = pp.box_min[k] + 0.5 * d.cell_length[k];
+        step[k] = d.cell_length[k];
+        sub[k] = step[k] / 2.0;
+        subsub[k] = step[k] / 4.0;
+    }
+    auto search = [&](int i, const double *c) {
+        for (int k = 0; k < 3; ++k) q.maxd[k] = maxd[i][k];
+        uint32_t cnt = 0;
+        count_sheared(q, c, 0, rad[i], cnt);
+        if (cnt > best[i]) best[i] = cnt;
+        sum[i] += cnt;
+    };
+    for (double z = z_min; z < z_max; z += step[2])
+        for (double y = cmin[1]; y < pp.box_max[1]; y += step[1])
+            for (double x = cmin[0]; x < pp.box_max[0]; x += step[0]) {
+                const double cc[3] = {x, y, z};
+                for (int i = 0; i < L; ++i) {
+                    search(i, cc);
+                    if (i < 3)
+                        for (int o = 0; o < 8; ++o) {
+                            double sc[3];
+                            for (int k = 0; k < 3; ++k) sc[k] = cc[k] + sub[k] * orth[o][k];
+                            search(i, sc);
+                            if (i == 0) {
+                                double ssc[3];
+                                for (int k = 0; k < 3; ++k) ssc[k] = sc[k] + subsub[k] * orth[o][k];
+                                search(i, ssc);
+                            }
+                        }
+                }
+            }
+    for (int i = 0; i < L && i < (int)cap; ++i) {
+        radii[i] = rad[i];
+        rhop[i] = best[i] * (d.mass_per_particle[0] / (4.1887902047863910 * std::pow(rad[i], 3)));
+        sums[i] = sum[i];
+    }
+    return (uint32_t)L;
 }
 
 /* positions of the particles merged into tree particle i (sink_particle_indices,

@@ -52,6 +52,7 @@ EXPORTS = [
     "plan_last_density_radius_ms", "plan_boundness_load", "plan_boundness_load_catalog", "plan_boundness_run",
     "plan_boundness_result", "plan_set_knn_list_limit", "plan_read_lis", "plan_set_particles_lis",
     "plan_get_particle_set", "plan_write_peaks", "plan_write_clump_particles", "plan_write_basic_results",
+    "plan_density_vs_scale",
 ]
 
 
@@ -102,6 +103,7 @@ def lib():
                                 C.POINTER(C.c_double), C.POINTER(C.c_uint32)]
     L.plan_set_particles_lis.argtypes = [vp, C.POINTER(PlanParams), C.c_uint32, vp]
     L.plan_get_particle_set.argtypes = [vp, C.c_uint32, vp, vp, vp, vp, vp, C.POINTER(C.c_uint32)]
+    L.plan_density_vs_scale.argtypes = [vp, C.c_uint32, vp, vp, vp, C.POINTER(C.c_uint32), C.c_char_p, C.c_int]
     L.plan_write_peaks.argtypes = [vp, C.POINTER(PlanCatalog), C.c_char_p]
     L.plan_write_clump_particles.argtypes = [vp, C.POINTER(PlanCatalog), C.c_char_p, C.c_uint32]
     L.plan_write_basic_results.argtypes = [vp, C.POINTER(PlanCatalog), C.c_char_p, C.c_int]
@@ -244,6 +246,21 @@ class Context:
             return catalog_to_py(out.contents)
         finally:
             lib().plan_catalog_free(out)
+
+    # --- analyses on the tree ------------------------------------------------------
+
+    def density_vs_scale(self, path=None, header: bool = False) -> dict:
+        """BasicAnalysesWithTree's density against scale on the built tree:
+        radii, max_rhop and the per-level sums of the counts; with path the
+        reference's max_rhop_vs_scale lines are appended there."""
+        cap = 64
+        radii, rhop, sums = np.empty(cap), np.empty(cap), np.empty(cap, np.uint64)
+        n = C.c_uint32()
+        self._chk(lib().plan_density_vs_scale(self._h, cap, radii.ctypes.data, rhop.ctypes.data, sums.ctypes.data,
+                                              C.byref(n), None if path is None else os.fsencode(path),
+                                              1 if header else 0), "plan_density_vs_scale")
+        k = n.value
+        return dict(radii=radii[:k].copy(), max_rhop=rhop[:k].copy(), sums=sums[:k].copy())
 
     # --- catalog files -----------------------------------------------------------
 

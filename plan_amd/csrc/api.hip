@@ -896,6 +896,38 @@ int plan_write_basic_results(plan_ctx *c, const plan_catalog *cat, const char *p
     });
 }
 
+int plan_density_vs_scale(plan_ctx *c, uint32_t cap, double *radii, double *max_rhop, uint64_t *sums,
+                          uint32_t *num_levels, const char *path, int header)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_tree) return fail(c, PLAN_E_STATE, "no tree built");
+        ScaleResult r;
+        dev_density_vs_scale(c->tree, c->pp, c->d, c->work, c->stream, r);
+        const uint32_t L = (uint32_t)r.radii.size();
+        if (num_levels) *num_levels = L;
+        for (uint32_t i = 0; i < L && i < cap; ++i) {
+            if (radii) radii[i] = r.radii[i];
+            if (max_rhop) max_rhop[i] = r.max_rhop[i];
+            if (sums) sums[i] = r.sums[i];
+        }
+        if (!path) return PLAN_OK;
+        /* the max_rhop_vs_scale file (src/analyses.hpp:255-264, 326-331):
+         * width 15, scientific, default precision */
+        std::FILE *f = std::fopen(path, "a");
+        if (!f) return fail(c, PLAN_E_IO, std::string("Failed to open file ") + path);
+        if (header) {
+            std::fprintf(f, "%15s", "#time");
+            for (double x : r.radii) std::fprintf(f, "%15.6e", x);
+            std::fprintf(f, "\n");
+        }
+        std::fprintf(f, "%15.6e", c->pp.time);
+        for (double x : r.max_rhop) std::fprintf(f, "%15.6e", x);
+        std::fprintf(f, "\n");
+        const bool ok = std::fclose(f) == 0;
+        return ok ? PLAN_OK : fail(c, PLAN_E_IO, std::string("Failed to write file ") + path);
+    });
+}
+
 void plan_catalog_free(plan_catalog *cat)
 {
     if (!cat) return;

@@ -104,6 +104,37 @@ void dev_build_tree(DevTree &t, const DevParticles &ps, const plan_derived &d, b
                     const double *root_center_in, double half_width_in, hipStream_t s);
 
 /* query.hip */
+/* the tree as the traversal kernels see it */
+struct TreeView {
+    const double4 *pm;
+    const double *rho;
+    const NodeGeo *ngeo;
+    const NodeTop *ntop;
+    const uint32_t *ninfo;
+    const uint32_t *skip;
+    const uint32_t *leaf_of;
+    uint32_t n;
+    uint32_t nn;
+    double to_diag;
+    double cell_len[3], half_cell[3], cell_sq[3], three_half_cell[3];
+    double cell_volume;
+    /* bound on how far a particle can sit outside its node's box (Morton
+     * quantisation of the key grid), with a wide allowance for rounding */
+    double margin_abs;
+    uint32_t list_limit; /* include-list entries per wave, at most the kernel's capacity */
+};
+TreeView tree_view(const DevTree &t, const plan_derived &d);
+
+/* analyses.hip: BasicAnalysesWithTree's density against scale */
+struct ScaleResult {
+    std::vector<double> radii, max_rhop; /* per level */
+    std::vector<uint64_t> sums;          /* sum of the counts of every search of the level */
+    double z_min = 0, z_max = 0;
+    uint64_t points = 0;                 /* cell centres searched */
+};
+void dev_density_vs_scale(const DevTree &t, const plan_params &pp, const plan_derived &d, Arena &a, hipStream_t s,
+                          ScaleResult &res);
+
 struct KnnOut {
     uint32_t *idx = nullptr;   /* nq * K, ascending d^2 (optional) */
     double *d2 = nullptr;      /* nq * K (optional) */

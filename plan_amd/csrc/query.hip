@@ -23,26 +23,7 @@
 
 namespace plan {
 
-struct TreeView {
-    const double4 *pm;
-    const double *rho;
-    const NodeGeo *ngeo;
-    const NodeTop *ntop;
-    const uint32_t *ninfo;
-    const uint32_t *skip;
-    const uint32_t *leaf_of;
-    uint32_t n;
-    uint32_t nn;
-    double to_diag;
-    double cell_len[3], half_cell[3], cell_sq[3], three_half_cell[3];
-    double cell_volume;
-    /* bound on how far a particle can sit outside its node's box (Morton
-     * quantisation of the key grid), with a wide allowance for rounding */
-    double margin_abs;
-    uint32_t list_limit; /* include-list entries per wave, at most the kernel's capacity */
-};
-
-static TreeView view_of(const DevTree &t, const plan_derived &d)
+TreeView tree_view(const DevTree &t, const plan_derived &d)
 {
     TreeView v;
     v.pm = t.pm;
@@ -1146,7 +1127,7 @@ void dev_knn(const DevTree &t, uint32_t K, int mode, int kernel, uint32_t nq, co
     if (nq == 0) return;
     if (K > t.n) throw std::runtime_error("KNN: k exceeds the number of tree particles");
     if (qpts && !est) throw std::runtime_error("KNN: point queries need a start radius");
-    TreeView tv = view_of(t, d);
+    TreeView tv = tree_view(t, d);
     switch (K) {
     case 8: launch_knn<8>(t, tv, nq, qidx, qpts, est, mode, kernel, rho_in, out, s); break;
     case 16: launch_knn<16>(t, tv, nq, qidx, qpts, est, mode, kernel, rho_in, out, s); break;
@@ -1372,7 +1353,7 @@ void dev_ball_lists(const DevTree &t, Arena &a, const std::vector<double4> &quer
     plan_derived d{};
     d.cell_length[0] = d.cell_length[1] = d.cell_length[2] = 1;
     d.cell_volume = 1;
-    const TreeView tv = view_of(t, d);
+    const TreeView tv = tree_view(t, d);
     double4 *cr = a.get_t<double4>("bl_q", nq);
     HIPCHK(hipMemcpyAsync(cr, queries.data(), nq * sizeof(double4), hipMemcpyHostToDevice, s));
     const int levels = t.depth + 1;
@@ -1477,7 +1458,7 @@ void dev_ball(const DevTree &t, uint32_t nq, const double4 *centre_r, uint32_t *
     plan_derived d{};
     d.cell_length[0] = d.cell_length[1] = d.cell_length[2] = 1;
     d.cell_volume = 1;
-    TreeView tv = view_of(t, d);
+    TreeView tv = tree_view(t, d);
     if (out) {
         hipLaunchKernelGGL(k_ball<true>, dim3(nq), dim3(64), 0, s, tv, nq, centre_r, counts, offsets, out, rho,
                            rho_min, filter);
