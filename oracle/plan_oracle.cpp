@@ -1885,7 +1885,7 @@ int orc_clumps_bound(const plan_params *pp, const double *sa, const double *sb)
     return grav_bound(a, b, X) ? 1 : 0;
 }
 
-/* Pairwise-potential boundness: the north star's extension, which has no
+/* Pairwise-potential boundness: an extension of the path, which has no
  * counterpart in the reference (its boundness is the two-body test of
  * src/tree.hpp:4615-4674).  Brute force in binary64, potentials recomputed
  * from scratch every iteration, in member order:

@@ -14,7 +14,8 @@
 //
 // The reference decides boundness by a two-body test of each particle against
 // the rest of the clump (src/tree.hpp:4615-4674) and has no pairwise potential
-// at all; this is the extension its north star describes, with its own FP64
+// at all; this is an extension of the path (the boundness clumps/s metric
+// compares its FP32 and FP64 forms), with its own FP64
 // oracle (oracle/plan_oracle.cpp, orc_boundness).  Definition, per clump with
 // bound set B (initially every member):
 //   M = sum_B m, X = sum_B m x / M, V = sum_B m v / M            (binary64)

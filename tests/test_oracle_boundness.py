@@ -1,4 +1,4 @@
-"""Pins of the FP64 pairwise-boundness oracle (the north star's extension;
+"""Pins of the FP64 pairwise-boundness oracle (an extension of the path;
 no reference counterpart): two- and three-body closed forms, the shear term,
 an independent numpy brute force of the whole iteration, and the Plummer
 potential energy."""

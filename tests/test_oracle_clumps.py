@@ -295,7 +295,7 @@ def test_small_mass_low_peak_and_grid_density_veto():
     """RemoveSmallMassAndLowPeak (src/tree.hpp:3524-3546): a clump goes when
     its mass is below the trusted mass, its peak's density below the peak
     threshold, or when none of its first min(n, K) members has a grid
-    density (ath) at or above the peak threshold (F10)."""
+    density (ath) at or above the peak threshold (src/tree.hpp:3533-3540)."""
     K, mthr, pthr = 4, 2.5, 30.0
     #            kept     light    low peak  ath veto  ath beyond K  ath exactly thr
     sizes = [5, 2, 5, 5, 6, 5]
