@@ -26,6 +26,18 @@ def duplicates_case():
     return snap
 
 
+def species_time_case():
+    """Two species with mass weights 1 : 3 (per-type particle masses,
+    src/tree.hpp:1571-1582) at time 0.37, so the ghost images in y move with
+    the shear (src/tree.hpp:1660-1675)."""
+    import numpy as np
+    snap = dense_case(32768, 10, 11, a=4e-4)
+    snap.params["type_info"] = [1.0, 3.0]
+    snap.params["time"] = 0.37
+    snap.property_index = np.random.default_rng(11).integers(0, 2, snap.n).astype(np.int32)
+    return snap
+
+
 CASES = {
     # tree small enough for the one-workgroup sorts of BuildTree and HOP
     "n1k": lambda: dense_case(900, 2, 9, a=2e-4),
@@ -42,4 +54,5 @@ CASES = {
     # Plummer spheres in 10%, K = 32, seed 0
     "c1": lambda: synth.config_c1(seed=0),
     "dup": duplicates_case,
+    "species_time": species_time_case,
 }
