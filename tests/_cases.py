@@ -38,6 +38,16 @@ def species_time_case():
     return snap
 
 
+def options_case():
+    """The parameter-file options of the path set away from their defaults:
+    --No_Ghost (src/tree.hpp:1627-1629), num_peaks = 4 (the ranking cap,
+    src/tree.hpp:4301-4316), clump_diffuse_threshold = 0.45 and
+    Hill_fraction_for_merge = 0.5 (src/global.cpp:192-214)."""
+    snap = dense_case(32768, 12, 12, a=4e-4)
+    snap.params.update(no_ghost=1, num_peaks=4, clump_diffuse_threshold=0.45, Hill_fraction_for_merge=0.5)
+    return snap
+
+
 CASES = {
     # tree small enough for the one-workgroup sorts of BuildTree and HOP
     "n1k": lambda: dense_case(900, 2, 9, a=2e-4),
@@ -55,4 +65,5 @@ CASES = {
     "c1": lambda: synth.config_c1(seed=0),
     "dup": duplicates_case,
     "species_time": species_time_case,
+    "options": options_case,
 }
