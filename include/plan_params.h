@@ -9,11 +9,11 @@
  *   solid_to_gas_ratio       : key "Z"             (src/global.cpp:110-113)
  *   box_resolution           : keys "Nx1..Nx3"     (src/global.cpp:115-122)
  *   four_pi_G                : key "four_pi_G_par" (src/global.cpp:123-126)
- *   num_knn                  : keys "num_neighbors_in_knn_search", "num_knn", "knn" (src/global.cpp:150-163)
- *   num_hop / fixed_num_hop  : keys "num_neighbors_to_hop", "num_hop", "hop"          (src/global.cpp:164-181)
- *   num_peaks                : keys "num_peaks", "num_peak" (src/global.cpp:140-149)
- *   clump_diffuse_threshold  : key "clump_diffuse_threshold" (src/global.cpp:192-202), <0 = automatic
- *   Hill_fraction_for_merge  : key "Hill_fraction_for_merge" (src/global.cpp:204-214), <0 = automatic
+ *   num_knn                  : keys "num_neighbors_in_knn_search", "num_knn", "knn" (src/global.cpp:156-170)
+ *   num_hop / fixed_num_hop  : keys "num_neighbors_to_hop", "num_hop", "hop"          (src/global.cpp:171-188)
+ *   num_peaks                : keys "num_peaks", "num_peak" (src/global.cpp:146-155)
+ *   clump_diffuse_threshold  : key "clump_diffuse_threshold" (src/global.cpp:199-208), <0 = automatic
+ *   Hill_fraction_for_merge  : key "Hill_fraction_for_merge" (src/global.cpp:210-219), <0 = automatic
  *   no_ghost                 : CLI flag --No_Ghost (src/tree.hpp:1627-1629)
  *   type_info                : .lis header per-species mass weights (src/tree.hpp:1602-1611)
  * Quantities the reference derives from these (cell sizes, particle masses,

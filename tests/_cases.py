@@ -42,7 +42,7 @@ def options_case():
     """The parameter-file options of the path set away from their defaults:
     --No_Ghost (src/tree.hpp:1627-1629), num_peaks = 4 (the ranking cap,
     src/tree.hpp:4301-4316), clump_diffuse_threshold = 0.45 and
-    Hill_fraction_for_merge = 0.5 (src/global.cpp:192-214)."""
+    Hill_fraction_for_merge = 0.5 (src/global.cpp:199-219)."""
     snap = dense_case(32768, 12, 12, a=4e-4)
     snap.params.update(no_ghost=1, num_peaks=4, clump_diffuse_threshold=0.45, Hill_fraction_for_merge=0.5)
     return snap
