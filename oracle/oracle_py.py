@@ -10,6 +10,7 @@ import ctypes as C
 import math
 import os
 import subprocess
+import weakref
 
 import numpy as np
 
@@ -25,6 +26,12 @@ _u32 = P(dtype=np.uint32, flags="C_CONTIGUOUS")
 _i32 = P(dtype=np.int32, flags="C_CONTIGUOUS")
 _u8 = P(dtype=np.uint8, flags="C_CONTIGUOUS")
 _u16 = P(dtype=np.uint16, flags="C_CONTIGUOUS")
+# the saddle-point density callback of orc_clump_stages: (point[3], est, user) -> density
+SADDLE_FN = C.CFUNCTYPE(C.c_double, C.POINTER(C.c_double), C.c_double, C.c_void_p)
+
+# stage bits of orc_clump_stages (steps 4-10 of FindPlanetesimals, in this order)
+STAGES = dict(merge_a=1, merge_b=2, rekey=4, unbind=8, rebind=16, filter6=32, rip=64, filter8=128, ghost=256,
+              J=512, rank=1024)
 
 
 def build() -> str:
@@ -85,6 +92,13 @@ def lib():
         L.orc_clumps_bound.argtypes = [vp, _d, _d]
         L.orc_filter_clumps.argtypes = [C.c_uint32, _u32, _u32, _d, _d, _d, C.c_uint, C.c_double, C.c_double, _u8]
         L.orc_rank_clumps.argtypes = [C.c_uint32, _u32, _d, C.c_uint32, _u8, _d]
+        L.orc_tree_set_densities.argtypes = [vp, vp, vp]
+        L.orc_clump_stages.restype = C.c_uint32
+        L.orc_clump_stages.argtypes = [vp, C.c_uint32, _u32, _u32, _u32, C.c_uint, _d, SADDLE_FN, vp]
+        L.orc_key0_erased.restype = C.c_uint32
+        L.orc_key0_erased.argtypes = [vp]
+        L.orc_clump_subpeaks.restype = C.c_uint32
+        L.orc_clump_subpeaks.argtypes = [vp, C.c_uint32, _u32, C.c_uint32]
         _lib = L
     return _lib
 
@@ -130,14 +144,9 @@ class OracleTree:
                                       np.ascontiguousarray(dens, dtype=np.float64),
                                       np.ascontiguousarray(prop, dtype=np.int32),
                                       np.ascontiguousarray(ids, dtype=np.uint32))
-
-    def __del__(self):
-        if getattr(self, "h", None):
-            try:
-                lib().orc_tree_free(self.h)
-            except TypeError:  # interpreter shutdown: module globals already cleared
-                pass
-            self.h = None
+        # freed when the object dies or, at the latest, before the interpreter
+        # tears the module down
+        weakref.finalize(self, lib().orc_tree_free, self.h)
 
     @property
     def n(self):
@@ -235,6 +244,46 @@ class OracleTree:
         out = np.empty((n, 3))
         lib().orc_tree_members(self.h, int(i), out.ravel(), n)
         return out
+
+    def key0_erased(self) -> int:
+        """Clumps keyed by tree index 0 that the merge loops' zero-filled
+        delete lists erased (src/tree.hpp:3874/4000, 4019/4146)."""
+        return int(lib().orc_key0_erased(self.h))
+
+    def set_densities(self, rho=None, ath=None):
+        """Replace new_density / ath_density of the tree particles (tree order)."""
+        r = None if rho is None else np.ascontiguousarray(rho, np.float64)
+        a = None if ath is None else np.ascontiguousarray(ath, np.float64)
+        lib().orc_tree_set_densities(self.h, None if r is None else r.ctypes.data,
+                                     None if a is None else a.ctypes.data)
+
+    def clump_stages(self, clumps, stages, density_threshold, hill_frac=0.75, mass_threshold=0.0, diffuse=0.35,
+                     saddle=None):
+        """Steps 4-10 of FindPlanetesimals on constructed naive clumps.
+        clumps: {key (tree index): member tree indices}; stages: names of
+        STAGES; saddle(point, est) -> density replaces the KNN density at
+        saddle points.  Returns the catalog with each clump's subpeaks."""
+        keys = sorted(clumps)
+        off = np.zeros(len(keys) + 1, np.uint32)
+        mem = []
+        for i, k in enumerate(keys):
+            mem.extend(int(m) for m in clumps[k])
+            off[i + 1] = len(mem)
+        bits = 0
+        for s in stages:
+            bits |= STAGES[s]
+        cb = SADDLE_FN(0) if saddle is None else SADDLE_FN(
+            lambda pt, est, user: float(saddle(np.array([pt[0], pt[1], pt[2]]), est)))
+        settings = np.array([density_threshold, hill_frac, mass_threshold, diffuse], np.float64)
+        L = lib()
+        L.orc_clump_stages(self.h, len(keys), np.array(keys, np.uint32), off, np.array(mem or [0], np.uint32),
+                           bits, settings, cb, None)
+        cat = self.catalog()
+        for i, c in enumerate(cat["clumps"]):
+            buf = np.empty(64, np.uint32)
+            n = L.orc_clump_subpeaks(self.h, i, buf, 64)
+            c["subpeaks"] = [int(x) for x in buf[:min(n, 64)]]
+        return cat
 
     def catalog(self):
         L = lib()

@@ -31,6 +31,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <numeric>
 #include <unordered_map>
@@ -53,7 +54,7 @@ static int g_threads = 1;
 
 /* ReadLisFile's parameter block (src/tree.hpp:1597-1612) after
  * CalculateNewParameters (src/global.cpp:28-47).  Pinned by
- * tests/test_oracle_params.py (closed-form values). */
+ * tests/test_oracle_tree.py::test_derived_parameters_closed_form. */
 static void derive(const plan_params *p, uint32_t N, plan_derived *d)
 {
     std::memset(d, 0, sizeof(*d));
@@ -103,7 +104,7 @@ struct Par {           /* Particle<3> (src/tree.hpp:1154-1198) */
 /* ParticleSet::MakeGhostParticles (src/tree.hpp:1626-1754): x copies with the
  * shearing offset and the int-truncation wrap in y, then y copies of the x
  * copies, then y copies of the originals; ids continue from N in that order.
- * Pinned by tests/test_oracle_ghosts.py (brute-force periodic-image check). */
+ * Pinned by tests/test_oracle_tree.py::test_ghosts_are_shearing_periodic_images (independent periodic images). */
 static void make_ghosts(const plan_params *pp, const plan_derived *d, std::vector<Par> &P)
 {
     const uint32_t N = (uint32_t)P.size();
@@ -160,7 +161,7 @@ static void make_ghosts(const plan_params *pp, const plan_derived *d, std::vecto
 
 /* BaseMortonKey::Double2Int (src/tree.cpp:106-113): low 32 bits of the
  * binary64 d*(2^32-2) + 1.5*2^52, i.e. round-half-even of d*(2^32-2) for
- * d in [0,1).  Pinned by tests/test_oracle_morton.py (closed form). */
+ * d in [0,1).  Pinned by tests/test_oracle_tree.py::test_double_to_grid_edge_values and test_morton_keys_and_order. */
 static inline uint32_t double2int(double d)
 {
     double t = d * 4294967294.0;
@@ -867,6 +868,7 @@ struct Result {
     double density_threshold = 0, peak_density_threshold = 0, clump_mass_threshold = 0;
     double diffuse = 0.55, hill_frac = 0.25;
     double Mp_max = 0, Mp_tot = 0;
+    uint32_t erased0 = 0;      /* clumps with key 0 erased after a merge loop */
 };
 
 static void rank_and_cap(std::map<uint32_t, Clump> &cl, uint32_t num_peaks, Result &R)
@@ -928,20 +930,27 @@ static void build_clump_tree(Tree &CT, const Tree &T, std::map<uint32_t, Clump> 
     CT.build(pp, d, cs, false);
 }
 
-static bool saddle_deep(const Tree &T, const Result &R, const Clump &a, const Clump &b, double ra, double rb,
-                        double est, unsigned K, int kind, double cv)
-{   /* IsSaddlePointDeepEnough / IsHillSaddlePointDeepEnough (src/tree.hpp:4880-4909) */
+static bool saddle_deep(const Result &R, const Clump &a, const Clump &b, double ra, double rb, double est,
+                        const std::function<double(const double *, double)> &saddle)
+{   /* IsSaddlePointDeepEnough / IsHillSaddlePointDeepEnough (src/tree.hpp:4880-4909): the
+     * density at the point ra / (ra + rb) of the way from a's centre to b's */
     double r12[3], sp[3];
     double f = ra / (ra + rb);
     for (int k = 0; k < 3; ++k) r12[k] = b.com[k] - a.com[k];
     for (int k = 0; k < 3; ++k) r12[k] *= f;
     for (int k = 0; k < 3; ++k) sp[k] = a.com[k] + r12[k];
+    return saddle(sp, est) < 2.5 * R.density_threshold;
+}
+
+/* the saddle-point density of the real path: KNN with start radius est and
+ * the density kernel, with particle 0 as the QS kernel's "self" (4885-4890) */
+static double saddle_density(const Tree &T, double *sp, double est, unsigned K, int kind, double cv)
+{
     std::vector<HeapItem> h;
     std::vector<uint32_t> idx(K);
     double rk = 0;
     T.knn(sp, K, rk, idx.data(), h, est);
-    double rho = density_kernel(T, kind, K, cv, 0, idx.data(), h);
-    return rho < 2.5 * R.density_threshold;
+    return density_kernel(T, kind, K, cv, 0, idx.data(), h);
 }
 
 static bool grav_bound(const Clump &a, const Clump &b, const Ctx &X)   /* src/tree.hpp:4867-4874 */
@@ -956,15 +965,26 @@ static bool grav_bound(const Clump &a, const Clump &b, const Ctx &X)   /* src/tr
     return P + Ek < 0.;
 }
 
+/* The density at a saddle point (a point and its KNN start radius est),
+ * IsSaddlePointDeepEnough's query (src/tree.hpp:4880-4892). */
+typedef std::function<double(const double *pt, double est)> SaddleFn;
+
 /* One of the two merge loops (src/tree.hpp:3863-4006 and 4011-4152) with
  * the serial semantics: predators in key order, a clump eaten in this pass
- * cannot eat, merges applied after the pass in order of first appearance. */
-static void merge_loop(Tree &T, Result &R, bool hill, unsigned K, int kind, const plan_params *pp,
-                       const plan_derived *d, const Ctx &X)
+ * cannot eat, merges applied after the pass in order of first appearance;
+ * the second loop recomputes the predator's Hill radius after each merge
+ * (4119-4126).  num is PlanetesimalList::num_planetesimals: the clump-tree
+ * size and the length of the zero-filled list of keys to delete
+ * (peaks_to_be_deleted.resize(num), 3874 / 4019).  The list is erased key by
+ * key after the loop (4000-4003 / 4146-4149), so its unused zero entries
+ * erase the clump whose key is tree index 0 whenever fewer than num clumps
+ * were deleted. */
+static void merge_loop(Tree &T, Result &R, bool hill, uint32_t &num, const SaddleFn &saddle,
+                       const plan_params *pp, const plan_derived *d, const Ctx &X, uint32_t &erased0)
 {
     std::map<uint32_t, Clump> &cl = R.clumps;
     double max_radius = 0.;
-    uint32_t num = (uint32_t)cl.size();
+    const uint32_t num_start = num;
     std::vector<uint32_t> deleted;
     bool happened = true;
     Tree CT;
@@ -991,7 +1011,7 @@ static void merge_loop(Tree &T, Result &R, bool hill, unsigned K, int kind, cons
                     double est = hill ? std::min(p1->second.hill, p2->second.hill) / R.hill_frac
                                       : std::min(r1, r2);
                     double ra = hill ? p1->second.hill : r1, rb = hill ? p2->second.hill : r2;
-                    if (!saddle_deep(T, R, p1->second, p2->second, ra, rb, est, K, kind, d->cell_volume)) eat = true;
+                    if (!saddle_deep(R, p1->second, p2->second, ra, rb, est, saddle)) eat = true;
                 }
                 if (eat) {
                     p2->second.mask = false;
@@ -1015,6 +1035,115 @@ static void merge_loop(Tree &T, Result &R, bool hill, unsigned K, int kind, cons
         num = (uint32_t)(cl.size() - deleted.size());
     }
     for (auto k : deleted) cl.erase(k);
+    if (num_start > deleted.size()) erased0 += (uint32_t)cl.erase(0u);
+}
+
+/* Steps 4-10 of FindPlanetesimals (src/tree.hpp:3863-4323) on the naive
+ * clumps in R, the stages selected by bits of `stages` run in this order:
+ * merge loop A (3863-4006), merge loop B with the Hill radii recomputed
+ * first (4011-4152), real-peak re-keying (4156-4182), unbinding
+ * (4189-4192), rebinding in key order (4196-4208), the mass / peak /
+ * grid-density and loose-stream filters (4213-4225), envelope ripping
+ * (4229-4249), the second RemoveSmallMassAndLowPeak (4253), ghost-clump
+ * removal (4258-4268), angular momenta (4273-4275) and the ranking
+ * (4291-4323).  num is num_planetesimals carried between the merge loops;
+ * after_merge (may be NULL) receives the clump count after merge loop B. */
+enum {
+    ST_MERGE_A = 1, ST_MERGE_B = 2, ST_REKEY = 4, ST_UNBIND = 8, ST_REBIND = 16, ST_FILTER6 = 32, ST_RIP = 64,
+    ST_FILTER8 = 128, ST_GHOST = 256, ST_J = 512, ST_RANK = 1024, ST_ALL = 2047
+};
+static void clump_stages(Tree &T, Result &R, const plan_params *pp, const plan_derived *d, const Ctx &X,
+                         unsigned K, unsigned stages, uint32_t &num, const SaddleFn &saddle, uint32_t *after_merge)
+{
+    std::map<uint32_t, Clump> &cl = R.clumps;
+    std::vector<uint32_t> del;
+    if (stages & ST_MERGE_A) merge_loop(T, R, false, num, saddle, pp, d, X, R.erased0);
+    if (stages & ST_MERGE_B) {
+        for (auto &it : cl) hill_radius(it.second, X);
+        merge_loop(T, R, true, num, saddle, pp, d, X, R.erased0);
+    }
+    if (after_merge) *after_merge = (uint32_t)cl.size();
+
+    /* real peaks into keys (src/tree.hpp:4156-4182): the densest of the key
+     * and the subpeaks, replacing only on a strictly larger density */
+    if (stages & ST_REKEY) {
+        std::vector<uint32_t> old_keys, new_keys;
+        for (auto &it : cl) {
+            uint32_t pk = it.first;
+            for (uint32_t o : it.second.subpeaks)
+                if (T.p[pk].new_density < T.p[o].new_density) pk = o;
+            if (pk != it.first) { old_keys.push_back(it.first); new_keys.push_back(pk); }
+        }
+        for (size_t i = 0; i < old_keys.size(); ++i) {
+            auto ins = cl.emplace(new_keys[i], Clump());
+            if (ins.second) {
+                auto orig = cl.find(old_keys[i]);
+                std::swap(orig->second, ins.first->second);
+                ins.first->second.subpeaks.push_back(orig->first);
+                cl.erase(orig);
+            }
+        }
+    }
+
+    /* 5: unbinding and rebinding */
+    if (stages & ST_UNBIND)
+        for (auto &it : cl) { remove_unbound(it.second, T.p, X); hill_radius(it.second, X); }
+    if (stages & ST_REBIND) {
+        for (auto &it : cl)
+            for (uint32_t m : it.second.indices) T.p[m].in_clump_flag = true;
+        for (auto &it : cl) {
+            size_t before;
+            do {
+                before = it.second.indices.size();
+                search_bound(it.second, T, R.density_threshold, X);
+                hill_radius(it.second, X);
+            } while (it.second.indices.size() > before);
+        }
+    }
+    /* 6: filters */
+    if (stages & ST_FILTER6) {
+        remove_small_mass_low_peak(cl, T, R, K);
+        for (auto &it : cl)
+            if (it.second.mask && dispersion_too_large(it.second, std::min(R.diffuse + 0.1, 0.475)))
+                del.push_back(it.first);
+        for (auto k : del) cl.erase(k);
+        del.clear();
+    }
+    /* 7: envelope ripping: cut at the Hill radius (lower_bound on the
+     * distances), recompute, unbind, until the Hill sphere holds every member */
+    if (stages & ST_RIP) {
+        for (auto &it : cl) {
+            Clump &c = it.second;
+            while (c.hill < c.particles.back().second) {
+                auto low = std::lower_bound(c.particles.begin(), c.particles.end(), PD(0, c.hill), pd_less);
+                c.particles.resize(low - c.particles.begin());
+                c.indices.resize(c.particles.size());
+                if (c.indices.size() == 0) { del.push_back(it.first); break; }
+                kinematics(c, T.p);
+                remove_unbound(c, T.p, X);
+                hill_radius(c, X);
+            }
+        }
+        for (auto k : del) cl.erase(k);
+        del.clear();
+    }
+    /* 8 */
+    if (stages & ST_FILTER8) remove_small_mass_low_peak(cl, T, R, K);
+    /* 9: clumps whose centre lies outside [box_min, box_max] (inclusive) */
+    if (stages & ST_GHOST) {
+        for (auto &it : cl) {
+            bool in = true;
+            for (int k = 0; k < 3; ++k)
+                in = in && (it.second.com[k] >= pp->box_min[k] && it.second.com[k] <= pp->box_max[k]);
+            if (!in) del.push_back(it.first);
+        }
+        for (auto k : del) cl.erase(k);
+        del.clear();
+    }
+    /* 10 */
+    if (stages & ST_J)
+        for (auto &it : cl) angular_momentum(it.second, T, X);
+    if (stages & ST_RANK) rank_and_cap(cl, pp->num_peaks, R);
 }
 
 /* BHtree::FindPlanetesimals (src/tree.hpp:3576-3612, 3619-4339), serial
@@ -1133,76 +1262,13 @@ static void find_planetesimals(Tree &T, const plan_params *pp, const plan_derive
     if (enough == 0) remove_small_mass_low_peak(cl, T, R, K);
     S.naive_clumps = (uint32_t)cl.size();
 
-    /* 4: merging */
-    merge_loop(T, R, false, K, kind, pp, d, X);
-    for (auto &it : cl) hill_radius(it.second, X);
-    merge_loop(T, R, true, K, kind, pp, d, X);
-    S.after_merge = (uint32_t)cl.size();
-
-    /* real peaks into keys (src/tree.hpp:4156-4182) */
-    std::vector<uint32_t> old_keys, new_keys;
-    for (auto &it : cl) {
-        uint32_t pk = it.first;
-        for (uint32_t o : it.second.subpeaks)
-            if (T.p[pk].new_density < T.p[o].new_density) pk = o;
-        if (pk != it.first) { old_keys.push_back(it.first); new_keys.push_back(pk); }
-    }
-    for (size_t i = 0; i < old_keys.size(); ++i) {
-        auto ins = cl.emplace(new_keys[i], Clump());
-        if (ins.second) {
-            auto orig = cl.find(old_keys[i]);
-            std::swap(orig->second, ins.first->second);
-            ins.first->second.subpeaks.push_back(orig->first);
-            cl.erase(orig);
-        }
-    }
-
-    /* 5: unbinding and rebinding */
-    for (auto &it : cl) { remove_unbound(it.second, T.p, X); hill_radius(it.second, X); }
-    for (auto &it : cl)
-        for (uint32_t m : it.second.indices) T.p[m].in_clump_flag = true;
-    for (auto &it : cl) {
-        size_t before;
-        do {
-            before = it.second.indices.size();
-            search_bound(it.second, T, R.density_threshold, X);
-            hill_radius(it.second, X);
-        } while (it.second.indices.size() > before);
-    }
-    /* 6: filters */
-    remove_small_mass_low_peak(cl, T, R, K);
-    for (auto &it : cl)
-        if (it.second.mask && dispersion_too_large(it.second, std::min(R.diffuse + 0.1, 0.475))) del.push_back(it.first);
-    for (auto k : del) cl.erase(k);
-    del.clear();
-    /* 7: envelope ripping */
-    for (auto &it : cl) {
-        Clump &c = it.second;
-        while (c.hill < c.particles.back().second) {
-            auto low = std::lower_bound(c.particles.begin(), c.particles.end(), PD(0, c.hill), pd_less);
-            c.particles.resize(low - c.particles.begin());
-            c.indices.resize(c.particles.size());
-            if (c.indices.size() == 0) { del.push_back(it.first); break; }
-            kinematics(c, T.p);
-            remove_unbound(c, T.p, X);
-            hill_radius(c, X);
-        }
-    }
-    for (auto k : del) cl.erase(k);
-    del.clear();
-    /* 8 */
-    remove_small_mass_low_peak(cl, T, R, K);
-    /* 9: ghost clumps */
-    for (auto &it : cl) {
-        bool in = true;
-        for (int k = 0; k < 3; ++k) in = in && (it.second.com[k] >= pp->box_min[k] && it.second.com[k] <= pp->box_max[k]);
-        if (!in) del.push_back(it.first);
-    }
-    for (auto k : del) cl.erase(k);
-    del.clear();
-    /* 10 */
-    for (auto &it : cl) angular_momentum(it.second, T, X);
-    rank_and_cap(cl, pp->num_peaks, R);
+    /* 4-10 */
+    const double cv = d->cell_volume;
+    uint32_t num = (uint32_t)cl.size();
+    clump_stages(T, R, pp, d, X, K, ST_ALL, num, [&](const double *pt, double est) {
+        double sp[3] = {pt[0], pt[1], pt[2]};
+        return saddle_density(T, sp, est, K, kind, cv);
+    }, &S.after_merge);
 }
 
 } /* namespace orc */
@@ -1587,6 +1653,76 @@ void orc_find_planetesimals(orc_tree *h)
 {
     find_planetesimals(h->T, &h->pp, &h->d, h->N, h->R, h->S);
     h->found = true;
+}
+
+/* ---- the clump stages on constructed clumps (pins of the multi-clump loops) ----
+ * Replaces the tree particles' new_density (rho) and ath_density (ath) when
+ * given (tree order). */
+void orc_tree_set_densities(orc_tree *h, const double *rho, const double *ath)
+{
+    for (uint32_t i = 0; i < h->T.n; ++i) {
+        if (rho) h->T.p[i].new_density = rho[i];
+        if (ath) h->T.p[i].ath_density = ath[i];
+    }
+}
+
+/* Steps 4-10 (clump_stages) on naive clumps of a built tree: clump c has
+ * key keys[c] (a tree index, also its peak) and members members[off[c] ..
+ * off[c+1]) (tree indices), set up by SortParticles as step 3 does
+ * (src/tree.hpp:3816-3817).  settings = {density_threshold, hill_frac,
+ * clump_mass_threshold, diffuse}; the peak threshold is 3x the density
+ * threshold (3660).  saddle(pt, est, user) gives the saddle-point density,
+ * NULL the tree's own KNN density.  The result is the handle's catalog
+ * (orc_catalog) and subpeaks (orc_clump_subpeaks); returns the clump count. */
+typedef double (*orc_saddle_fn)(const double *pt, double est, void *user);
+uint32_t orc_clump_stages(orc_tree *h, uint32_t nc, const uint32_t *keys, const uint32_t *off,
+                          const uint32_t *members, unsigned stages, const double *settings, orc_saddle_fn fn,
+                          void *user)
+{
+    Result &R = h->R;
+    R = Result();
+    R.density_threshold = settings[0];
+    R.hill_frac = settings[1];
+    R.clump_mass_threshold = settings[2];
+    R.diffuse = settings[3];
+    R.peak_density_threshold = 3 * R.density_threshold;
+    for (uint32_t i = 0; i < h->T.n; ++i) h->T.p[i].in_clump_flag = false;
+    for (uint32_t c = 0; c < nc; ++c) {
+        Clump &q = R.clumps[keys[c]];
+        q.peak_index = keys[c];
+        for (uint32_t i = off[c]; i < off[c + 1]; ++i) q.indices.push_back(members[i]);
+        sort_particles(q, h->T.p);
+    }
+    Ctx X;
+    X.d = &h->d;
+    X.G = h->d.grav_constant;
+    X.shear = h->d.q * h->d.Omega;
+    const unsigned K = h->pp.num_knn;
+    const double cv = h->d.cell_volume;
+    const Tree &T = h->T;
+    SaddleFn sf = [&](const double *pt, double est) {
+        if (fn) return fn(pt, est, user);
+        double sp[3] = {pt[0], pt[1], pt[2]};
+        return saddle_density(T, sp, est, K, 0, cv);
+    };
+    uint32_t num = (uint32_t)R.clumps.size();
+    clump_stages(h->T, R, &h->pp, &h->d, X, K, stages, num, sf, &h->S.after_merge);
+    h->found = true;
+    return (uint32_t)R.clumps.size();
+}
+
+/* clumps whose key (tree index 0) the merge loops' zero-filled delete list erased */
+uint32_t orc_key0_erased(const orc_tree *h) { return h->R.erased0; }
+
+/* subpeaks (potential_subpeak_indices) of the i-th clump in map order */
+uint32_t orc_clump_subpeaks(const orc_tree *h, uint32_t i, uint32_t *out, uint32_t cap)
+{
+    auto it = h->R.clumps.begin();
+    for (uint32_t k = 0; k < i && it != h->R.clumps.end(); ++k) ++it;
+    if (it == h->R.clumps.end()) return 0;
+    const std::vector<uint32_t> &s = it->second.subpeaks;
+    for (size_t k = 0; k < s.size() && k < cap; ++k) out[k] = s[k];
+    return (uint32_t)s.size();
 }
 
 void orc_densities(const orc_tree *h, double *rho)
