@@ -62,6 +62,8 @@ def lib():
         L.orc_tree_nodes.argtypes = [vp, _d, _d, _u32, _u32, _u32, _u32, _u8, _u8, _u16, _i32, _i32]
         L.orc_knn.restype = C.c_uint32
         L.orc_knn.argtypes = [vp, _d, C.c_uint32, C.c_uint32, vp, _u32, _d, _d, _d, _u32, _u32]
+        L.orc_knn_candidates.restype = C.c_uint32
+        L.orc_knn_candidates.argtypes = [vp, _d, C.c_uint32, C.c_double, _u32, _d, C.c_uint32]
         L.orc_ball.restype = C.c_uint32
         L.orc_ball.argtypes = [vp, _d, C.c_double, _u32, C.c_uint32]
         L.orc_find_planetesimals.argtypes = [vp]
@@ -201,6 +203,13 @@ class OracleTree:
         if bad:
             raise RuntimeError(f"oracle knn: start node missing for query {bad - 1}")
         return dict(idx=idx, d2=d2, rk=rk, rstop=rs, ncand=nc, ndrop=nd)
+
+    def knn_candidates(self, q, K, est=0.0, cap=1 << 16):
+        """S(r*) of one query: (tree indices, squared distances)."""
+        idx = np.empty(cap, np.uint32); d2 = np.empty(cap)
+        n = lib().orc_knn_candidates(self.h, np.ascontiguousarray(q, np.float64), int(K), float(est), idx, d2, cap)
+        assert n <= cap
+        return idx[:n].copy(), d2[:n].copy()
 
     def ball(self, c, r, cap=1 << 22):
         out = np.empty(cap, np.uint32)

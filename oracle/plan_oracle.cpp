@@ -1639,6 +1639,26 @@ uint32_t orc_knn(const orc_tree *h, const double *q, uint32_t nq, uint32_t K, co
     return bad;
 }
 
+/* The reference's candidate set S(r*) of one KNN query (every particle the
+ * traversal at the final radius offers to the heap, src/tree.hpp:3177-3193):
+ * tree indices and squared distances, at most cap; returns |S(r*)|. */
+uint32_t orc_knn_candidates(const orc_tree *h, const double *q, uint32_t K, double est, uint32_t *idx, double *d2,
+                            uint32_t cap)
+{
+    const Tree &T = h->T;
+    std::vector<HeapItem> hp;
+    std::vector<uint32_t> tmp(K);
+    double r = 0, rs = 0;
+    if (!T.knn(q, K, r, tmp.data(), hp, est, &rs)) return 0;
+    std::vector<HeapItem> all;
+    T.rec_collect(q, 0, rs, all);
+    for (size_t i = 0; i < all.size() && i < cap; ++i) {
+        idx[i] = (uint32_t)all[i].first;
+        d2[i] = all[i].second;
+    }
+    return (uint32_t)all.size();
+}
+
 /* ball search: returns count; fills out (cap entries) in traversal order */
 uint32_t orc_ball(const orc_tree *h, const double *c, double r, uint32_t *out, uint32_t cap)
 {

@@ -1177,7 +1177,7 @@ int plan_stage_times(plan_ctx *c, double *ms, int n)
 int plan_clump_stats(plan_ctx *c, double *v, int n)
 {
     if (!c || !v) return PLAN_E_ARG;
-    double a[43] = {(double)c->hop_na, (double)c->hop_groups, (double)c->cstats.naive, (double)c->cstats.after_merge,
+    double a[44] = {(double)c->hop_na, (double)c->hop_groups, (double)c->cstats.naive, (double)c->cstats.after_merge,
                     (double)c->cstats.merge_passes, (double)c->cstats.pair_candidates,
                     (double)c->cstats.saddle_queries, (double)c->cstats.rebind_searches, c->d2h_ms,
                     (double)c->cstats.rebind_reruns};
@@ -1201,7 +1201,8 @@ int plan_clump_stats(plan_ctx *c, double *v, int n)
     a[40] = c->cstats.rebind_last_ms;
     a[41] = c->density_res_queries;
     a[42] = c->hop.knn_res_queries;
-    for (int i = 0; i < n && i < 43; ++i) v[i] = a[i];
+    a[43] = c->cstats.key0_erased;
+    for (int i = 0; i < n && i < 44; ++i) v[i] = a[i];
     return PLAN_OK;
 }
 

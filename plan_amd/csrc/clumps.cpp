@@ -496,11 +496,17 @@ static std::vector<uint32_t> dispersed_keys(std::map<uint32_t, HClump> &cl, doub
 
 /* one merge loop (src/tree.hpp:3863-4006 with one10th radii, 4011-4152 with
  * Hill radii), serial semantics: predators in key order; a clump eaten in a
- * pass cannot eat in it; merges applied after the pass in pair order */
+ * pass cannot eat in it; merges applied after the pass in pair order.
+ * num is num_planetesimals, carried from one loop to the next: the
+ * reference sizes its list of keys to delete to num, zero-filled
+ * (3874 / 4019), and erases every entry after the loop (4000-4003 /
+ * 4146-4149), so the clump keyed by tree index 0 goes too whenever fewer
+ * than num clumps were deleted. */
 void merge_loop(std::map<uint32_t, HClump> &cl, const HostMembers &hm, const ClumpSettings &cs, const Ops &ops,
-                ClumpServices &svc, bool hill, ClumpStats &st)
+                ClumpServices &svc, bool hill, uint32_t &num, ClumpStats &st)
 {
     double max_radius = 0.;
+    const uint32_t num_start = num;
     std::vector<uint32_t> deleted;
     bool happened = true;
     while (happened) {
@@ -651,8 +657,10 @@ void merge_loop(std::map<uint32_t, HClump> &cl, const HostMembers &hm, const Clu
             cl[k].mask = false;
             deleted.push_back(k);
         }
+        num = (uint32_t)(cl.size() - deleted.size());
     }
     for (auto k : deleted) cl.erase(k);
+    if (num_start > deleted.size()) st.key0_erased += (uint32_t)cl.erase(0u);
 }
 
 }  // namespace
@@ -1335,10 +1343,11 @@ void run_clump_pipeline(HostMembers &hm, const ClumpSettings &cs, ClumpServices 
     lap(0);
 
     /* 4: merging */
-    merge_loop(cl, hm, cs, ops, svc, false, st);
+    uint32_t num = (uint32_t)cl.size();
+    merge_loop(cl, hm, cs, ops, svc, false, num, st);
     lap(1);
     for (auto &it : cl) ops.hill_radius(it.second);
-    merge_loop(cl, hm, cs, ops, svc, true, st);
+    merge_loop(cl, hm, cs, ops, svc, true, num, st);
     lap(2);
     st.after_merge = (uint32_t)cl.size();
 

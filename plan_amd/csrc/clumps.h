@@ -92,6 +92,8 @@ struct ClumpStats {
     uint32_t rebind_searches = 0;
     uint32_t pair_candidates = 0;
     uint32_t rebind_reruns = 0;
+    /* clumps keyed by tree index 0 erased after a merge loop */
+    uint32_t key0_erased = 0;
     uint32_t rebind_iters = 0, rebind_max_iters = 0, rebind_max_n = 0, rebind_max_exact = 0;
     double rebind_max_ms = 0, rebind_max_exact_ms = 0, rebind_max_sort_ms = 0;
     /* rebinding runs: summed time, summed exact-search time (waits included),

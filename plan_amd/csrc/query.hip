@@ -971,8 +971,8 @@ __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, con
                             rho_out, densest, idx_out, d2_out);
         }
     } else {
-        double *bd2 = sd2 + (size_t)blockIdx.x * 64 * K;
-        uint32_t *bix = sidx + (size_t)blockIdx.x * 64 * K;
+        double *bd2 = sd2 + (size_t)blockIdx.x * qpw * K;
+        uint32_t *bix = sidx + (size_t)blockIdx.x * qpw * K;
         if (lane < K)
             for (uint32_t j = 0; j < nvalid; ++j) {
                 bd2[j * K + lane] = inf;
@@ -1073,8 +1073,8 @@ static void launch_select(const TreeView &tv, Arena &a, uint32_t nb, const uint3
             rq = 1;
         }
         if (nr) {
-            double *sd2 = a.get_t<double>("knn_osd2", (size_t)nr * 64 * K);
-            uint32_t *sidx = a.get_t<uint32_t>("knn_osidx", (size_t)nr * 64 * K);
+            double *sd2 = a.get_t<double>("knn_osd2", (size_t)nr * rq * K);
+            uint32_t *sidx = a.get_t<uint32_t>("knn_osidx", (size_t)nr * rq * K);
             hipLaunchKernelGGL((k_knn_select<K, true, FUSE>), dim3(nr), dim3(64), 0, s, tv, nb, qi, qp, rbuf, cbuf, rt,
                                q0, waves, ovf, sd2, sidx, mode, kernel, rho_in, rho_o, dn_o, idx_o, d2_o, rq);
             LAUNCHCHK();

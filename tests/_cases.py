@@ -48,6 +48,62 @@ def options_case():
     return snap
 
 
+def c5_small_case():
+    """The 134M configuration's mix (85% uniform, 15% in 2,000 Plummer clumps,
+    a in [2e-4, 3e-3], K = 64) at 2^22 particles, H = 64 as the full size
+    takes it (src/tree.hpp:3585-3586)."""
+    snap = synth.config_c5(seed=0, n=1 << 22)
+    snap.params.update(hop=64, fixed_hop=1)
+    return snap
+
+
+def lattice_case():
+    """51^3 particles on the cubic lattice k 2^-8, |k| <= 25: every pairwise
+    difference is exact, so squared distances tie exactly and most K-th
+    neighbours are tied (the index tie-break decides)."""
+    import numpy as np
+    k = np.arange(-25, 26, dtype=np.float64) * 2.0 ** -8
+    g = np.stack(np.meshgrid(k, k, k, indexing="ij"), -1).reshape(-1, 3)
+    n = g.shape[0]
+    params = synth.default_params(n, nx=64, knn=32)
+    vel = np.zeros((n, 3), np.float32)
+    vel[:, 1] = (1.5 * g[:, 0]).astype(np.float32)
+    dens = np.full(n, params["Z"] * np.sqrt(2 * np.pi) / 0.2, np.float32)
+    return synth.Snapshot(pos=g.astype(np.float32), vel=vel, density=dens, property_index=np.zeros(n, np.int32),
+                          id=np.arange(n, dtype=np.uint32), params=params)
+
+
+def key0_case():
+    """A clump keyed by tree index 0 that survives merge loop A, so the
+    reference's zero-filled delete list erases it (src/tree.hpp:3874 ->
+    4000-4003).  Without ghosts the particle at box_min in all three
+    coordinates is the first in Morton order; 60 duplicates there make it a
+    heavy sink, 16 particles about 1e-5 away along the three axes and one at
+    4e-5 on the diagonal have it as their densest neighbour (its K/2-th
+    neighbour is nearer than theirs), and the group is compact enough to
+    pass the dispersion filters."""
+    import numpy as np
+    snap = dense_case(32768, 10, 13, a=4e-4)
+    snap.params.update(no_ghost=1)
+    rng = np.random.default_rng(13)
+    c = np.float64(np.float32(-0.1))
+    eps = 1e-5
+    pts = [np.array([c, c, c])] * 60
+    for ax, cnt in ((0, 6), (1, 5), (2, 5)):
+        for _ in range(cnt):
+            o = np.abs(rng.normal(0, 0.03, 3)) * eps
+            o[ax] = eps * (1 + 0.05 * rng.uniform())
+            pts.append(c + o)
+    pts.append(c + 4 * eps / np.sqrt(3) * np.ones(3))
+    pts = np.array(pts).astype(np.float32)
+    k = len(pts)
+    snap.pos[:k] = pts
+    snap.vel[:k] = 0
+    snap.vel[:k, 1] = 1.5 * pts[:, 0]
+    snap.density[:k] = 1e6
+    return snap
+
+
 CASES = {
     # tree small enough for the one-workgroup sorts of BuildTree and HOP
     "n1k": lambda: dense_case(900, 2, 9, a=2e-4),
@@ -66,4 +122,7 @@ CASES = {
     "dup": duplicates_case,
     "species_time": species_time_case,
     "options": options_case,
+    "key0": key0_case,
+    "c5_4m": c5_small_case,
+    "lattice": lattice_case,
 }
