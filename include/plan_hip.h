@@ -176,6 +176,22 @@ int plan_get_hop(plan_ctx *ctx, uint32_t *densest, uint32_t *peak);
  * squared distances in ascending order, and the final search radius. */
 int plan_knn_query(plan_ctx *ctx, uint32_t nq, const uint32_t *particle_idx, const double *pts, const double *est,
                    uint32_t K, uint32_t *idx, double *d2, double *rstop);
+/* plan_knn_query for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc);
+ * it has no counterpart in the reference.  Same queries and results as
+ * plan_knn_query, but every pointer is DEVICE memory on the context's device:
+ * particle_idx[nq] or pts[nq x 3] with est[nq], and the outputs idx[nq x K],
+ * d2[nq x K], rstop[nq] (any output may be NULL).  It calls no hipMalloc or
+ * hipFree: its working buffers live in a fixed device array of
+ * PLAN_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends, so nq is
+ * limited to PLAN_NOALLOC_MAX_QUERIES (larger is PLAN_E_ARG) and K to 8, 16,
+ * 32 or 64.  The array is shared by every call on one GPU: calls that overlap
+ * on one GPU race.  Returns after the results are written (the context's
+ * stream is synchronized). */
+#define PLAN_NOALLOC_MAX_QUERIES 16384u
+#define PLAN_NOALLOC_SCRATCH_BYTES (16u << 20)
+int plan_knn_query_noalloc(plan_ctx *ctx, uint32_t nq, const uint32_t *particle_idx, const double *pts,
+                           const double *est, uint32_t K, uint32_t *idx, double *d2, double *rstop);
 /* Densities at points (IsSaddlePointDeepEnough's query, src/tree.hpp:4880-4892). */
 int plan_density_at(plan_ctx *ctx, uint32_t nq, const double *pts, const double *est, uint32_t K, int kernel,
                     double *rho);

@@ -52,7 +52,7 @@ EXPORTS = [
     "plan_last_density_radius_ms", "plan_boundness_load", "plan_boundness_load_catalog", "plan_boundness_run",
     "plan_boundness_result", "plan_set_knn_list_limit", "plan_read_lis", "plan_set_particles_lis",
     "plan_get_particle_set", "plan_write_peaks", "plan_write_clump_particles", "plan_write_basic_results",
-    "plan_density_vs_scale",
+    "plan_density_vs_scale", "plan_knn_query_noalloc",
 ]
 
 
@@ -85,6 +85,7 @@ def lib():
     L.plan_get_nodes.argtypes = [vp] + [vp] * 11
     L.plan_get_hop.argtypes = [vp, vp, vp]
     L.plan_knn_query.argtypes = [vp, C.c_uint32, vp, vp, vp, C.c_uint32, vp, vp, vp]
+    L.plan_knn_query_noalloc.argtypes = [vp, C.c_uint32, vp, vp, vp, C.c_uint32, vp, vp, vp]
     L.plan_density_at.argtypes = [vp, C.c_uint32, _d, _d, C.c_uint32, C.c_int, _d]
     L.plan_ball_query.argtypes = [vp, _d, C.c_double, vp, C.c_uint32, C.POINTER(C.c_uint32)]
     L.plan_stage_times.argtypes = [vp, _d, C.c_int]
@@ -355,6 +356,15 @@ class Context:
         dn = np.empty(n, np.uint32); pk = np.empty(n, np.uint32)
         self._chk(lib().plan_get_hop(self._h, _ptr(dn), _ptr(pk)), "plan_get_hop")
         return dn, pk
+
+    def knn_device(self, K: int, nq: int, particle_idx: int = 0, pts: int = 0, est: int = 0, idx: int = 0,
+                   d2: int = 0, rstop: int = 0):
+        """plan_knn_query_noalloc: the same queries on DEVICE buffers given by
+        address (0 = none), without any device allocation; nq is limited to
+        PLAN_NOALLOC_MAX_QUERIES."""
+        a = lambda x: vp(int(x)) if x else None
+        self._chk(lib().plan_knn_query_noalloc(self._h, int(nq), a(particle_idx), a(pts), a(est), int(K), a(idx),
+                                               a(d2), a(rstop)), "plan_knn_query_noalloc")
 
     def knn(self, K: int, particle_idx=None, pts=None, est=None):
         if particle_idx is not None:

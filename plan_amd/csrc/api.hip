@@ -384,6 +384,10 @@ double density_threshold(const plan_ctx *c, uint32_t N)
 
 extern "C" {
 
+namespace {
+int scratch_init(int device);
+}
+
 int plan_ctx_create(plan_ctx **out, int device)
 {
     if (!out) return PLAN_E_ARG;
@@ -394,6 +398,7 @@ int plan_ctx_create(plan_ctx **out, int device)
         HIPCHK(hipSetDevice(device));
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
+        if (scratch_init(device) != PLAN_OK) throw std::runtime_error("scratch symbol address");
     } catch (const std::exception &e) {
         delete c;
         return PLAN_E_DEVICE;
@@ -1079,6 +1084,97 @@ int plan_knn_query(plan_ctx *c, uint32_t nq, const uint32_t *pidx, const double 
         if (idx) HIPCHK(hipMemcpy(idx, di, (size_t)nq * K * 4, hipMemcpyDeviceToHost));
         if (d2) HIPCHK(hipMemcpy(d2, dd, (size_t)nq * K * 8, hipMemcpyDeviceToHost));
         if (rstop) HIPCHK(hipMemcpy(rstop, dr, (size_t)nq * 8, hipMemcpyDeviceToHost));
+        return PLAN_OK;
+    });
+}
+
+/* ---- the KNN query entry point that allocates nothing ----------------------
+ *
+ * Its working buffers (radii, counts, overflow flags and scans, the best
+ * lists of queries walked in several fills, the packed query points) are
+ * carved from `scratch`, a fixed array in device memory that each GPU holds
+ * until the process ends.  A kernel needs its device address, which
+ * hipGetSymbolAddress returns once per device (the array's name in host code
+ * is only a host-side stand-in); plan_ctx_create does that with the device
+ * current.  Every call on one device shares the array: calls that overlap on
+ * one GPU race. */
+static __device__ __align__(256) unsigned char scratch[PLAN_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+/* device address of this device's scratch array (the device is current) */
+int scratch_init(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return PLAN_E_ARG;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return PLAN_OK;
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return PLAN_E_DEVICE;
+    g_scratch_ptr[device] = p;
+    return PLAN_OK;
+}
+
+void *scratch_ptr(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+}  // namespace
+
+/* bad[0] += queries with a particle index out of range or, for point
+ * queries, a start radius that is not positive */
+__global__ void k_check_queries(const uint32_t *__restrict__ pidx, const double *__restrict__ est, uint32_t nq,
+                                uint32_t n, uint32_t *__restrict__ bad)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    const bool wrong = pidx ? pidx[i] >= n : !(est[i] > 0);
+    if (wrong) atomicAdd(bad, 1u);
+}
+
+int plan_knn_query_noalloc(plan_ctx *c, uint32_t nq, const uint32_t *pidx, const double *pts, const double *est,
+                           uint32_t K, uint32_t *idx, double *d2, double *rstop)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_tree) return fail(c, PLAN_E_STATE, "no tree built");
+        if (!pidx && (!pts || !est)) return fail(c, PLAN_E_ARG, "point queries need pts and est");
+        if (K == 0 || K > c->tree.n) return fail(c, PLAN_E_ARG, "K must be in 1..num_particles");
+        if (K != 8 && K != 16 && K != 32 && K != 64) return fail(c, PLAN_E_ARG, "KNN: supported k are 8, 16, 32, 64");
+        if (nq > PLAN_NOALLOC_MAX_QUERIES)
+            return fail(c, PLAN_E_ARG, "plan_knn_query_noalloc: at most PLAN_NOALLOC_MAX_QUERIES queries per call");
+        if (nq == 0) return PLAN_OK;
+        void *base = scratch_ptr(c->device);
+        if (!base) return fail(c, PLAN_E_STATE, "scratch array not set up on this device");
+        hipStream_t s = c->stream;
+        Arena a(base, PLAN_NOALLOC_SCRATCH_BYTES);
+        uint32_t *bad = a.get_t<uint32_t>("kq_bad", 4);
+        HIPCHK(hipMemsetAsync(bad, 0, 4, s));
+        hipLaunchKernelGGL(k_check_queries, dim3(ceil_div(nq, 256)), dim3(256), 0, s, pidx, est, nq, c->tree.n, bad);
+        LAUNCHCHK();
+        uint32_t nbad = 0;
+        HIPCHK(hipMemcpyAsync(&nbad, bad, 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (nbad)
+            return fail(c, PLAN_E_ARG, pidx ? "particle index out of range"
+                                            : "point queries need a positive start radius");
+        KnnOut ko;
+        ko.idx = idx;
+        ko.d2 = d2;
+        ko.rstop = rstop;
+        if (pidx) {
+            dev_knn(c->tree, K, 2, 0, nq, pidx, nullptr, est, c->tree.rho, c->d, ko, s, &a);
+        } else {
+            double4 *q4 = a.get_t<double4>("kq_q4", nq);
+            double *e2 = a.get_t<double>("kq_e2", nq);
+            hipLaunchKernelGGL(k_split_pts, dim3(ceil_div(nq, 256)), dim3(256), 0, s, pts, est, nq, q4, e2);
+            LAUNCHCHK();
+            dev_knn(c->tree, K, 2, 0, nq, nullptr, q4, e2, c->tree.rho, c->d, ko, s, &a);
+        }
+        HIPCHK(hipStreamSynchronize(s));
         return PLAN_OK;
     });
 }

@@ -147,10 +147,11 @@ struct KnnOut {
 };
 /* mode: 0 density, 1 hop (densest neighbour), 2 lists only.
  * Queries are tree particles qidx[i] (or i when qidx is null) or explicit
- * points qpts with start radii est. kernel: 0 median spherical, 1 QS. */
+ * points qpts with start radii est. kernel: 0 median spherical, 1 QS.
+ * Working buffers come from scratch when given, else from the tree's arena. */
 void dev_knn(const DevTree &t, uint32_t K, int mode, int kernel, uint32_t nq, const uint32_t *qidx,
              const double4 *qpts, const double *est, const double *rho_in, const plan_derived &d, KnnOut out,
-             hipStream_t s);
+             hipStream_t s, Arena *scratch = nullptr);
 /* ball search for nq centres/radii: two-pass (count, fill) into a CSR list,
  * optionally keeping only particles with rho >= rho_min (rho_min_on). */
 void dev_ball(const DevTree &t, uint32_t nq, const double4 *centre_r, uint32_t *counts, const uint32_t *offsets,

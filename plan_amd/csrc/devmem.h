@@ -8,14 +8,29 @@
 namespace plan {
 
 /* A named, grow-only device allocation pool.  Buffers keep their capacity
- * across calls so repeated pipeline runs on one context do not allocate. */
+ * across calls so repeated pipeline runs on one context do not allocate.
+ * Built over a fixed device region (base, cap) instead, it never allocates:
+ * each name gets a 256-byte aligned slice carved from the region (reused
+ * while it is large enough), and a request that does not fit throws. */
 class Arena {
 public:
+    Arena() {}
+    Arena(void *base, size_t cap) : base_(static_cast<char *>(base)), cap_(cap) {}
     ~Arena() { release(); }
     void *get(const std::string &name, size_t bytes)
     {
         if (bytes == 0) bytes = 16;
         auto &b = bufs_[name];
+        if (base_) {
+            if (b.cap < bytes) {
+                const size_t need = (bytes + 255) & ~(size_t)255;
+                if (need > cap_ - used_) throw std::runtime_error("fixed scratch region exhausted by " + name);
+                b.p = base_ + used_;
+                b.cap = need;
+                used_ += need;
+            }
+            return b.p;
+        }
         if (b.cap < bytes) {
             if (b.p) HIPCHK(hipFree(b.p));
             b.p = nullptr;
@@ -31,9 +46,11 @@ public:
     }
     void release()
     {
-        for (auto &kv : bufs_)
-            if (kv.second.p) (void)hipFree(kv.second.p);
+        if (!base_)
+            for (auto &kv : bufs_)
+                if (kv.second.p) (void)hipFree(kv.second.p);
         bufs_.clear();
+        used_ = 0;
     }
     size_t bytes() const
     {
@@ -48,6 +65,8 @@ private:
         size_t cap = 0;
     };
     std::unordered_map<std::string, Buf> bufs_;
+    char *base_ = nullptr;   /* fixed region, or null for hipMalloc'ed buffers */
+    size_t cap_ = 0, used_ = 0;
 };
 
 /* device bytes owned by one call (freed when it returns) */

@@ -1092,9 +1092,10 @@ static void launch_select(const TreeView &tv, Arena &a, uint32_t nb, const uint3
  * scattered, so each gets a wave of its own. */
 template <int K>
 static void launch_knn(const DevTree &t, const TreeView &tv, uint32_t nq, const uint32_t *qidx, const double4 *qpts,
-                       const double *est, int mode, int kernel, const double *rho_in, KnnOut out, hipStream_t s)
+                       const double *est, int mode, int kernel, const double *rho_in, KnnOut out, hipStream_t s,
+                       Arena *scratch)
 {
-    Arena &a = const_cast<DevTree &>(t).arena;
+    Arena &a = scratch ? *scratch : const_cast<DevTree &>(t).arena;
     double *rbuf = a.get_t<double>("knn_r", (size_t)nq);
     uint32_t *cbuf = a.get_t<uint32_t>("knn_c", (size_t)nq);
     const bool fuse = !qpts && !est;
@@ -1122,17 +1123,17 @@ static void launch_knn(const DevTree &t, const TreeView &tv, uint32_t nq, const 
 
 void dev_knn(const DevTree &t, uint32_t K, int mode, int kernel, uint32_t nq, const uint32_t *qidx,
              const double4 *qpts, const double *est, const double *rho_in, const plan_derived &d, KnnOut out,
-             hipStream_t s)
+             hipStream_t s, Arena *scratch)
 {
     if (nq == 0) return;
     if (K > t.n) throw std::runtime_error("KNN: k exceeds the number of tree particles");
     if (qpts && !est) throw std::runtime_error("KNN: point queries need a start radius");
     TreeView tv = tree_view(t, d);
     switch (K) {
-    case 8: launch_knn<8>(t, tv, nq, qidx, qpts, est, mode, kernel, rho_in, out, s); break;
-    case 16: launch_knn<16>(t, tv, nq, qidx, qpts, est, mode, kernel, rho_in, out, s); break;
-    case 32: launch_knn<32>(t, tv, nq, qidx, qpts, est, mode, kernel, rho_in, out, s); break;
-    case 64: launch_knn<64>(t, tv, nq, qidx, qpts, est, mode, kernel, rho_in, out, s); break;
+    case 8: launch_knn<8>(t, tv, nq, qidx, qpts, est, mode, kernel, rho_in, out, s, scratch); break;
+    case 16: launch_knn<16>(t, tv, nq, qidx, qpts, est, mode, kernel, rho_in, out, s, scratch); break;
+    case 32: launch_knn<32>(t, tv, nq, qidx, qpts, est, mode, kernel, rho_in, out, s, scratch); break;
+    case 64: launch_knn<64>(t, tv, nq, qidx, qpts, est, mode, kernel, rho_in, out, s, scratch); break;
     default: throw std::runtime_error("KNN: supported k are 8, 16, 32, 64");
     }
 }
