@@ -122,6 +122,9 @@ struct TreeView {
      * quantisation of the key grid), with a wide allowance for rounding */
     double margin_abs;
     uint32_t list_limit; /* include-list entries per wave, at most the kernel's capacity */
+    /* msum[k]: k masses m0 summed one by one from zero (k <= 64), or null */
+    const double *msum = nullptr;
+    double m0 = 0;
 };
 TreeView tree_view(const DevTree &t, const plan_derived &d);
 

@@ -689,7 +689,7 @@ template <int CAP> struct ResumeLds {
  * then candidates are taken 64 at a time (lane c of a chunk locates its
  * particle by binary search over the prefix), measured, filtered by T,
  * sorted and merged. */
-template <class LDS>
+template <int K, class LDS>
 __device__ __forceinline__ void select_list(const TreeView &tv, LDS &L, uint32_t nent, int j, double qx,
                                             double qy, double qz, double T, double &bd, uint32_t &bi, bool &first)
 {
@@ -739,6 +739,15 @@ __device__ __forceinline__ void select_list(const TreeView &tv, LDS &L, uint32_t
             if (dd <= T) {
                 d = dd;
                 id = p;
+            }
+        }
+        if (!first) {
+            /* only keys below the current K-th can enter the K best */
+            const double kd = lane_f64(bd, K - 1);
+            const uint32_t ki = lane_u32(bi, K - 1);
+            if (id != PLAN_NONE && !key_less(d, id, kd, ki)) {
+                d = inf;
+                id = PLAN_NONE;
             }
         }
         if (!__ballot(id != PLAN_NONE)) continue;
@@ -806,10 +815,15 @@ __device__ __forceinline__ void finish_query(const TreeView &tv, double bd, uint
         const int is = in ? 63 - __clzll((long long)in) : K / 2;
         const double tm = lane_f64(bd, is);
         const double w = (lane <= is) ? tv.pm[bi].w : 0.0;
-        /* masses from i* down to 0; the lanes above i* add +0 to the
-         * leading zero, which changes nothing */
+        if (tv.msum && !__ballot(lane <= is && w != tv.m0)) {
+            /* i* + 1 equal masses: their sum in that order is the table's */
+            rho = tv.msum[is + 1];
+        } else {
+            /* masses from i* down to 0; the lanes above i* add +0 to the
+             * leading zero, which changes nothing */
 #pragma unroll
-        for (int j = K - 1; j >= 0; --j) rho = __dadd_rn(rho, lane_f64(w, j));
+            for (int j = K - 1; j >= 0; --j) rho = __dadd_rn(rho, lane_f64(w, j));
+        }
         const double four_pi_over_three = (3.14159265358979323846 * 4.) / 3.;
         rho = __ddiv_rn(rho, __dmul_rn(four_pi_over_three, pow15(tm)));
     } else {
@@ -965,7 +979,7 @@ __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, con
             double bd = inf;
             uint32_t bi = PLAN_NONE;
             bool first = true;
-            select_list(tv, L, nent, (int)j, lane_f64(qx, j), lane_f64(qy, j), lane_f64(qz, j), lane_f64(T, j), bd,
+            select_list<K>(tv, L, nent, (int)j, lane_f64(qx, j), lane_f64(qy, j), lane_f64(qz, j), lane_f64(T, j), bd,
                         bi, first);
             finish_query<K>(tv, bd, bi, wv * qpw + j, lane_u32(self, j), qpts || qidx, mode, kernel, rho_in,
                             rho_out, densest, idx_out, d2_out);
@@ -995,7 +1009,7 @@ __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, con
                 double bd = lane < K ? bd2[j * K + lane] : inf;
                 uint32_t bi = lane < K ? bix[j * K + lane] : PLAN_NONE;
                 bool first = false;
-                select_list(tv, L, nent, (int)j, lane_f64(qx, j), lane_f64(qy, j), lane_f64(qz, j), lane_f64(T, j),
+                select_list<K>(tv, L, nent, (int)j, lane_f64(qx, j), lane_f64(qy, j), lane_f64(qz, j), lane_f64(T, j),
                             bd, bi, first);
                 if (lane < K) {
                     bd2[j * K + lane] = bd;
@@ -1090,6 +1104,19 @@ static void launch_select(const TreeView &tv, Arena &a, uint32_t nb, const uint3
  * are consecutive in Morton order, so 64 of them share one wave's walk;
  * point queries (saddle points between clumps, callers' points) are
  * scattered, so each gets a wave of its own. */
+/* msum[k] = ((0 + m) + m) + ... with k terms (k <= 64): the median kernel's
+ * mass sum when its i* + 1 neighbours all have mass m, in its order */
+__global__ void k_mass_sums(double m, double *__restrict__ msum)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s = 0.0;
+    msum[0] = s;
+    for (int k = 1; k <= 64; ++k) {
+        s = __dadd_rn(s, m);
+        msum[k] = s;
+    }
+}
+
 template <int K>
 static void launch_knn(const DevTree &t, const TreeView &tv, uint32_t nq, const uint32_t *qidx, const double4 *qpts,
                        const double *est, int mode, int kernel, const double *rho_in, KnnOut out, hipStream_t s,
@@ -1129,6 +1156,15 @@ void dev_knn(const DevTree &t, uint32_t K, int mode, int kernel, uint32_t nq, co
     if (K > t.n) throw std::runtime_error("KNN: k exceeds the number of tree particles");
     if (qpts && !est) throw std::runtime_error("KNN: point queries need a start radius");
     TreeView tv = tree_view(t, d);
+    if (mode == 0 && kernel == 0) {
+        /* sums of equal masses for the median kernel (species 0's mass) */
+        Arena &a = scratch ? *scratch : const_cast<DevTree &>(t).arena;
+        double *ms = a.get_t<double>("knn_msum", 65);
+        hipLaunchKernelGGL(k_mass_sums, dim3(1), dim3(64), 0, s, d.mass_per_particle[0], ms);
+        LAUNCHCHK();
+        tv.msum = ms;
+        tv.m0 = d.mass_per_particle[0];
+    }
     switch (K) {
     case 8: launch_knn<8>(t, tv, nq, qidx, qpts, est, mode, kernel, rho_in, out, s, scratch); break;
     case 16: launch_knn<16>(t, tv, nq, qidx, qpts, est, mode, kernel, rho_in, out, s, scratch); break;
