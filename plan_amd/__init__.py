@@ -403,8 +403,8 @@ class Context:
         return dict(zip(("ghosts", "build", "density", "hop", "clumps", "find_total"), t.tolist()))
 
     def clump_stats(self) -> dict:
-        v = np.zeros(44)
-        self._chk(lib().plan_clump_stats(self._h, v, 44), "plan_clump_stats")
+        v = np.zeros(46)
+        self._chk(lib().plan_clump_stats(self._h, v, 46), "plan_clump_stats")
         names = ["num_above", "hop_groups", "naive", "after_merge", "merge_passes", "pair_candidates",
                  "saddle_queries", "rebind_searches", "d2h_ms", "rebind_reruns", "ms_sort", "ms_mergeA", "ms_mergeB",
                  "ms_rekey", "ms_unbind", "ms_rebind", "ms_filters", "ms_rip", "ms_J", "ms_rank",
@@ -415,7 +415,8 @@ class Context:
                  "rebind_max_exact", "rebind_max_exact_ms", "rebind_max_sort_ms",
                  "rebind_work_ms", "rebind_exact_total_ms", "rebind_max_slot",
                  "rebind_last_end_ms", "rebind_last_slot", "rebind_last_ms",
-                 "density_resumed_queries", "hop_resumed_queries", "key0_erased"]
+                 "density_resumed_queries", "hop_resumed_queries", "key0_erased",
+                 "density_pow_checks", "density_pow_fixes"]
         return {k: float(x) for k, x in zip(names, v)}
 
     def keep_input(self):

@@ -51,6 +51,8 @@ struct plan_ctx {
     double density_radius_ms = 0;
     uint32_t density_ovf_waves = 0;
     uint32_t density_res_queries = 0; /* queries walked in several fills */
+    uint32_t density_pow_checks = 0; /* densities checked against the C library's pow */
+    uint32_t density_pow_fixes = 0;  /* densities redone with the C library's pow */
     hipEvent_t ev[8];
 };
 
@@ -436,6 +438,10 @@ static int set_common(plan_ctx *c, const plan_params *params, uint32_t N)
     }
     if (params->clump_diffuse_threshold == 0)
         return fail(c, PLAN_E_ARG, "clump_diffuse_threshold must be larger than 0 (or negative for automatic)");
+    if (!knn_k_supported(params->num_knn))
+        return fail(c, PLAN_E_ARG, "num_knn must be 8, 16, 32 or 64 (neighbour lists hold at most 64 per query)");
+    if (!knn_k_supported(params->num_hop))
+        return fail(c, PLAN_E_ARG, "num_hop must be 8, 16, 32 or 64 (neighbour lists hold at most 64 per query)");
     c->pp = *params;
     derive_params(params, N, &c->d);
     c->ps.N = c->ps.T = N;
@@ -656,8 +662,12 @@ static int run_density(plan_ctx *c, uint32_t K, int kernel)
     ko.ev = &c->ev[5];
     c->density_ovf_waves = 0;
     c->density_res_queries = 0;
+    c->density_pow_checks = 0;
+    c->density_pow_fixes = 0;
     ko.novf = &c->density_ovf_waves;
     ko.nres = &c->density_res_queries;
+    ko.nchk = &c->density_pow_checks;
+    ko.npow = &c->density_pow_fixes;
     HIPCHK(hipEventRecord(c->ev[2], c->stream));
     dev_knn(c->tree, K, 0, kernel, c->tree.n, nullptr, nullptr, nullptr, nullptr, c->d, ko, c->stream);
     HIPCHK(hipEventRecord(c->ev[3], c->stream));
@@ -1273,7 +1283,7 @@ int plan_stage_times(plan_ctx *c, double *ms, int n)
 int plan_clump_stats(plan_ctx *c, double *v, int n)
 {
     if (!c || !v) return PLAN_E_ARG;
-    double a[44] = {(double)c->hop_na, (double)c->hop_groups, (double)c->cstats.naive, (double)c->cstats.after_merge,
+    double a[46] = {(double)c->hop_na, (double)c->hop_groups, (double)c->cstats.naive, (double)c->cstats.after_merge,
                     (double)c->cstats.merge_passes, (double)c->cstats.pair_candidates,
                     (double)c->cstats.saddle_queries, (double)c->cstats.rebind_searches, c->d2h_ms,
                     (double)c->cstats.rebind_reruns};
@@ -1298,7 +1308,9 @@ int plan_clump_stats(plan_ctx *c, double *v, int n)
     a[41] = c->density_res_queries;
     a[42] = c->hop.knn_res_queries;
     a[43] = c->cstats.key0_erased;
-    for (int i = 0; i < n && i < 44; ++i) v[i] = a[i];
+    a[44] = c->density_pow_checks;
+    a[45] = c->density_pow_fixes;
+    for (int i = 0; i < n && i < 46; ++i) v[i] = a[i];
     return PLAN_OK;
 }
 

@@ -42,6 +42,7 @@ struct DevTree {
     int32_t *node2leaf = nullptr;
     int32_t *leaf_nodes = nullptr;
     Arena arena;
+    HostPinned host_stage; /* pinned host staging of the density re-check */
 };
 
 /* Particle set before the tree (ParticleSet<3>, src/tree.hpp:1200-1298). */
@@ -125,6 +126,12 @@ struct TreeView {
     /* msum[k]: k masses m0 summed one by one from zero (k <= 64), or null */
     const double *msum = nullptr;
     double m0 = 0;
+    /* densities to recompute with the C library's pow: count, capacity, and
+     * per entry the output index, x and the mass sum (null: none kept) */
+    uint32_t *pw_count = nullptr;
+    uint32_t pw_cap = 0;
+    uint32_t *pw_idx = nullptr;
+    double *pw_x = nullptr, *pw_num = nullptr;
 };
 TreeView tree_view(const DevTree &t, const plan_derived &d);
 
@@ -147,8 +154,13 @@ struct KnnOut {
     hipEvent_t *ev = nullptr;  /* 3 events: before the radius pass, around the first select launch */
     uint32_t *novf = nullptr;  /* host counter: waves whose include list overflowed */
     uint32_t *nres = nullptr;  /* host counter: queries walked in several fills (one per wave) */
+    uint32_t *nchk = nullptr;  /* host counter: densities checked against the C library's pow */
+    uint32_t *npow = nullptr;  /* host counter: densities redone with the C library's pow */
 };
+/* neighbour counts the KNN kernels are built for */
+inline bool knn_k_supported(uint32_t K) { return K == 8 || K == 16 || K == 32 || K == 64; }
 /* mode: 0 density, 1 hop (densest neighbour), 2 lists only.
+ * K must satisfy knn_k_supported(K).
  * Queries are tree particles qidx[i] (or i when qidx is null) or explicit
  * points qpts with start radii est. kernel: 0 median spherical, 1 QS.
  * Working buffers come from scratch when given, else from the tree's arena. */

@@ -21,6 +21,8 @@
 // neighbour resolve to the smaller index.
 #include "ctx.h"
 
+#include <cmath>
+
 namespace plan {
 
 TreeView tree_view(const DevTree &t, const plan_derived &d)
@@ -148,17 +150,62 @@ __device__ __forceinline__ bool geo_intersect(const NodeGeo &g, double qx, doubl
 /* x^1.5 correctly rounded in all but astronomically rare cases:
  * x * sqrt(x) carried in double-double.  s = RN(sqrt x); e = x - s^2 exactly
  * (one fused multiply-add); sqrt(x) = s + e / (2 s) to first order;
- * x s = p_hi + p_lo exactly; the result rounds p_hi + (p_lo + x e / (2 s))
- * once. */
-__device__ __forceinline__ double pow15(double x)
+ * x s = p_hi + p_lo exactly; the result r rounds p_hi + (p_lo + x e / (2 s))
+ * once.  rem = (p_hi - r) + (p_lo + x e / (2 s)) is the exact x^1.5 minus r
+ * to far below 0.001 ulp.  The same operations on the host and the device
+ * (no contraction), so both give the same r bit for bit. */
+__host__ __device__ inline double pow15_rn(double x, double &rem)
 {
+#pragma clang fp contract(off)
+    rem = 0.0;
     if (!(x > 0.0)) return 0.0;
-    double s = __dsqrt_rn(x);
-    double e = __fma_rn(-s, s, x);
-    double ph = __dmul_rn(x, s);
-    double pl = __fma_rn(x, s, -ph);
-    double c = __ddiv_rn(__dmul_rn(x, e), __dadd_rn(s, s));
-    return __dadd_rn(ph, __dadd_rn(pl, c));
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double s = __dsqrt_rn(x);
+    const double e = __fma_rn(-s, s, x);
+    const double ph = x * s;
+    const double pl = __fma_rn(x, s, -ph);
+#else
+    const double s = std::sqrt(x);
+    const double e = std::fma(-s, s, x);
+    const double ph = x * s;
+    const double pl = std::fma(x, s, -ph);
+#endif
+    const double xe = x * e;
+    const double c = xe / (s + s);
+    const double t = pl + c;
+    const double r = ph + t;
+    const double hd = ph - r;
+    rem = hd + t;
+    return r;
+}
+
+/* pow15_rn with near = true when the C library's pow may round x^1.5 the
+ * other way: the reference calls it (src/tree.hpp:3489), and it is not
+ * correctly rounded.  Its stated worst case (exp within 0.511 ulp, log within
+ * 1.5 * 2^-68 relative, so about 0.011 + 7e-5 |ln x| ulp before the final
+ * rounding for y = 1.5) bounds how far from a rounding midpoint the exact
+ * value can lie and still be rounded the other way; the band below is 0.012 +
+ * 5e-5 (|e| + 1) ulp with e the binary exponent of x (|ln x| <= 0.7 (|e| + 1)).
+ * Results that are powers of two (the spacing below halves) or tiny are
+ * always flagged.  Flagged densities are checked on the host against that
+ * same libm call. */
+__device__ __forceinline__ double pow15_near(double x, bool &near)
+{
+    double rem;
+    const double r = pow15_rn(x, rem);
+    near = false;
+    if (!(x > 0.0)) return r;
+    const uint64_t rb = (uint64_t)__double_as_longlong(r);
+    const uint64_t ex = rb >> 52;
+    if ((rb & 0xfffffffffffffull) == 0ull || ex <= 53ull) {
+        near = true;
+    } else {
+        const int ee = (int)(((uint64_t)__double_as_longlong(x) >> 52) & 0x7ffull) - 1023;
+        const double band = 0.012 + 5e-5 * (double)(abs(ee) + 1);
+        const double ulp = __longlong_as_double((long long)((ex - 52ull) << 52));
+        near = 0.5 - fabs(rem) / ulp < band;
+    }
+    return r;
 }
 
 /* The nodes of one sibling group (siblings are contiguous in the
@@ -825,7 +872,17 @@ __device__ __forceinline__ void finish_query(const TreeView &tv, double bd, uint
             for (int j = K - 1; j >= 0; --j) rho = __dadd_rn(rho, lane_f64(w, j));
         }
         const double four_pi_over_three = (3.14159265358979323846 * 4.) / 3.;
-        rho = __ddiv_rn(rho, __dmul_rn(four_pi_over_three, pow15(tm)));
+        bool near;
+        const double num = rho;
+        rho = __ddiv_rn(rho, __dmul_rn(four_pi_over_three, pow15_near(tm, near)));
+        if (near && tv.pw_count && lane == 0) {
+            const uint32_t slot = atomicAdd(tv.pw_count, 1u);
+            if (slot < tv.pw_cap) {
+                tv.pw_idx[slot] = byq ? q : self;
+                tv.pw_x[slot] = tm;
+                tv.pw_num[slot] = num;
+            }
+        }
     } else {
         /* QseudoQuadraticSplinesKernel; point queries use particle 0 as "self" */
         double term = 0;
@@ -1104,6 +1161,20 @@ static void launch_select(const TreeView &tv, Arena &a, uint32_t nb, const uint3
  * are consecutive in Morton order, so 64 of them share one wave's walk;
  * point queries (saddle points between clumps, callers' points) are
  * scattered, so each gets a wave of its own. */
+/* densities whose x^1.5 the C library's pow rounds the other way: the
+ * median kernel's quotient again with that pow (slot: entry of the
+ * re-check list) */
+__global__ void k_pow_fix(const uint32_t *__restrict__ slot, const double *__restrict__ p, uint32_t n,
+                          const uint32_t *__restrict__ pw_idx, const double *__restrict__ pw_num,
+                          double *__restrict__ rho)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t e = slot[i];
+    const double four_pi_over_three = (3.14159265358979323846 * 4.) / 3.;
+    rho[pw_idx[e]] = __ddiv_rn(pw_num[e], __dmul_rn(four_pi_over_three, p[i]));
+}
+
 /* msum[k] = ((0 + m) + m) + ... with k terms (k <= 64): the median kernel's
  * mass sum when its i* + 1 neighbours all have mass m, in its order */
 __global__ void k_mass_sums(double m, double *__restrict__ msum)
@@ -1164,6 +1235,16 @@ void dev_knn(const DevTree &t, uint32_t K, int mode, int kernel, uint32_t nq, co
         LAUNCHCHK();
         tv.msum = ms;
         tv.m0 = d.mass_per_particle[0];
+        if (out.rho) {
+            /* densities whose x^1.5 lies near a rounding midpoint, recomputed
+             * below with the C library's pow (each query finishes once) */
+            tv.pw_count = a.get_t<uint32_t>("knn_pw_count", 1);
+            tv.pw_cap = nq;
+            tv.pw_idx = a.get_t<uint32_t>("knn_pw_idx", nq);
+            tv.pw_x = a.get_t<double>("knn_pw_x", nq);
+            tv.pw_num = a.get_t<double>("knn_pw_num", nq);
+            HIPCHK(hipMemsetAsync(tv.pw_count, 0, 4, s));
+        }
     }
     switch (K) {
     case 8: launch_knn<8>(t, tv, nq, qidx, qpts, est, mode, kernel, rho_in, out, s, scratch); break;
@@ -1171,6 +1252,55 @@ void dev_knn(const DevTree &t, uint32_t K, int mode, int kernel, uint32_t nq, co
     case 32: launch_knn<32>(t, tv, nq, qidx, qpts, est, mode, kernel, rho_in, out, s, scratch); break;
     case 64: launch_knn<64>(t, tv, nq, qidx, qpts, est, mode, kernel, rho_in, out, s, scratch); break;
     default: throw std::runtime_error("KNN: supported k are 8, 16, 32, 64");
+    }
+    if (tv.pw_count) {
+        /* the reference's pow(x, 1.5) (src/tree.hpp:3489) from the C library
+         * for the listed x; the few that differ from the correctly rounded
+         * value the kernel used are sent back and their densities redone */
+        uint32_t cnt = 0;
+        HIPCHK(hipMemcpyAsync(&cnt, tv.pw_count, 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (cnt > tv.pw_cap) throw std::runtime_error("KNN: density re-check list overflow");
+        if (cnt) {
+            const double *x = (const double *)const_cast<DevTree &>(t).host_stage.get((size_t)cnt * 8);
+            HIPCHK(hipMemcpyAsync((void *)x, tv.pw_x, (size_t)cnt * 8, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            std::vector<uint32_t> bad_slot;
+            std::vector<double> bad_p;
+#pragma omp parallel
+            {
+                std::vector<uint32_t> ls;
+                std::vector<double> lp;
+#pragma omp for schedule(static) nowait
+                for (long i = 0; i < (long)cnt; ++i) {
+                    double rem;
+                    const double g = std::pow(x[i], 1.5);
+                    if (g != pow15_rn(x[i], rem)) {
+                        ls.push_back((uint32_t)i);
+                        lp.push_back(g);
+                    }
+                }
+#pragma omp critical
+                {
+                    bad_slot.insert(bad_slot.end(), ls.begin(), ls.end());
+                    bad_p.insert(bad_p.end(), lp.begin(), lp.end());
+                }
+            }
+            const uint32_t nb = (uint32_t)bad_slot.size();
+            if (nb) {
+                Arena &a = scratch ? *scratch : const_cast<DevTree &>(t).arena;
+                uint32_t *ds = a.get_t<uint32_t>("knn_pw_bad", nb);
+                double *dp = a.get_t<double>("knn_pw_p", nb);
+                HIPCHK(hipMemcpyAsync(ds, bad_slot.data(), (size_t)nb * 4, hipMemcpyHostToDevice, s));
+                HIPCHK(hipMemcpyAsync(dp, bad_p.data(), (size_t)nb * 8, hipMemcpyHostToDevice, s));
+                hipLaunchKernelGGL(k_pow_fix, dim3(ceil_div(nb, 256)), dim3(256), 0, s, ds, dp, nb, tv.pw_idx,
+                                   tv.pw_num, out.rho);
+                LAUNCHCHK();
+                HIPCHK(hipStreamSynchronize(s));
+            }
+            if (out.npow) *out.npow += nb;
+        }
+        if (out.nchk) *out.nchk += cnt;
     }
 }
 

@@ -55,9 +55,8 @@ def test_catalog_properties(run):
 
 def test_whole_snapshot_against_the_oracle(run):
     """The CPU restatement on all 16,777,216 particles (about 20 s on the
-    host's 16 cores): tree order and HOP bit-exact, every density within
-    1e-5 (bit-exact but for the last bit of x^1.5 in about 0.1%), the whole
-    catalog bit-exact."""
+    host's 16 cores): tree order, every density, HOP and the whole catalog
+    bit-exact."""
     import oracle_py as O
 
     snap, ctx, T, info, cat = run
@@ -76,7 +75,7 @@ def test_whole_snapshot_against_the_oracle(run):
     rel = np.abs(rho - orho) / np.abs(orho)
     exact = float(np.mean(rho == orho))
     print(f"[fullsize] densities: max rel {rel.max():.2e}, bit-exact {exact:.6f}", flush=True)
-    assert rel.max() <= 1e-5 and exact > 0.998
+    assert exact == 1.0, (exact, rel.max())
     dn, pk = ctx.hop_result()
     odn, opk = ot.hop()
     assert np.array_equal(dn, odn) and np.array_equal(pk, opk)

@@ -3,10 +3,11 @@
 Bit-exact: ghost images, Morton keys, tree order, node arrays, HOP pointers
 and peaks, neighbour index sets (queries whose K-th distance is not tied;
 tied ones follow the (d^2, index) rule, test_knn_ties_broken_by_index),
-clump peaks, member lists and every catalog field.  Densities: bit-exact for nearly all
-particles, and every one within 1e-5 relative (the only non-bit-exact step
-is x^1.5 in the median kernel: correctly rounded on the device, glibc pow in
-the oracle).
+clump peaks, member lists, every catalog field and every density (the median
+kernel's x^1.5 is correctly rounded on the device; the few values within
+about 0.013 ulp of a rounding midpoint are checked against the C library's pow, which
+the reference and the oracle call, and redone with it where it rounds the
+other way).
 """
 import numpy as np
 import pytest
@@ -90,10 +91,9 @@ def test_density(name):
     orho = ot.densities()
     grho = g["parts_after"]["rho"]
     rel = np.abs(grho - orho) / np.maximum(np.abs(orho), 1e-300)
-    assert rel.max() <= 1e-5, rel.max()
     exact = np.mean(grho == orho)
     print(f"{name}: density bit-exact fraction {exact:.6f}, max rel {rel.max():.3e}")
-    assert exact > 0.999
+    assert exact == 1.0, (exact, rel.max())
 
 
 @pytest.mark.parametrize("name", NAMES)
@@ -292,8 +292,7 @@ def test_overflow_paths(limit):
     assert st["density_overflow_waves"] > 0 and st["hop_overflow_waves"] > 0, st
     assert st["density_resumed_queries"] > 0 and st["hop_resumed_queries"] > 0, st
     orho = ot.densities()
-    rel = np.abs(rho - orho) / np.maximum(np.abs(orho), 1e-300)
-    assert rel.max() <= 1e-5 and np.mean(rho == orho) > 0.999
+    assert np.array_equal(rho, orho)
     assert np.array_equal(rho, g["parts_after"]["rho"])
     odn, opk = ot.hop()
     assert np.array_equal(dn, odn) and np.array_equal(pk, opk)
