@@ -96,7 +96,8 @@ int plan_make_ghost_particles(plan_ctx *ctx, uint32_t *T_out);
 int plan_build_tree(plan_ctx *ctx, int check_pos, plan_tree_info *info);
 
 /* Step 1 of FindPlanetesimals (src/tree.hpp:3664-3689): new_density of every
- * tree particle from its K neighbours.  kernel: 0 median spherical
+ * tree particle from its K neighbours (K in 1..64 here and in every KNN entry
+ * point below; H likewise).  kernel: 0 median spherical
  * (src/tree.hpp:3473-3491), 1 pseudo quadratic spline (src/tree.hpp:3427-3434),
  * -1 the reference's automatic choice (src/tree.hpp:3576-3612). */
 int plan_knn_density(plan_ctx *ctx, uint32_t K, int kernel);
@@ -184,8 +185,8 @@ int plan_knn_query(plan_ctx *ctx, uint32_t nq, const uint32_t *particle_idx, con
  * d2[nq x K], rstop[nq] (any output may be NULL).  It calls no hipMalloc or
  * hipFree: its working buffers live in a fixed device array of
  * PLAN_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends, so nq is
- * limited to PLAN_NOALLOC_MAX_QUERIES (larger is PLAN_E_ARG) and K to 8, 16,
- * 32 or 64.  The array is shared by every call on one GPU: calls that overlap
+ * limited to PLAN_NOALLOC_MAX_QUERIES (larger is PLAN_E_ARG) and K to 1..64.
+ * The array is shared by every call on one GPU: calls that overlap
  * on one GPU race.  Returns after the results are written (the context's
  * stream is synchronized). */
 #define PLAN_NOALLOC_MAX_QUERIES 16384u

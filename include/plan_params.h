@@ -9,8 +9,10 @@
  *   solid_to_gas_ratio       : key "Z"             (src/global.cpp:110-113)
  *   box_resolution           : keys "Nx1..Nx3"     (src/global.cpp:115-122)
  *   four_pi_G                : key "four_pi_G_par" (src/global.cpp:123-126)
- *   num_knn                  : keys "num_neighbors_in_knn_search", "num_knn", "knn" (src/global.cpp:156-170)
- *   num_hop / fixed_num_hop  : keys "num_neighbors_to_hop", "num_hop", "hop"          (src/global.cpp:171-188)
+ *   num_knn                  : keys "num_neighbors_in_knn_search", "num_knn", "knn" (src/global.cpp:156-170),
+ *                              1..64 (one wave's lanes hold a query's list; larger is PLAN_E_ARG)
+ *   num_hop / fixed_num_hop  : keys "num_neighbors_to_hop", "num_hop", "hop"          (src/global.cpp:171-188),
+ *                              1..64 likewise (the automatic choice is 64 at most, src/tree.hpp:3583-3589)
  *   num_peaks                : keys "num_peaks", "num_peak" (src/global.cpp:146-155)
  *   clump_diffuse_threshold  : key "clump_diffuse_threshold" (src/global.cpp:199-208), <0 = automatic
  *   Hill_fraction_for_merge  : key "Hill_fraction_for_merge" (src/global.cpp:210-219), <0 = automatic

@@ -439,9 +439,9 @@ static int set_common(plan_ctx *c, const plan_params *params, uint32_t N)
     if (params->clump_diffuse_threshold == 0)
         return fail(c, PLAN_E_ARG, "clump_diffuse_threshold must be larger than 0 (or negative for automatic)");
     if (!knn_k_supported(params->num_knn))
-        return fail(c, PLAN_E_ARG, "num_knn must be 8, 16, 32 or 64 (neighbour lists hold at most 64 per query)");
+        return fail(c, PLAN_E_ARG, "num_knn must be in 1..64 (neighbour lists hold at most 64 per query)");
     if (!knn_k_supported(params->num_hop))
-        return fail(c, PLAN_E_ARG, "num_hop must be 8, 16, 32 or 64 (neighbour lists hold at most 64 per query)");
+        return fail(c, PLAN_E_ARG, "num_hop must be in 1..64 (neighbour lists hold at most 64 per query)");
     c->pp = *params;
     derive_params(params, N, &c->d);
     c->ps.N = c->ps.T = N;
@@ -649,7 +649,7 @@ int plan_build_tree(plan_ctx *c, int check_pos, plan_tree_info *info)
 static int run_density(plan_ctx *c, uint32_t K, int kernel)
 {
     if (!c->have_tree) return fail(c, PLAN_E_STATE, "no tree built");
-    if (K == 0 || K > c->tree.n) return fail(c, PLAN_E_ARG, "K must be in 1..num_particles");
+    if (!knn_k_supported(K) || K > c->tree.n) return fail(c, PLAN_E_ARG, "K must be in 1..min(64, num_particles)");
     if (kernel < 0) {
         uint32_t H;
         int enough;
@@ -687,7 +687,7 @@ int plan_knn_density(plan_ctx *c, uint32_t K, int kernel)
 static int run_hop(plan_ctx *c, uint32_t H, double thr)
 {
     if (!c->have_density) return fail(c, PLAN_E_STATE, "densities not computed");
-    if (H == 0 || H > c->tree.n) return fail(c, PLAN_E_ARG, "H must be in 1..num_particles");
+    if (!knn_k_supported(H) || H > c->tree.n) return fail(c, PLAN_E_ARG, "H must be in 1..min(64, num_particles)");
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
     dev_hop(c->tree, c->work, c->d, thr, H, c->hop, c->stream);
     HIPCHK(hipEventRecord(c->ev[4], c->stream));
@@ -1056,7 +1056,7 @@ int plan_knn_query(plan_ctx *c, uint32_t nq, const uint32_t *pidx, const double 
     return guarded(c, [&]() -> int {
         if (!c->have_tree) return fail(c, PLAN_E_STATE, "no tree built");
         if (!pidx && (!pts || !est)) return fail(c, PLAN_E_ARG, "point queries need pts and est");
-        if (K == 0 || K > c->tree.n) return fail(c, PLAN_E_ARG, "K must be in 1..num_particles");
+        if (!knn_k_supported(K) || K > c->tree.n) return fail(c, PLAN_E_ARG, "K must be in 1..min(64, num_particles)");
         if (nq == 0) return PLAN_OK;
         hipStream_t s = c->stream;
         uint32_t *di = c->work.get_t<uint32_t>("kq_idx", (size_t)nq * K);
@@ -1152,8 +1152,7 @@ int plan_knn_query_noalloc(plan_ctx *c, uint32_t nq, const uint32_t *pidx, const
     return guarded(c, [&]() -> int {
         if (!c->have_tree) return fail(c, PLAN_E_STATE, "no tree built");
         if (!pidx && (!pts || !est)) return fail(c, PLAN_E_ARG, "point queries need pts and est");
-        if (K == 0 || K > c->tree.n) return fail(c, PLAN_E_ARG, "K must be in 1..num_particles");
-        if (K != 8 && K != 16 && K != 32 && K != 64) return fail(c, PLAN_E_ARG, "KNN: supported k are 8, 16, 32, 64");
+        if (!knn_k_supported(K) || K > c->tree.n) return fail(c, PLAN_E_ARG, "K must be in 1..min(64, num_particles)");
         if (nq > PLAN_NOALLOC_MAX_QUERIES)
             return fail(c, PLAN_E_ARG, "plan_knn_query_noalloc: at most PLAN_NOALLOC_MAX_QUERIES queries per call");
         if (nq == 0) return PLAN_OK;
@@ -1195,7 +1194,7 @@ int plan_density_at(plan_ctx *c, uint32_t nq, const double *pts, const double *e
     return guarded(c, [&]() -> int {
         if (!c->have_tree) return fail(c, PLAN_E_STATE, "no tree built");
         if (!pts || !est || !rho) return fail(c, PLAN_E_ARG, "NULL argument");
-        if (K == 0 || K > c->tree.n) return fail(c, PLAN_E_ARG, "K must be in 1..num_particles");
+        if (!knn_k_supported(K) || K > c->tree.n) return fail(c, PLAN_E_ARG, "K must be in 1..min(64, num_particles)");
         std::vector<double4> q(nq);
         for (uint32_t i = 0; i < nq; ++i) {
             if (!(est[i] > 0)) return fail(c, PLAN_E_ARG, "point queries need a positive start radius");

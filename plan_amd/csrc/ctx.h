@@ -116,6 +116,9 @@ struct TreeView {
     const uint32_t *leaf_of;
     uint32_t n;
     uint32_t nn;
+    /* neighbours per query (1..64); the kernels are built for 8, 16, 32 and
+     * 64 and keep the kq nearest of the next of those */
+    uint32_t kq = 0;
     double to_diag;
     double cell_len[3], half_cell[3], cell_sq[3], three_half_cell[3];
     double cell_volume;
@@ -157,8 +160,8 @@ struct KnnOut {
     uint32_t *nchk = nullptr;  /* host counter: densities checked against the C library's pow */
     uint32_t *npow = nullptr;  /* host counter: densities redone with the C library's pow */
 };
-/* neighbour counts the KNN kernels are built for */
-inline bool knn_k_supported(uint32_t K) { return K == 8 || K == 16 || K == 32 || K == 64; }
+/* neighbour counts the KNN kernels take (a wave's lanes hold one list) */
+inline bool knn_k_supported(uint32_t K) { return K >= 1 && K <= 64; }
 /* mode: 0 density, 1 hop (densest neighbour), 2 lists only.
  * K must satisfy knn_k_supported(K).
  * Queries are tree particles qidx[i] (or i when qidx is null) or explicit

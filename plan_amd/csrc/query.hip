@@ -218,9 +218,8 @@ struct NodeCache {
 };
 
 /* query position and the reference's start radius: hw of the first node on
- * the leaf-to-root path holding >= K/4 particles (src/tree.hpp:3206-3219),
+ * the leaf-to-root path holding >= kq/4 particles (src/tree.hpp:3206-3219),
  * or the caller's estimate */
-template <int K>
 __device__ __forceinline__ void query_setup(const TreeView &tv, uint32_t q, const uint32_t *qidx, const double4 *qpts,
                                             const double *est, double &qx, double &qy, double &qz, double &r,
                                             uint32_t &self, uint32_t q0 = 0)
@@ -242,7 +241,7 @@ __device__ __forceinline__ void query_setup(const TreeView &tv, uint32_t q, cons
         r = est[q];
     } else {
         uint32_t node = tv.leaf_of[self];
-        while (tv.ntop[node].end - tv.ntop[node].begin < (uint32_t)(K / 4)) node = tv.ntop[node].parent;
+        while (tv.ntop[node].end - tv.ntop[node].begin < tv.kq / 4u) node = tv.ntop[node].parent;
         double hw = tv.ngeo[node].hw;
         r = __dsqrt_rn(__dmul_rn(hw, hw));
     }
@@ -469,9 +468,8 @@ struct CountLds {
     uint32_t qcnt[64];
 };
 
-/* Pass 1: radius schedule r, 2r, 4r ... until |S(r)| >= K (node sizes only,
- * no heap; K <= n guarantees an end, 2200 doublings exceed any finite start). */
-template <int K>
+/* Pass 1: radius schedule r, 2r, 4r ... until |S(r)| >= kq (node sizes only,
+ * no heap; kq <= n guarantees an end, 2200 doublings exceed any finite start). */
 __global__ __launch_bounds__(64) void k_knn_radius(TreeView tv, uint32_t nq, const uint32_t *__restrict__ qidx,
                                                    const double4 *__restrict__ qpts, const double *__restrict__ est,
                                                    const double *__restrict__ rt, double *__restrict__ r_out,
@@ -483,20 +481,20 @@ __global__ __launch_bounds__(64) void k_knn_radius(TreeView tv, uint32_t nq, con
     const bool valid = (uint32_t)lane < qpw && q < nq;
     double qx = 0, qy = 0, qz = 0, r = 0;
     uint32_t self = 0;
-    if (valid) query_setup<K>(tv, q, qidx, qpts, est, qx, qy, qz, r, self);
-    /* a point query resolved by k_point_tight: S(est) holds >= K particles,
+    if (valid) query_setup(tv, q, qidx, qpts, est, qx, qy, qz, r, self);
+    /* a point query resolved by k_point_tight: S(est) holds >= kq particles,
      * so the schedule stops at est */
     const bool tight = valid && rt && rt[q] > 0;
     bool done = !valid || tight;
     int rounds = 0;
-    uint32_t inside = tight ? (uint32_t)K : 0u;
+    uint32_t inside = tight ? tv.kq : 0u;
     while (__ballot(!done) && rounds++ < 2200) {
         uint32_t cnt = 0, cin = 0, nent = 0;
         uint32_t sp = walk_dense_start(L.dw, __ballot(!done));
         walk_dense_run<0, WALK_STACK_FULL, WALK_COUNT>(tv, L.nc, L.dw, L.cont, nullptr, nent, qx, qy, qz, r, sp, cnt,
                                                       cin);
         if (!done) {
-            if (cnt >= (uint32_t)K) {
+            if (cnt >= tv.kq) {
                 done = true;
                 inside = cin;
             } else {
@@ -511,16 +509,15 @@ __global__ __launch_bounds__(64) void k_knn_radius(TreeView tv, uint32_t nq, con
 }
 
 /* Point queries (saddle points, user points) with a start radius est: when
- * some radius r_t <= est - margin already holds >= K particles within
+ * some radius r_t <= est - margin already holds >= kq particles within
  * distance r_t, every particle within r_t belongs to the reference's
- * candidate set S(est) (src/tree.hpp:3160-3230), so the K smallest
- * (d^2, index) of S(est) are the K nearest overall and can be gathered from
+ * candidate set S(est) (src/tree.hpp:3160-3230), so the kq smallest
+ * (d^2, index) of S(est) are the kq nearest overall and can be gathered from
  * the ball of radius r_t instead of all of S(est).  r_t starts at the half
- * width of the deepest node holding q with >= K/4 particles and doubles.
+ * width of the deepest node holding q with >= max(kq/4, 1) particles and doubles.
  * rt_out[q] = r_t, or 0 when no such radius exists (the query then takes
  * the candidate-set path).  The counts are exact (boundary leaves particle
  * by particle) and come from the group-wise walk with the ball tests. */
-template <int K>
 __global__ __launch_bounds__(64) void k_point_tight(TreeView tv, uint32_t nq, const double4 *__restrict__ qpts,
                                                     const double *__restrict__ est, double *__restrict__ rt_out,
                                                     uint32_t qpw)
@@ -552,7 +549,7 @@ __global__ __launch_bounds__(64) void k_point_tight(TreeView tv, uint32_t nq, co
             }
             if (next == PLAN_NONE) break;
             const NodeTop t = tv.ntop[next];
-            if (t.end - t.begin < (uint32_t)(K / 4 > 0 ? K / 4 : 1)) break;
+            if (t.end - t.begin < max(tv.kq / 4u, 1u)) break;
             node = next;
         }
         r = tv.ngeo[node].hw;
@@ -569,7 +566,7 @@ __global__ __launch_bounds__(64) void k_point_tight(TreeView tv, uint32_t nq, co
         __syncthreads();
         const uint32_t cnt = L.qcnt[lane];
         if (!done) {
-            if (cnt >= (uint32_t)K) {
+            if (cnt >= tv.kq) {
                 done = true;
                 ok = true;
             } else {
@@ -789,9 +786,9 @@ __device__ __forceinline__ void select_list(const TreeView &tv, LDS &L, uint32_t
             }
         }
         if (!first) {
-            /* only keys below the current K-th can enter the K best */
-            const double kd = lane_f64(bd, K - 1);
-            const uint32_t ki = lane_u32(bi, K - 1);
+            /* only keys below the current kq-th can enter the kq best */
+            const double kd = lane_f64(bd, (int)tv.kq - 1);
+            const uint32_t ki = lane_u32(bi, (int)tv.kq - 1);
             if (id != PLAN_NONE && !key_less(d, id, kd, ki)) {
                 d = inf;
                 id = PLAN_NONE;
@@ -809,7 +806,7 @@ __device__ __forceinline__ void select_list(const TreeView &tv, LDS &L, uint32_t
     }
 }
 
-/* What a query's sorted neighbour list (lane l holds the l-th nearest, l < K)
+/* What a query's sorted neighbour list (lane l holds the l-th nearest, l < kq)
  * is used for: the densest neighbour (HOP), the density (median spherical or
  * QS kernel) and/or the lists themselves.  Sums run in the reference's
  * order (src/tree.hpp:3320-3420). */
@@ -820,7 +817,8 @@ __device__ __forceinline__ void finish_query(const TreeView &tv, double bd, uint
                                              uint32_t *__restrict__ idx_out, double *__restrict__ d2_out)
 {
     const int lane = threadIdx.x & 63;
-    const bool mine = lane < K;
+    const int kq = (int)tv.kq;
+    const bool mine = lane < kq;
     if (mode == 1) {
         /* maximum of (rho, -index) over self and the neighbours */
         double v = -__longlong_as_double(0x7ff0000000000000ll);
@@ -848,18 +846,18 @@ __device__ __forceinline__ void finish_query(const TreeView &tv, double bd, uint
     }
     if (mode == 2 || idx_out) {
         if (mine) {
-            if (idx_out) idx_out[(size_t)q * K + lane] = bi;
-            if (d2_out) d2_out[(size_t)q * K + lane] = bd;
+            if (idx_out) idx_out[(size_t)q * kq + lane] = bi;
+            if (d2_out) d2_out[(size_t)q * kq + lane] = bd;
         }
         if (mode == 2) return;
     }
     double rho = 0;
     if (kernel == 0) {
-        /* MedianSphericalDensityKernel: m = 4 d2[K/2]; i* = last index with
+        /* MedianSphericalDensityKernel: m = 4 d2[kq/2]; i* = last index with
          * d2 <= m; masses summed from i* down to 0 */
-        const double m = __dmul_rn(4.0, lane_f64(bd, K / 2));
-        const uint64_t in = __ballot(lane > K / 2 && lane < K && !(bd > m));
-        const int is = in ? 63 - __clzll((long long)in) : K / 2;
+        const double m = __dmul_rn(4.0, lane_f64(bd, kq / 2));
+        const uint64_t in = __ballot(lane > kq / 2 && lane < kq && !(bd > m));
+        const int is = in ? 63 - __clzll((long long)in) : kq / 2;
         const double tm = lane_f64(bd, is);
         const double w = (lane <= is) ? tv.pm[bi].w : 0.0;
         if (tv.msum && !__ballot(lane <= is && w != tv.m0)) {
@@ -867,7 +865,7 @@ __device__ __forceinline__ void finish_query(const TreeView &tv, double bd, uint
             rho = tv.msum[is + 1];
         } else {
             /* masses from i* down to 0; the lanes above i* add +0 to the
-             * leading zero, which changes nothing */
+             * leading zero, which changes nothing (K >= kq) */
 #pragma unroll
             for (int j = K - 1; j >= 0; --j) rho = __dadd_rn(rho, lane_f64(w, j));
         }
@@ -904,12 +902,14 @@ __device__ __forceinline__ void finish_query(const TreeView &tv, double bd, uint
             }
             term = __dmul_rn(__ddiv_rn(x.w, tv.cell_volume), w);
         }
+        /* lanes from kq on add +0 after the sum, which changes nothing */
         for (int j = 0; j < K; ++j) rho = __dadd_rn(rho, lane_f64(term, j));
     }
     if (lane == 0) rho_out[byq ? q : self] = rho;
 }
 
-/* The K nearest of S(r*) per query, and their use, in one kernel.
+/* The kq nearest of S(r*) per query, and their use, in one kernel (K, the
+ * list length the kernel is built for, is kq rounded up to 8, 16, 32 or 64).
  * A wave walks the tree once in lock step for its 64 queries (lane =
  * query), recording the nodes some lane takes whole; then it selects for one
  * query at a time, all 64 lanes on that query's candidates.
@@ -946,16 +946,16 @@ __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, con
     const bool valid = (uint32_t)lane < qpw && q < nq;
     double qx = 0, qy = 0, qz = 0, r = 0;
     uint32_t self = 0;
-    if (valid) query_setup<K>(tv, q, qidx, qpts, FUSE ? nullptr : r_io, qx, qy, qz, r, self, q0);
+    if (valid) query_setup(tv, q, qidx, qpts, FUSE ? nullptr : r_io, qx, qy, qz, r, self, q0);
     const double inf = __longlong_as_double(0x7ff0000000000000ll);
     const uint32_t nvalid = (uint32_t)__popcll(__ballot(valid));
     uint32_t cin = 0;
     uint32_t nent = 0;
     bool filled = false;
     if constexpr (FUSE && !RESUME) {
-        /* the radius schedule r0, 2 r0, ... until |S(r)| >= K
+        /* the radius schedule r0, 2 r0, ... until |S(r)| >= kq
          * (src/tree.hpp:3206-3223): each round walks every query at its
-         * current radius; the queries short of K candidates double theirs.
+         * current radius; the queries short of kq candidates double theirs.
          * The list of the last round is the one selected from. */
         const uint64_t V = __ballot(valid);
         for (int round = 0; round < 2200; ++round) {
@@ -966,7 +966,7 @@ __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, con
             }
             __syncthreads();
             const uint32_t sc = list_count(L.lst, 0u, nent);
-            const bool need = valid && sc < (uint32_t)K;
+            const bool need = valid && sc < tv.kq;
             if (!__ballot(need)) break;
             if (need) r = __dmul_rn(r, 2.0);
         }
@@ -977,7 +977,7 @@ __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, con
         }
     } else if constexpr (FUSE) {
         /* waves that walk in several fills: the radius schedule r0, 2 r0, ...
-         * until |S(r)| >= K (src/tree.hpp:3206-3223) by counting walks, which
+         * until |S(r)| >= kq (src/tree.hpp:3206-3223) by counting walks, which
          * also sum the particles of the nodes each sphere contains */
         const uint64_t V = __ballot(valid);
         for (int round = 0; round < 2200; ++round) {
@@ -985,7 +985,7 @@ __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, con
             uint32_t sp = walk_dense_start(L.dw, V);
             walk_dense_run<CAP, WALK_STACK_FULL, WALK_COUNT>(tv, L.nc, L.dw, L.cont, L.lst, nent, qx, qy, qz, r, sp,
                                                              sc, cc);
-            const bool need = valid && sc < (uint32_t)K;
+            const bool need = valid && sc < tv.kq;
             if (!need) cin = cc;
             if (!__ballot(need)) break;
             __syncthreads();
@@ -1000,12 +1000,12 @@ __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, con
     }
     /* Candidates in nodes the sphere contains lie within r* (a contained
      * node's box is inside the sphere with a margin far above the Morton
-     * quantisation); when there are >= K of them the K nearest are within
+     * quantisation); when there are >= kq of them the kq nearest are within
      * r*, and only d^2 <= (r* + 1e-9)^2 can enter the list.  A point query
-     * with a tight radius (k_point_tight) has >= K particles within r_t and
-     * takes the K nearest within r_t. */
+     * with a tight radius (k_point_tight) has >= kq particles within r_t and
+     * takes the kq nearest within r_t. */
     double T = inf;
-    if (valid && cin >= (uint32_t)K) {
+    if (valid && cin >= tv.kq) {
         const double rr = __dadd_rn(r, 1e-9);
         T = __dmul_rn(rr, rr);
     }
@@ -1207,10 +1207,10 @@ static void launch_knn(const DevTree &t, const TreeView &tv, uint32_t nq, const 
         double *rt = nullptr;
         if (qpts) {
             rt = a.get_t<double>("knn_rt", (size_t)nq);
-            hipLaunchKernelGGL(k_point_tight<K>, dim3(ceil_div(nq, qpw)), dim3(64), 0, s, tv, nq, qpts, est, rt, qpw);
+            hipLaunchKernelGGL(k_point_tight, dim3(ceil_div(nq, qpw)), dim3(64), 0, s, tv, nq, qpts, est, rt, qpw);
             LAUNCHCHK();
         }
-        hipLaunchKernelGGL(k_knn_radius<K>, dim3(ceil_div(nq, qpw)), dim3(64), 0, s, tv, nq, qidx, qpts, est, rt,
+        hipLaunchKernelGGL(k_knn_radius, dim3(ceil_div(nq, qpw)), dim3(64), 0, s, tv, nq, qidx, qpts, est, rt,
                            rbuf, cbuf, qpw);
         LAUNCHCHK();
         launch_select<K, false>(tv, a, nq, qidx, qpts, rbuf, cbuf, rt, q0, mode, kernel, rho_in, out.rho,
@@ -1224,9 +1224,11 @@ void dev_knn(const DevTree &t, uint32_t K, int mode, int kernel, uint32_t nq, co
              hipStream_t s, Arena *scratch)
 {
     if (nq == 0) return;
+    if (!knn_k_supported(K)) throw std::runtime_error("KNN: k must be in 1..64");
     if (K > t.n) throw std::runtime_error("KNN: k exceeds the number of tree particles");
     if (qpts && !est) throw std::runtime_error("KNN: point queries need a start radius");
     TreeView tv = tree_view(t, d);
+    tv.kq = K;
     if (mode == 0 && kernel == 0) {
         /* sums of equal masses for the median kernel (species 0's mass) */
         Arena &a = scratch ? *scratch : const_cast<DevTree &>(t).arena;
@@ -1246,13 +1248,10 @@ void dev_knn(const DevTree &t, uint32_t K, int mode, int kernel, uint32_t nq, co
             HIPCHK(hipMemsetAsync(tv.pw_count, 0, 4, s));
         }
     }
-    switch (K) {
-    case 8: launch_knn<8>(t, tv, nq, qidx, qpts, est, mode, kernel, rho_in, out, s, scratch); break;
-    case 16: launch_knn<16>(t, tv, nq, qidx, qpts, est, mode, kernel, rho_in, out, s, scratch); break;
-    case 32: launch_knn<32>(t, tv, nq, qidx, qpts, est, mode, kernel, rho_in, out, s, scratch); break;
-    case 64: launch_knn<64>(t, tv, nq, qidx, qpts, est, mode, kernel, rho_in, out, s, scratch); break;
-    default: throw std::runtime_error("KNN: supported k are 8, 16, 32, 64");
-    }
+    if (K <= 8) launch_knn<8>(t, tv, nq, qidx, qpts, est, mode, kernel, rho_in, out, s, scratch);
+    else if (K <= 16) launch_knn<16>(t, tv, nq, qidx, qpts, est, mode, kernel, rho_in, out, s, scratch);
+    else if (K <= 32) launch_knn<32>(t, tv, nq, qidx, qpts, est, mode, kernel, rho_in, out, s, scratch);
+    else launch_knn<64>(t, tv, nq, qidx, qpts, est, mode, kernel, rho_in, out, s, scratch);
     if (tv.pw_count) {
         /* the reference's pow(x, 1.5) (src/tree.hpp:3489) from the C library
          * for the listed x; the few that differ from the correctly rounded

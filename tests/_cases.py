@@ -57,7 +57,7 @@ def c5_small_case():
     return snap
 
 
-def lattice_case():
+def lattice_case(knn=32):
     """51^3 particles on the cubic lattice k 2^-8, |k| <= 25: every pairwise
     difference is exact, so squared distances tie exactly and most K-th
     neighbours are tied (the index tie-break decides)."""
@@ -65,7 +65,7 @@ def lattice_case():
     k = np.arange(-25, 26, dtype=np.float64) * 2.0 ** -8
     g = np.stack(np.meshgrid(k, k, k, indexing="ij"), -1).reshape(-1, 3)
     n = g.shape[0]
-    params = synth.default_params(n, nx=64, knn=32)
+    params = synth.default_params(n, nx=64, knn=knn)
     vel = np.zeros((n, 3), np.float32)
     vel[:, 1] = (1.5 * g[:, 0]).astype(np.float32)
     dens = np.full(n, params["Z"] * np.sqrt(2 * np.pi) / 0.2, np.float32)
@@ -114,6 +114,11 @@ CASES = {
     "n32k_k16": lambda: dense_case(32768, 10, 5, a=4e-4, knn=16),
     "n32k_k64_h64": lambda: dense_case(32768, 10, 6, a=4e-4, knn=64, hop=64),
     "n16k_qs": lambda: dense_case(16384, 6, 7, a=3e-4, nx=128),
+    # neighbour counts other than 8, 16, 32, 64 (the kernels keep the kq
+    # nearest of the next of those): K = 48 with a fixed H = 24, and K = 3
+    # (start node of one leaf, kq / 4 = 0) with H = 5
+    "n32k_k48_h24": lambda: dense_case(32768, 10, 14, a=4e-4, knn=48, hop=24),
+    "n8k_k3_h5": lambda: dense_case(8192, 6, 15, a=3e-4, knn=3, hop=5),
     # the benchmark's mix (200 Plummer clumps in 10% of the particles) at 2^20
     "c2_1m": lambda: synth.config_c2(seed=3, n=1 << 20),
     # the reference's CPU-runnable configuration: 262,144 particles, 16
@@ -125,4 +130,5 @@ CASES = {
     "key0": key0_case,
     "c5_4m": c5_small_case,
     "lattice": lattice_case,
+    "lattice_k20": lambda: lattice_case(knn=20),
 }

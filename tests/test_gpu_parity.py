@@ -114,13 +114,14 @@ def test_knn_sets(name):
     assert np.array_equal(gk["d2"][~tied], ok["d2"][~tied])
 
 
-def test_knn_ties_broken_by_index():
+@pytest.mark.parametrize("name", ["lattice", "lattice_k20"])
+def test_knn_ties_broken_by_index(name):
     """On the lattice, where squared distances tie exactly, the neighbour set
     of every query is the K smallest (d^2, tree index) pairs of the
     reference's candidate set S(r*) (the CPU restatement's, pinned against
     brute force in test_oracle_knn.py); the reference itself leaves ties at
     the K-th distance to its heap's order."""
-    snap, ot, g = run_case("lattice")
+    snap, ot, g = run_case(name)
     ctx = g["ctx"]
     K = snap.params["knn"]
     pos = ot.particles()["pos"]
@@ -135,7 +136,7 @@ def test_knn_ties_broken_by_index():
         tied += int(np.sum(cd == kth) > np.sum(cd[order] == kth))
         assert np.array_equal(np.sort(gk["idx"][j]), np.sort(ci[order])), int(i)
         assert np.array_equal(np.sort(gk["d2"][j]), cd[order])
-    print(f"lattice: {len(q)} queries, {tied} with the K-th distance tied beyond the list")
+    print(f"{name}: {len(q)} queries, {tied} with the K-th distance tied beyond the list")
     assert tied > 100
 
 
@@ -271,15 +272,16 @@ def test_catalog_files(name, tmp_path):
     print(f"{name}: {len(cat['clumps'])} clumps, sinks in clumps: {int(sinks.sum())}")
 
 
-@pytest.mark.parametrize("limit", [32, 8])
-def test_overflow_paths(limit):
+@pytest.mark.parametrize("name,limit", [("n65k", 32), ("n65k", 8), ("n32k_k48_h24", 8)])
+def test_overflow_paths(name, limit):
     """Waves whose include list overflows are redone as 16-query waves and,
     if still too long, walked in several fills; ball searches whose node
     frontier outgrows its buffers are redone with buffers sized from the
     counts.  Capping both (a host test hook) sends the density,
     densest-neighbour, saddle-point and ball searches of a small case down
-    those paths; every result must stay as above."""
-    snap, ot, g = run_case("n65k")
+    those paths; every result must stay as above (K = 48 keeps its 48 best
+    of 64-entry lists between fills)."""
+    snap, ot, g = run_case(name)
     ctx = g["ctx"]
     ctx.set_knn_list_limit(limit)
     try:
@@ -299,3 +301,24 @@ def test_overflow_paths(limit):
     oc = ot.catalog()
     assert [c["peak_index"] for c in cat["clumps"]] == [c["peak_index"] for c in oc["clumps"]]
     assert np.array_equal(cat["member_original_ids"], oc["member_original_ids"])
+
+
+def test_neighbour_counts_outside_1_to_64_rejected():
+    """num_knn and num_hop above 64 (or 0) are refused when the particles are
+    set, and every KNN entry point refuses such a K, before any device work."""
+    import plan_amd
+
+    snap = CASES["n1k"]()
+    ctx = plan_amd.Context(0)
+    for key, val in (("knn", 65), ("knn", 0), ("hop", 128), ("hop", 0)):
+        params = dict(snap.params, **{key: val})
+        with pytest.raises(plan_amd.PlanError, match="must be in 1..64"):
+            ctx.set_particles(params, snap.pos, snap.vel, snap.density, snap.property_index, snap.id)
+    ctx.set_particles(snap.params, snap.pos, snap.vel, snap.density, snap.property_index, snap.id)
+    ctx.make_ghosts()
+    ctx.build_tree(True)
+    q = np.arange(10, dtype=np.uint32)
+    with pytest.raises(plan_amd.PlanError, match="1..min"):
+        ctx.knn(65, particle_idx=q)
+    with pytest.raises(plan_amd.PlanError, match="1..min"):
+        ctx.knn_density(65, 0)
