@@ -43,7 +43,9 @@ typedef struct plan_tree_info {
     uint32_t num_leaf_nodes;
     uint32_t num_sink_particles;
     uint32_t depth;           /* deepest node level (root = 1) */
-    uint32_t reserved;
+    uint32_t hop_auto;        /* 1: FindPlanetesimals raises H to 64 for this snapshot by the
+                                 automatic choice (src/tree.hpp:3583-3586), which the reference
+                                 keeps in its parameters for the snapshots after */
     double   root_center[3];
     double   half_width;
     int32_t  level_count[32];
@@ -247,6 +249,13 @@ int plan_last_density_radius_ms(plan_ctx *ctx, double *ms);
  * waves, resumable multi-fill walks, frontier regrowth).  Results do not
  * change, only speed. */
 int plan_set_knn_list_limit(plan_ctx *ctx, uint32_t max_entries);
+/* num_hop for the next plan_find_planetesimals of the loaded snapshot (1..64).
+ * The reference keeps H in its global parameters and its automatic choice
+ * overwrites it (src/tree.hpp:3583-3586), so in a series the H a snapshot
+ * starts from is the previous one's: the params' num_hop, or 64 once an
+ * earlier snapshot's plan_tree_info.hop_auto was 1.  No reference function:
+ * it replaces that carried parameter. */
+int plan_set_num_hop(plan_ctx *ctx, uint32_t num_hop);
 
 /* ---- pairwise-potential boundness (extension: no reference counterpart) ----
  *

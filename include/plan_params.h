@@ -45,7 +45,14 @@ typedef struct plan_params {
     int32_t  fixed_num_hop;
     uint32_t num_peaks;
     int32_t  no_ghost;
-    uint32_t reserved;
+    /* 0: this snapshot starts a series, and box, species weights and particle
+     * masses come from it.  Otherwise the particle count of the series' first
+     * snapshot, whose box and species weights these params carry: the
+     * reference derives them at its first loop only (ReadLisFile at
+     * loop_begin, src/tree.hpp:1563-1581), so later snapshots keep the first
+     * one's particle masses (M_tot / N_first) and plan_read_lis keeps these
+     * params' box and weights instead of the file header's. */
+    uint32_t series_first_n;
 } plan_params;
 
 /* Values of the reference's NumericalParameters after ReadLisFile

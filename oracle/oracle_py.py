@@ -325,6 +325,36 @@ def run_pipeline(snap, params=None, threads: int | None = None):
     return t
 
 
+def series_params(first: dict, first_n: int, params: dict, hop: int) -> dict:
+    """Parameters of a later snapshot of a series run in one process: box and
+    species weights of the first snapshot, particle masses from its count
+    (ReadLisFile derives them at loop_begin only, src/tree.hpp:1563-1581),
+    and H as the previous FindPlanetesimals left it in the parameters
+    (src/tree.hpp:3583-3586)."""
+    p = dict(params)
+    p.update(box_min=list(first["box_min"]), box_max=list(first["box_max"]),
+             type_info=list(first.get("type_info", [1.0])), series_first_n=int(first_n), hop=int(hop))
+    return p
+
+
+def run_series(snaps, params=None, threads: int | None = None):
+    """The reference's loop over snapshots in one process (src/main.cpp:56-79):
+    each snapshot through ghosts, BuildTree and FindPlanetesimals, with the
+    parameters the earlier snapshots left behind.  Returns the trees."""
+    trees = []
+    first = first_n = hop = None
+    for s in snaps:
+        p = params or s.params
+        if first is None:
+            first, first_n = p, s.n
+        else:
+            p = series_params(first, first_n, p, hop)
+        t = run_pipeline(s, p, threads)
+        hop = t.stage_info()["hop"]
+        trees.append(t)
+    return trees
+
+
 def boundness(offsets, pos, vel, mass, G, Omega=1.0, q=1.5, max_iter=64):
     """FP64 brute-force pairwise boundness of clumps in CSR form; returns
     (bound flags, per-clump dict arrays)."""

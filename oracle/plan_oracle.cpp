@@ -88,7 +88,9 @@ static void derive(const plan_params *p, uint32_t N, plan_derived *d)
     for (uint32_t i = 0; i < nt; ++i) {
         frac[i] /= tmp_sum;
         d->mass_total_code_units = ((p->solid_to_gas_ratio * d->box_length[0]) * d->box_length[1]) * std::sqrt(2. * kPI);
-        d->mass_per_particle[i] = (frac[i] * d->mass_total_code_units) / (double)N;
+        /* a later snapshot of a series keeps the first one's masses: the
+         * reference derives them at loop_begin only (src/tree.hpp:1563-1581) */
+        d->mass_per_particle[i] = (frac[i] * d->mass_total_code_units) / (double)(p->series_first_n ? p->series_first_n : N);
     }
 }
 

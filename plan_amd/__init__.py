@@ -39,7 +39,7 @@ vp = C.c_void_p
 
 class TreeInfo(C.Structure):
     _fields_ = [("num_particles", C.c_uint32), ("num_nodes", C.c_uint32), ("num_leaf_nodes", C.c_uint32),
-                ("num_sink_particles", C.c_uint32), ("depth", C.c_uint32), ("reserved", C.c_uint32),
+                ("num_sink_particles", C.c_uint32), ("depth", C.c_uint32), ("hop_auto", C.c_uint32),
                 ("root_center", C.c_double * 3), ("half_width", C.c_double), ("level_count", C.c_int32 * 32)]
 
 
@@ -52,7 +52,7 @@ EXPORTS = [
     "plan_last_density_radius_ms", "plan_boundness_load", "plan_boundness_load_catalog", "plan_boundness_run",
     "plan_boundness_result", "plan_set_knn_list_limit", "plan_read_lis", "plan_set_particles_lis",
     "plan_get_particle_set", "plan_write_peaks", "plan_write_clump_particles", "plan_write_basic_results",
-    "plan_density_vs_scale", "plan_knn_query_noalloc",
+    "plan_density_vs_scale", "plan_knn_query_noalloc", "plan_set_num_hop",
 ]
 
 
@@ -100,6 +100,7 @@ def lib():
     L.plan_boundness_run.argtypes = [vp, C.c_int, C.c_uint32, C.POINTER(PlanBoundnessStats)]
     L.plan_boundness_result.argtypes = [vp, vp, C.POINTER(PlanBoundnessClump)]
     L.plan_set_knn_list_limit.argtypes = [vp, C.c_uint32]
+    L.plan_set_num_hop.argtypes = [vp, C.c_uint32]
     L.plan_read_lis.argtypes = [vp, C.POINTER(PlanParams), C.POINTER(C.c_char_p), C.c_uint32,
                                 C.POINTER(C.c_double), C.POINTER(C.c_uint32)]
     L.plan_set_particles_lis.argtypes = [vp, C.POINTER(PlanParams), C.c_uint32, vp]
@@ -429,6 +430,11 @@ class Context:
         v = C.c_uint64()
         self._chk(lib().plan_device_memory(self._h, C.byref(v)), "plan_device_memory")
         return v.value
+
+    def set_num_hop(self, num_hop: int):
+        """H the next find_planetesimals starts from (a series carries the
+        previous snapshot's, as the reference's parameters do)."""
+        self._chk(lib().plan_set_num_hop(self._h, int(num_hop)), "plan_set_num_hop")
 
     def set_knn_list_limit(self, max_entries: int):
         """Test hook: cap the per-wave KNN include list (0 = full capacity)

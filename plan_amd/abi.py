@@ -23,7 +23,7 @@ class PlanParams(C.Structure):
         ("fixed_num_hop", C.c_int32),
         ("num_peaks", C.c_uint32),
         ("no_ghost", C.c_int32),
-        ("reserved", C.c_uint32),
+        ("series_first_n", C.c_uint32),
     ]
 
 
@@ -102,6 +102,7 @@ def make_params(d: dict) -> PlanParams:
     p.fixed_num_hop = int(d.get("fixed_hop", 0))
     p.num_peaks = int(d.get("num_peaks", 0))
     p.no_ghost = int(d.get("no_ghost", 0))
+    p.series_first_n = int(d.get("series_first_n", 0))
     return p
 
 

@@ -95,7 +95,7 @@ void derive_params(const plan_params *p, uint32_t N, plan_derived *d)
     for (uint32_t i = 0; i < nt; ++i) {
         frac[i] /= sum;
         d->mass_total_code_units = ((p->solid_to_gas_ratio * d->box_length[0]) * d->box_length[1]) * std::sqrt(2. * PI);
-        d->mass_per_particle[i] = (frac[i] * d->mass_total_code_units) / (double)N;
+        d->mass_per_particle[i] = (frac[i] * d->mass_total_code_units) / (double)(p->series_first_n ? p->series_first_n : N);
     }
 }
 
@@ -537,7 +537,8 @@ int plan_read_lis(plan_ctx *c, plan_params *params, const char *const *paths, ui
         std::string e = lis_read_headers(paths, num_paths, h, counts, io);
         if (!e.empty()) return fail(c, io ? PLAN_E_IO : PLAN_E_ARG, e);
         plan_params p = *params;
-        params_from_lis(h, p);
+        if (p.series_first_n) p.time = h.time; /* a later snapshot of a series: box and weights as given */
+        else params_from_lis(h, p);
         const uint32_t N = h.num_particles;
         int r = set_common(c, &p, N);
         if (r) return r;
@@ -641,6 +642,10 @@ int plan_build_tree(plan_ctx *c, int check_pos, plan_tree_info *info)
             for (int k = 0; k < 3; ++k) info->root_center[k] = c->tree.root_center[k];
             info->half_width = c->tree.half_width;
             for (int l = 0; l < 32; ++l) info->level_count[l] = c->tree.level_count[l];
+            int kernel, enough;
+            uint32_t H;
+            choose_kernel(c, c->ps.N, kernel, H, enough);
+            info->hop_auto = (enough && !c->pp.fixed_num_hop && c->ps.N >= 1.67e7) ? 1u : 0u;
         }
         return PLAN_OK;
     });
@@ -1317,6 +1322,15 @@ int plan_last_density_kernel_ms(plan_ctx *c, double *ms)
 {
     if (!c || !ms) return PLAN_E_ARG;
     ms[0] = c->density_kernel_ms;
+    return PLAN_OK;
+}
+
+int plan_set_num_hop(plan_ctx *c, uint32_t num_hop)
+{
+    if (!c) return PLAN_E_ARG;
+    if (!c->have_particles) return fail(c, PLAN_E_STATE, "no particle set loaded");
+    if (!knn_k_supported(num_hop)) return fail(c, PLAN_E_ARG, "num_hop must be in 1..64");
+    c->pp.num_hop = num_hop;
     return PLAN_OK;
 }
 

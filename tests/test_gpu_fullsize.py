@@ -91,3 +91,41 @@ def test_whole_snapshot_against_the_oracle(run):
         for k in ("center_of_mass", "vel_com", "J", "J_quarter_Hill"):
             assert tuple(a[k]) == tuple(b[k]), k
     assert cat["Mp_tot"] == oc["Mp_tot"]
+
+
+def test_series_carries_the_raised_hop(run):
+    """A series of this snapshot and the same one after 100,000 particles left
+    (16,677,216 < 1.67e7, so its own run keeps H = 32): the first snapshot's
+    automatic choice raised H to 64, the reference keeps that in its
+    parameters (src/tree.hpp:3583-3586), so the second runs with H = 64 and
+    the first snapshot's particle masses; its catalog equals a single
+    context given those parameters."""
+    import copy
+
+    import plan_amd
+    from plan_amd.pipeline import find_clumps_series
+
+    snap, ctx, T, info, cat = run
+    assert info.hop_auto == 1
+    keep = snap.n - 100_000
+    s2 = copy.copy(snap)
+    s2.pos, s2.vel, s2.density = snap.pos[:keep], snap.vel[:keep], snap.density[:keep]
+    s2.property_index, s2.id = snap.property_index[:keep], snap.id[:keep]
+    out = find_clumps_series([snap, s2], contexts=2)
+    assert out[0][1]["Mp_tot"] == cat["Mp_tot"]
+    p2, cat2 = out[1]
+    assert p2["hop"] == 64 and p2["series_first_n"] == snap.n
+    one = plan_amd.Context(0)
+    try:
+        one.set_particles(dict(snap.params, series_first_n=snap.n), s2.pos, s2.vel, s2.density,
+                          s2.property_index, s2.id)
+        one.make_ghosts()
+        assert one.build_tree().hop_auto == 0
+        one.set_num_hop(64)
+        want = one.find_planetesimals()
+    finally:
+        one.close()
+    print(f"[fullsize] series: second snapshot {keep} particles, {len(cat2['clumps'])} clumps", flush=True)
+    assert [c["peak_index"] for c in cat2["clumps"]] == [c["peak_index"] for c in want["clumps"]]
+    assert np.array_equal(cat2["member_original_ids"], want["member_original_ids"])
+    assert cat2["Mp_tot"] == want["Mp_tot"]

@@ -30,6 +30,21 @@ def test_derived_parameters_closed_form():
     assert d.max_ghost_zone_width == 0.025
 
 
+def test_series_masses_from_the_first_snapshot():
+    """A later snapshot of a series keeps the first one's particle masses,
+    M_tot * weight / N_first (ReadLisFile sets them at loop_begin only,
+    src/tree.hpp:1563-1581), whatever its own count."""
+    p = synth.default_params(1000)
+    p["type_info"] = [1.0, 3.0]
+    first = O.derive(p, 1000)
+    later = O.derive(dict(p, series_first_n=1000), 900)
+    alone = O.derive(p, 900)
+    M = first.mass_total_code_units
+    assert later.mass_per_particle[0] == first.mass_per_particle[0] == (0.25 * M) / 1000
+    assert later.mass_per_particle[1] == (0.75 * M) / 1000
+    assert alone.mass_per_particle[0] == (0.25 * M) / 900
+
+
 def _ghost_case(seed=0, n=3000, time=0.37):
     rng = np.random.default_rng(seed)
     p = synth.default_params(n, time=time)
