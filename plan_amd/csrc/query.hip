@@ -643,12 +643,6 @@ template <int J> __device__ __forceinline__ uint32_t xor_lane(uint32_t v)
     else if constexpr (J == 16) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x401F);
     else return (uint32_t)__shfl_xor((int)v, J, 64);
 }
-template <int J> __device__ __forceinline__ double xor_lane_f64(double d)
-{
-    const uint64_t b = (uint64_t)__double_as_longlong(d);
-    const uint32_t lo = xor_lane<J>((uint32_t)b), hi = xor_lane<J>((uint32_t)(b >> 32));
-    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
-}
 /* lanes l whose compare-exchange keeps the smaller key: for a bitonic stage
  * (k, j) those with ((l & k) == 0) == ((l & j) == 0); for a half-cleaner
  * (k = 0) those with (l & j) == 0 */
@@ -661,36 +655,90 @@ constexpr uint64_t keep_low_mask(int k, int j)
     }
     return m;
 }
-/* One compare-exchange of (d^2, index) keys with lane l ^ J.  Keys of one
- * list are distinct except for the (+inf, NONE) padding, and swapping equal
- * keys changes nothing, so "take the partner's key" is (partner < mine) on
- * keep-low lanes and !(partner < mine) elsewhere, without branches. */
-template <int K2, int J> __device__ __forceinline__ void cex(double &d, uint32_t &i)
+/* The selection sorts 32-bit keys: the top 26 bits of d^2 below the sign
+ * (exponent and 15 mantissa bits; d^2 >= 0, so its bits order like its
+ * value) with the key's source lane in the low 6 bits.  These keys are
+ * distinct, a compare-exchange is one min or max, and only the source lane
+ * travels; the (d^2, index) pairs follow by one permute at the end.  Keys
+ * whose top bits agree may come out in the wrong exact order (only among
+ * themselves: the truncation is monotone), which exact_fixup repairs. */
+__device__ __forceinline__ uint32_t trunc_key(double d, int lane)
 {
-    const double pd = xor_lane_f64<J>(d);
-    const uint32_t pi = xor_lane<J>(i);
-    const bool lt = (pd < d) | ((pd == d) & (pi < i));
-    const bool low = __builtin_amdgcn_inverse_ballot_w64(keep_low_mask(K2, J));
-    const bool sw = !(lt ^ low);
-    d = sw ? pd : d;
-    i = sw ? pi : i;
+    const uint32_t hi = (uint32_t)((uint64_t)__double_as_longlong(d) >> 32);
+    return ((hi << 1) & ~63u) | (uint32_t)lane;
 }
 
-/* Bitonic sort of one (d^2, index) key per lane over the 64 lanes,
- * ascending in lane order. */
+/* one compare-exchange of the 32-bit keys with lane l ^ J (bitonic stage
+ * (K2, J), or a half-cleaner for K2 = 0) */
+template <int K2, int J> __device__ __forceinline__ void cex_key(uint32_t &k)
+{
+    const uint32_t pk = xor_lane<J>(k);
+    const bool low = __builtin_amdgcn_inverse_ballot_w64(keep_low_mask(K2, J));
+    k = low ? min(k, pk) : max(k, pk);
+}
+
+/* Odd-even transposition passes on the exact (d^2, index) keys until no
+ * pair is out of order.  Runs only when two neighbours share their truncated
+ * key, and then moves only keys within such runs (the runs are already in
+ * place relative to each other), so one or two passes usually end it. */
+__device__ __forceinline__ void exact_fixup(double &d, uint32_t &i)
+{
+    const int lane = threadIdx.x & 63;
+    for (;;) {
+        bool moved = false;
+#pragma unroll
+        for (int ph = 0; ph < 2; ++ph) {
+            const int partner = ph == 0 ? (lane ^ 1) : ((lane & 1) ? lane + 1 : lane - 1);
+            const bool ok = partner >= 0 && partner < 64;
+            const int src = ok ? partner : lane;
+            const double pd = __shfl(d, src, 64);
+            const uint32_t pi = (uint32_t)__shfl((int)i, src, 64);
+            const bool p_less = key_less(pd, pi, d, i);
+            const bool m_less = key_less(d, i, pd, pi);
+            /* the lower lane keeps the smaller key */
+            const bool sw = ok && (lane < partner ? p_less : m_less);
+            d = sw ? pd : d;
+            i = sw ? pi : i;
+            moved = moved || sw;
+        }
+        if (!__ballot(moved)) break;
+    }
+}
+
+/* after a key sort: fetch the (d^2, index) pairs in key order, repair runs
+ * of equal truncated keys */
+__device__ __forceinline__ void keys_to_pairs(uint32_t k, double &d, uint32_t &i)
+{
+    const int lane = threadIdx.x & 63;
+    const int src = (int)(k & 63u);
+    d = __shfl(d, src, 64);
+    i = (uint32_t)__shfl((int)i, src, 64);
+    /* (+inf, NONE) padding shares one key and needs no order */
+    const uint32_t t = k >> 6;
+    const uint32_t tn = (uint32_t)__shfl_down((int)t, 1, 64);
+    if (__ballot(lane < 63 && t == tn && d < __longlong_as_double(0x7ff0000000000000ll))) exact_fixup(d, i);
+}
+
+/* Sort of one (d^2, index) key per lane over the 64 lanes, ascending in lane
+ * order (bitonic network on the truncated keys, then the exact repair). */
 __device__ __forceinline__ void sort64(double &d, uint32_t &i)
 {
-    cex<2, 1>(d, i);
-    cex<4, 2>(d, i); cex<4, 1>(d, i);
-    cex<8, 4>(d, i); cex<8, 2>(d, i); cex<8, 1>(d, i);
-    cex<16, 8>(d, i); cex<16, 4>(d, i); cex<16, 2>(d, i); cex<16, 1>(d, i);
-    cex<32, 16>(d, i); cex<32, 8>(d, i); cex<32, 4>(d, i); cex<32, 2>(d, i); cex<32, 1>(d, i);
-    cex<64, 32>(d, i); cex<64, 16>(d, i); cex<64, 8>(d, i); cex<64, 4>(d, i); cex<64, 2>(d, i); cex<64, 1>(d, i);
+    uint32_t k = trunc_key(d, threadIdx.x & 63);
+    cex_key<2, 1>(k);
+    cex_key<4, 2>(k); cex_key<4, 1>(k);
+    cex_key<8, 4>(k); cex_key<8, 2>(k); cex_key<8, 1>(k);
+    cex_key<16, 8>(k); cex_key<16, 4>(k); cex_key<16, 2>(k); cex_key<16, 1>(k);
+    cex_key<32, 16>(k); cex_key<32, 8>(k); cex_key<32, 4>(k); cex_key<32, 2>(k); cex_key<32, 1>(k);
+    cex_key<64, 32>(k); cex_key<64, 16>(k); cex_key<64, 8>(k); cex_key<64, 4>(k); cex_key<64, 2>(k); cex_key<64, 1>(k);
+    keys_to_pairs(k, d, i);
 }
 
 /* (bd, bi) and (cd, ci) sorted ascending over the lanes: (bd, bi) becomes
- * the 64 smallest keys of both, sorted (the lane-wise minimum against the
- * reversed second list is bitonic; six half-cleaners sort it). */
+ * the 64 smallest keys of both, sorted.  The lane-wise minimum against the
+ * reversed second list (exact keys) keeps the 64 smallest and is bitonic;
+ * six half-cleaners on the truncated keys sort it up to runs of equal
+ * truncated keys (a bitonic sequence stays bitonic for every threshold
+ * between distinct truncated keys), which the exact repair orders. */
 __device__ __forceinline__ void merge64(double &bd, uint32_t &bi, double cd, uint32_t ci)
 {
     const int lane = threadIdx.x & 63;
@@ -699,7 +747,9 @@ __device__ __forceinline__ void merge64(double &bd, uint32_t &bi, double cd, uin
     const bool t = (rd < bd) | ((rd == bd) & (ri < bi));
     bd = t ? rd : bd;
     bi = t ? ri : bi;
-    cex<0, 32>(bd, bi); cex<0, 16>(bd, bi); cex<0, 8>(bd, bi); cex<0, 4>(bd, bi); cex<0, 2>(bd, bi); cex<0, 1>(bd, bi);
+    uint32_t k = trunc_key(bd, lane);
+    cex_key<0, 32>(k); cex_key<0, 16>(k); cex_key<0, 8>(k); cex_key<0, 4>(k); cex_key<0, 2>(k); cex_key<0, 1>(k);
+    keys_to_pairs(k, bd, bi);
 }
 
 /* LDS of one select wave: node cache, the include list of the walk and the
