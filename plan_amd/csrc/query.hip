@@ -752,17 +752,26 @@ __device__ __forceinline__ void merge64(double &bd, uint32_t &bi, double cd, uin
     keys_to_pairs(k, bd, bi);
 }
 
+/* Selection scratch of the query being selected: its list entries (sent),
+ * and either the candidate prefix over them (spref, general path) or up to
+ * 64 staged candidates (sd, si) waiting to be sorted (leaf path). */
+template <int CAP> struct SelectScratch {
+    uint16_t sent[CAP];
+    union {
+        uint32_t spref[CAP + 1];
+        struct {
+            double sd[64];
+            uint32_t si[64];
+        };
+    };
+};
 /* LDS of one select wave: node cache, the include list of the walk and the
- * compacted entries (list index, candidate prefix) of the query being
- * selected */
+ * selection scratch (sharing its space with the walk's stack) */
 template <int CAP> struct SelectLds {
     NodeCache nc;
     ListEnt lst[CAP];
     union {
-        struct {
-            uint16_t sent[CAP];
-            uint32_t spref[CAP + 1];
-        };
+        SelectScratch<CAP> ss;
         DenseWalkLds dw;
     };
 };
@@ -772,48 +781,131 @@ static_assert(sizeof(SelectLds<256>) <= 6144, "a K <= 32 select wave should fit 
 template <int CAP> struct ResumeLds {
     NodeCache nc;
     ListEnt lst[CAP];
-    uint16_t sent[CAP];
-    uint32_t spref[CAP + 1];
+    SelectScratch<CAP> ss;
     DenseWalkLdsT<WALK_STACK_FULL> dw;
     uint64_t cont[8];
 };
 
+/* 64 candidates (one per lane, (+inf, NONE) where none) into the running
+ * best-64 (bd, bi) */
+__device__ __forceinline__ void feed64(double d, uint32_t id, double &bd, uint32_t &bi, bool &first)
+{
+    sort64(d, id);
+    if (first) {
+        bd = d;
+        bi = id;
+        first = false;
+    } else {
+        merge64(bd, bi, d, id);
+    }
+}
+
 /* Feed the candidates of query bit j of the recorded list to the running
- * best-64 (bd, bi): the entries of the query are compacted into sent/spref,
- * then candidates are taken 64 at a time (lane c of a chunk locates its
- * particle by binary search over the prefix), measured, filtered by T,
- * sorted and merged. */
+ * best-64 (bd, bi).  The query's entries are compacted into sent in one
+ * ballot pass.  Entries are leaves of at most 8 particles nearly always:
+ * then 8 entries x 8 slots are measured per round, the candidates within T
+ * that can still enter the kq best (below the current kq-th key) are staged
+ * in LDS, and every 64 staged are sorted and merged.  A query with a larger
+ * entry (a node the sphere contains, or a level-32 leaf) takes its
+ * candidates 64 at a time instead, each lane locating its particle by
+ * binary search over the entries' candidate prefix. */
 template <int K, class LDS>
 __device__ __forceinline__ void select_list(const TreeView &tv, LDS &L, uint32_t nent, int j, double qx,
                                             double qy, double qz, double T, double &bd, uint32_t &bi, bool &first)
 {
     const int lane = threadIdx.x & 63;
-    uint32_t ne = 0, np = 0;
-    for (uint32_t e = lane; e < nent; e += 64) {
-        const ListEnt le = L.lst[e];
-        if ((le.mask >> j) & 1ull) {
-            ++ne;
-            np += le.end - le.begin;
-        }
-    }
-    const uint32_t ne_in = wave_incl_scan(ne), np_in = wave_incl_scan(np);
-    const uint32_t M = lane_u32(ne_in, 63), C = lane_u32(np_in, 63);
-    __syncthreads();
-    {
-        uint32_t slot = ne_in - ne, pref = np_in - np;
-        for (uint32_t e = lane; e < nent; e += 64) {
-            const ListEnt le = L.lst[e];
-            if ((le.mask >> j) & 1ull) {
-                L.sent[slot] = (uint16_t)e;
-                L.spref[slot] = pref;
-                ++slot;
-                pref += le.end - le.begin;
-            }
-        }
-        if (lane == 0) L.spref[M] = C;
-    }
-    __syncthreads();
     const double inf = __longlong_as_double(0x7ff0000000000000ll);
+    SelectScratch<ListCap<K>::value> &S = L.ss;
+    uint32_t M = 0;
+    bool big = false;
+    __syncthreads();
+    for (uint32_t e0 = 0; e0 < nent; e0 += 64) {
+        const uint32_t e = e0 + (uint32_t)lane;
+        bool has = false;
+        uint32_t sz = 0;
+        if (e < nent) {
+            const ListEnt le = L.lst[e];
+            has = (le.mask >> j) & 1ull;
+            sz = le.end - le.begin;
+        }
+        const uint64_t b = __ballot(has);
+        big = big || __ballot(has && sz > 8u) != 0ull;
+        if (has) S.sent[M + mbcnt64(b)] = (uint16_t)e;
+        M += (uint32_t)__popcll(b);
+    }
+    __syncthreads();
+    if (!big) {
+        uint32_t ns = 0; /* staged candidates */
+        for (uint32_t x0 = 0; x0 < M; x0 += 8) {
+            const uint32_t x = x0 + ((uint32_t)lane >> 3);
+            double d = inf;
+            uint32_t id = PLAN_NONE;
+            bool v = false;
+            if (x < M) {
+                const ListEnt le = L.lst[S.sent[x]];
+                const uint32_t p = le.begin + ((uint32_t)lane & 7u);
+                if (p < le.end) {
+                    const double4 q = tv.pm[p];
+                    d = dist2(qx, qy, qz, q.x, q.y, q.z);
+                    id = p;
+                    v = d <= T;
+                }
+            }
+            if (!first) {
+                /* only keys below the current kq-th can enter the kq best */
+                const double kd = lane_f64(bd, (int)tv.kq - 1);
+                const uint32_t ki = lane_u32(bi, (int)tv.kq - 1);
+                v = v && key_less(d, id, kd, ki);
+            }
+            const uint64_t b = __ballot(v);
+            const uint32_t nv = (uint32_t)__popcll(b);
+            if (nv == 0) continue;
+            const uint32_t off = ns + mbcnt64(b);
+            if (v && off < 64u) {
+                S.sd[off] = d;
+                S.si[off] = id;
+            }
+            if (ns + nv < 64u) {
+                ns += nv;
+                continue;
+            }
+            __syncthreads();
+            feed64(S.sd[lane], S.si[lane], bd, bi, first);
+            __syncthreads();
+            if (v && off >= 64u) {
+                S.sd[off - 64u] = d;
+                S.si[off - 64u] = id;
+            }
+            ns = ns + nv - 64u;
+        }
+        if (ns > 0) {
+            __syncthreads();
+            const bool h = (uint32_t)lane < ns;
+            feed64(h ? S.sd[lane] : inf, h ? S.si[lane] : PLAN_NONE, bd, bi, first);
+        }
+        __syncthreads();
+        return;
+    }
+    /* general path: the candidate prefix over the query's entries */
+    uint32_t np = 0;
+    for (uint32_t x = (uint32_t)lane; x < M; x += 64) np += L.lst[S.sent[x]].end - L.lst[S.sent[x]].begin;
+    const uint32_t np_in = wave_incl_scan(np);
+    const uint32_t C = lane_u32(np_in, 63);
+    {
+        /* lane l holds the entries l, l + 64, ...: prefix of its own entries
+         * in lane order first, then the chunks in turn */
+        __syncthreads();
+        uint32_t run = 0;
+        for (uint32_t x0 = 0; x0 < M; x0 += 64) {
+            const uint32_t x = x0 + (uint32_t)lane;
+            const uint32_t sz = x < M ? L.lst[S.sent[x]].end - L.lst[S.sent[x]].begin : 0u;
+            const uint32_t in = wave_incl_scan(sz);
+            if (x < M) S.spref[x] = run + in - sz;
+            run += lane_u32(in, 63);
+        }
+        if (lane == 0) S.spref[M] = C;
+    }
+    __syncthreads();
     uint32_t steps = 1;
     while (steps * 2 < M) steps *= 2;
     for (uint32_t base = 0; base < C; base += 64) {
@@ -825,9 +917,9 @@ __device__ __forceinline__ void select_list(const TreeView &tv, LDS &L, uint32_t
             uint32_t lo = 0;
             for (uint32_t step = steps; step > 0; step >>= 1) {
                 const uint32_t t = lo + step;
-                lo = (t < M && L.spref[t] <= c) ? t : lo;
+                lo = (t < M && S.spref[t] <= c) ? t : lo;
             }
-            const uint32_t p = L.lst[L.sent[lo]].begin + (c - L.spref[lo]);
+            const uint32_t p = L.lst[S.sent[lo]].begin + (c - S.spref[lo]);
             const double4 x = tv.pm[p];
             const double dd = dist2(qx, qy, qz, x.x, x.y, x.z);
             if (dd <= T) {
@@ -836,7 +928,6 @@ __device__ __forceinline__ void select_list(const TreeView &tv, LDS &L, uint32_t
             }
         }
         if (!first) {
-            /* only keys below the current kq-th can enter the kq best */
             const double kd = lane_f64(bd, (int)tv.kq - 1);
             const uint32_t ki = lane_u32(bi, (int)tv.kq - 1);
             if (id != PLAN_NONE && !key_less(d, id, kd, ki)) {
@@ -845,15 +936,9 @@ __device__ __forceinline__ void select_list(const TreeView &tv, LDS &L, uint32_t
             }
         }
         if (!__ballot(id != PLAN_NONE)) continue;
-        sort64(d, id);
-        if (first) {
-            bd = d;
-            bi = id;
-            first = false;
-        } else {
-            merge64(bd, bi, d, id);
-        }
+        feed64(d, id, bd, bi, first);
     }
+    __syncthreads();
 }
 
 /* What a query's sorted neighbour list (lane l holds the l-th nearest, l < kq)
