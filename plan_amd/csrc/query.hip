@@ -101,7 +101,7 @@ __device__ __forceinline__ bool sphere_intersect(const NodeGeo &g, double qx, do
  * it (a wave-uniform branch).  An axis where the query lies inside the box
  * adds +0 to the box distance, which leaves the sum unchanged. */
 __device__ __forceinline__ int classify_sphere(const NodeGeo &g, bool leaf, double qx, double qy, double qz, double r,
-                                               double to_diag)
+                                               double to_diag, bool active = true)
 {
     const double t = __dmul_rn(g.hw, to_diag);
     const double dx = __dsub_rn(g.cx, qx), dy = __dsub_rn(g.cy, qy), dz = __dsub_rn(g.cz, qz);
@@ -114,7 +114,7 @@ __device__ __forceinline__ int classify_sphere(const NodeGeo &g, bool leaf, doub
     const bool far = c2c > __dmul_rn(tt, tt);
     const bool quirk = c2c < r || c2c < g.hw;
     bool inter = !far && quirk;
-    const bool need_box = !far && !quirk && !contain;
+    const bool need_box = active && !far && !quirk && !contain;
     if (__ballot(need_box)) {
         const double lx = __dsub_rn(g.cx, g.hw), hx = __dadd_rn(g.cx, g.hw);
         const double ly = __dsub_rn(g.cy, g.hw), hy = __dadd_rn(g.cy, g.hw);
@@ -268,6 +268,9 @@ template <int K> struct ListCap {
  * select wave fits 6 KB of LDS (26 waves per CU instead of 22 at 7 KB); a
  * walk that needs more is treated as a list overflow. */
 constexpr int WALK_STACK = 108;
+/* a step tests each lane's own query against the children when at least
+ * this many queries descended into the group */
+constexpr uint32_t PER_LANE_MIN = 40;
 /* A depth-first walk holds at most 7 pending sibling groups per level of
  * its current path (32 levels) plus the root: 256 groups never overflow. */
 constexpr int WALK_STACK_FULL = 256;
@@ -351,64 +354,96 @@ __device__ WalkEnd walk_dense_run(const TreeView &tv, NodeCache &nc, DenseWalkLd
             nc.t[lane] = tv.ntop[fd + lane];
             nc.info[lane] = tv.ninfo[fd + lane];
         }
-        if ((M >> lane) & 1ull) D.rk[mbcnt64(M)] = (uint8_t)lane;
-        if (lane < 8) {
-            D.inc[lane] = 0;
-            D.desc[lane] = 0;
-            if (MODE == WALK_COUNT) cont[lane] = 0;
-        }
-        __syncthreads();
         const uint32_t cnt = (uint32_t)__popcll(M);
-        const uint32_t total = nd * cnt;
-        /* hardware reciprocal (1 ulp): (p + 0.5) rc stays within 2^-11 of
-         * (p + 0.5) / cnt for p < 512, far inside the 0.5 / cnt margin */
-        const float rc = __builtin_amdgcn_rcpf((float)cnt);
-        for (uint32_t base = 0; base < total; base += 64) {
-            const uint32_t p = base + (uint32_t)lane;
-            const bool pv = p < total;
-            /* p / cnt, exact for p < 512 (half a step of margin) */
-            uint32_t c = (uint32_t)(((float)p + 0.5f) * rc);
-            c = pv ? c : 0u;
-            const uint32_t k = pv ? p - c * cnt : 0u;
-            const int j = (int)D.rk[k];
-            const double xj = __shfl(qx, j, 64), yj = __shfl(qy, j, 64), zj = __shfl(qz, j, 64), rj = __shfl(r, j, 64);
-            const NodeGeo g = nc.g[c];
-            const bool leaf = (nc.info[c] & 0xffu) == 0u;
-            int act;
-            if constexpr (GEO) act = classify_geo(g, leaf, xj, yj, zj, rj, tv.margin_abs);
-            else act = classify_sphere(g, leaf, xj, yj, zj, rj, tv.to_diag);
-            if constexpr (MODE == WALK_GEOCOUNT) {
-                if (pv && act == 0) atomicAdd(&qcnt[j], nc.t[c].end - nc.t[c].begin);
-                if (pv && act == 1) {
-                    const NodeTop t = nc.t[c];
-                    const double r2 = rj * rj;
-                    uint32_t in = 0;
-                    for (uint32_t q = t.begin; q < t.end; ++q) {
-                        const double4 x = tv.pm[q];
-                        if (dist2(xj, yj, zj, x.x, x.y, x.z) <= r2) ++in;
-                    }
-                    atomicAdd(&qcnt[j], in);
-                }
-            } else {
-                if (pv && act <= 1) atomicOr((unsigned long long *)&D.inc[c], 1ull << j);
-                if (MODE == WALK_COUNT && pv && act == 0) atomicOr((unsigned long long *)&cont[c], 1ull << j);
+        /* Many queries on one group (the upper levels, 45% of all pairs at
+         * 16.8M particles): each lane tests its own query against the
+         * children in turn and ballots give the masks, with no lane
+         * exchanges or LDS atomics.  Few queries: the (child, query) pairs
+         * are packed densely over the lanes. */
+        constexpr bool CAN_LANE = MODE == WALK_LIST || MODE == WALK_COUNT;
+        const bool per_lane = CAN_LANE && cnt >= PER_LANE_MIN;
+        if (!per_lane) {
+            if ((M >> lane) & 1ull) D.rk[mbcnt64(M)] = (uint8_t)lane;
+            if (lane < 8) {
+                D.inc[lane] = 0;
+                D.desc[lane] = 0;
+                if (MODE == WALK_COUNT) cont[lane] = 0;
             }
-            if (pv && act == 2) atomicOr((unsigned long long *)&D.desc[c], 1ull << j);
         }
         __syncthreads();
-        if constexpr (MODE == WALK_COUNT) {
+        uint64_t I = 0, Dm = 0;
+        if (per_lane) {
+            const bool mine = (M >> lane) & 1ull;
             for (uint32_t c = 0; c < nd; ++c) {
-                const uint32_t sz = nc.t[c].end - nc.t[c].begin;
-                if ((D.inc[c] >> lane) & 1ull) sc += sz;
-                if ((cont[c] >> lane) & 1ull) cc += sz;
+                const NodeGeo g = nc.g[c];
+                const bool leaf = (nc.info[c] & 0xffu) == 0u;
+                const int act = classify_sphere(g, leaf, qx, qy, qz, r, tv.to_diag, mine);
+                const uint64_t tk = __ballot(mine && act <= 1);
+                const uint64_t dn = __ballot(mine && act == 2);
+                if constexpr (MODE == WALK_COUNT) {
+                    const uint32_t sz = nc.t[c].end - nc.t[c].begin;
+                    if (mine && act <= 1) sc += sz;
+                    if (mine && act == 0) cc += sz;
+                }
+                if ((uint32_t)lane == c) {
+                    I = tk;
+                    Dm = dn;
+                }
+            }
+        } else {
+            const uint32_t total = nd * cnt;
+            /* hardware reciprocal (1 ulp): (p + 0.5) rc stays within 2^-11 of
+             * (p + 0.5) / cnt for p < 512, far inside the 0.5 / cnt margin */
+            const float rc = __builtin_amdgcn_rcpf((float)cnt);
+            for (uint32_t base = 0; base < total; base += 64) {
+                const uint32_t p = base + (uint32_t)lane;
+                const bool pv = p < total;
+                /* p / cnt, exact for p < 512 (half a step of margin) */
+                uint32_t c = (uint32_t)(((float)p + 0.5f) * rc);
+                c = pv ? c : 0u;
+                const uint32_t k = pv ? p - c * cnt : 0u;
+                const int j = (int)D.rk[k];
+                const double xj = __shfl(qx, j, 64), yj = __shfl(qy, j, 64), zj = __shfl(qz, j, 64),
+                             rj = __shfl(r, j, 64);
+                const NodeGeo g = nc.g[c];
+                const bool leaf = (nc.info[c] & 0xffu) == 0u;
+                int act;
+                if constexpr (GEO) act = classify_geo(g, leaf, xj, yj, zj, rj, tv.margin_abs);
+                else act = classify_sphere(g, leaf, xj, yj, zj, rj, tv.to_diag, pv);
+                if constexpr (MODE == WALK_GEOCOUNT) {
+                    if (pv && act == 0) atomicAdd(&qcnt[j], nc.t[c].end - nc.t[c].begin);
+                    if (pv && act == 1) {
+                        const NodeTop t = nc.t[c];
+                        const double r2 = rj * rj;
+                        uint32_t in = 0;
+                        for (uint32_t q = t.begin; q < t.end; ++q) {
+                            const double4 x = tv.pm[q];
+                            if (dist2(xj, yj, zj, x.x, x.y, x.z) <= r2) ++in;
+                        }
+                        atomicAdd(&qcnt[j], in);
+                    }
+                } else {
+                    if (pv && act <= 1) atomicOr((unsigned long long *)&D.inc[c], 1ull << j);
+                    if (MODE == WALK_COUNT && pv && act == 0) atomicOr((unsigned long long *)&cont[c], 1ull << j);
+                }
+                if (pv && act == 2) atomicOr((unsigned long long *)&D.desc[c], 1ull << j);
+            }
+            __syncthreads();
+            if constexpr (MODE == WALK_COUNT) {
+                for (uint32_t c = 0; c < nd; ++c) {
+                    const uint32_t sz = nc.t[c].end - nc.t[c].begin;
+                    if ((D.inc[c] >> lane) & 1ull) sc += sz;
+                    if ((cont[c] >> lane) & 1ull) cc += sz;
+                }
+            }
+            if ((uint32_t)lane < nd) {
+                I = D.inc[lane];
+                Dm = D.desc[lane];
             }
         }
-        uint64_t I = 0, Dm = 0;
         NodeTop t{0u, 0u, 0u, 0u};
         uint32_t cnd = 0;
         if ((uint32_t)lane < nd) {
-            I = D.inc[lane];
-            Dm = D.desc[lane];
             t = nc.t[lane];
             cnd = nc.info[lane] & 0xffu;
         }
