@@ -270,7 +270,7 @@ template <int K> struct ListCap {
 constexpr int WALK_STACK = 108;
 /* a step tests each lane's own query against the children when at least
  * this many queries descended into the group */
-constexpr uint32_t PER_LANE_MIN = 40;
+constexpr uint32_t PER_LANE_MIN = 32;
 /* A depth-first walk holds at most 7 pending sibling groups per level of
  * its current path (32 levels) plus the root: 256 groups never overflow. */
 constexpr int WALK_STACK_FULL = 256;
@@ -355,7 +355,7 @@ __device__ WalkEnd walk_dense_run(const TreeView &tv, NodeCache &nc, DenseWalkLd
             nc.info[lane] = tv.ninfo[fd + lane];
         }
         const uint32_t cnt = (uint32_t)__popcll(M);
-        /* Many queries on one group (the upper levels, 45% of all pairs at
+        /* Many queries on one group (the upper levels, about 60% of all pairs at
          * 16.8M particles): each lane tests its own query against the
          * children in turn and ballots give the masks, with no lane
          * exchanges or LDS atomics.  Few queries: the (child, query) pairs
