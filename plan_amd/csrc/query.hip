@@ -376,6 +376,15 @@ __device__ WalkEnd walk_dense_run(const TreeView &tv, NodeCache &nc, DenseWalkLd
             const bool mine = (M >> lane) & 1ull;
             for (uint32_t c = 0; c < nd; ++c) {
                 const NodeGeo g = nc.g[c];
+                /* a child whose centre lies farther than t + r from the
+                 * query along one axis fails the bounding-sphere test
+                 * (c2c >= dx^2 > (t + r)^2, with a margin far above
+                 * rounding); when that holds for every query of the group
+                 * the child is rejected for all of them at once */
+                const double lim = __dmul_rn(__dadd_rn(__dmul_rn(g.hw, tv.to_diag), r), 1.0 + 1e-12);
+                const bool near = fabs(__dsub_rn(g.cx, qx)) <= lim && fabs(__dsub_rn(g.cy, qy)) <= lim &&
+                                  fabs(__dsub_rn(g.cz, qz)) <= lim;
+                if (!__ballot(mine && near)) continue;
                 const bool leaf = (nc.info[c] & 0xffu) == 0u;
                 const int act = classify_sphere(g, leaf, qx, qy, qz, r, tv.to_diag, mine);
                 const uint64_t tk = __ballot(mine && act <= 1);
