@@ -718,7 +718,9 @@ template <int K2, int J> __device__ __forceinline__ void cex_key(uint32_t &k)
 {
     const uint32_t pk = xor_lane<J>(k);
     const bool low = __builtin_amdgcn_inverse_ballot_w64(keep_low_mask(K2, J));
-    k = low ? min(k, pk) : max(k, pk);
+    /* keys are distinct: keep-low lanes take the partner's key when it is
+     * smaller, the others when it is larger */
+    k = ((pk < k) == low) ? pk : k;
 }
 
 /* Odd-even transposition passes on the exact (d^2, index) keys until no
@@ -860,6 +862,7 @@ __device__ __forceinline__ void select_list(const TreeView &tv, LDS &L, uint32_t
     const int lane = threadIdx.x & 63;
     const double inf = __longlong_as_double(0x7ff0000000000000ll);
     SelectScratch<ListCap<K>::value> &S = L.ss;
+    nent = __builtin_amdgcn_readfirstlane(nent);
     uint32_t M = 0;
     bool big = false;
     __syncthreads();
