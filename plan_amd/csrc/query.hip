@@ -991,12 +991,16 @@ __device__ __forceinline__ void select_list(const TreeView &tv, LDS &L, uint32_t
 /* What a query's sorted neighbour list (lane l holds the l-th nearest, l < kq)
  * is used for: the densest neighbour (HOP), the density (median spherical or
  * QS kernel) and/or the lists themselves.  Sums run in the reference's
- * order (src/tree.hpp:3320-3420). */
+ * order (src/tree.hpp:3320-3420).  The median kernel's last step, the same
+ * for every query, is left to median_density: its x (d^2 of the i*-th
+ * neighbour) and mass sum go to lane `slot` (tm, num), so one pass does it
+ * for the wave's queries a lane each. */
 template <int K>
 __device__ __forceinline__ void finish_query(const TreeView &tv, double bd, uint32_t bi, uint32_t q, uint32_t self,
                                              bool byq, int mode, int kernel, const double *__restrict__ rho_in,
                                              double *__restrict__ rho_out, uint32_t *__restrict__ densest,
-                                             uint32_t *__restrict__ idx_out, double *__restrict__ d2_out)
+                                             uint32_t *__restrict__ idx_out, double *__restrict__ d2_out, int slot,
+                                             double &tm_out, double &num_out)
 {
     const int lane = threadIdx.x & 63;
     const int kq = (int)tv.kq;
@@ -1051,18 +1055,11 @@ __device__ __forceinline__ void finish_query(const TreeView &tv, double bd, uint
 #pragma unroll
             for (int j = K - 1; j >= 0; --j) rho = __dadd_rn(rho, lane_f64(w, j));
         }
-        const double four_pi_over_three = (3.14159265358979323846 * 4.) / 3.;
-        bool near;
-        const double num = rho;
-        rho = __ddiv_rn(rho, __dmul_rn(four_pi_over_three, pow15_near(tm, near)));
-        if (near && tv.pw_count && lane == 0) {
-            const uint32_t slot = atomicAdd(tv.pw_count, 1u);
-            if (slot < tv.pw_cap) {
-                tv.pw_idx[slot] = byq ? q : self;
-                tv.pw_x[slot] = tm;
-                tv.pw_num[slot] = num;
-            }
+        if (lane == slot) {
+            tm_out = tm;
+            num_out = rho;
         }
+        return;
     } else {
         /* QseudoQuadraticSplinesKernel; point queries use particle 0 as "self" */
         double term = 0;
@@ -1088,6 +1085,26 @@ __device__ __forceinline__ void finish_query(const TreeView &tv, double bd, uint
         for (int j = 0; j < K; ++j) rho = __dadd_rn(rho, lane_f64(term, j));
     }
     if (lane == 0) rho_out[byq ? q : self] = rho;
+}
+
+/* The median kernel's density of this lane's query, num / (4 pi / 3 x^1.5)
+ * (src/tree.hpp:3489), with x^1.5 correctly rounded; densities whose x^1.5
+ * the C library's pow may round the other way are listed for the host. */
+__device__ __forceinline__ void median_density(const TreeView &tv, double tm, double num, uint32_t q, uint32_t self,
+                                               bool byq, double *__restrict__ rho_out)
+{
+    const double four_pi_over_three = (3.14159265358979323846 * 4.) / 3.;
+    bool near;
+    const double rho = __ddiv_rn(num, __dmul_rn(four_pi_over_three, pow15_near(tm, near)));
+    if (near && tv.pw_count) {
+        const uint32_t slot = atomicAdd(tv.pw_count, 1u);
+        if (slot < tv.pw_cap) {
+            tv.pw_idx[slot] = byq ? q : self;
+            tv.pw_x[slot] = tm;
+            tv.pw_num[slot] = num;
+        }
+    }
+    rho_out[byq ? q : self] = rho;
 }
 
 /* The kq nearest of S(r*) per query, and their use, in one kernel (K, the
@@ -1214,6 +1231,7 @@ __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, con
             }
         }
         __syncthreads();
+        double d_tm = 0, d_num = 0;
         for (uint32_t j = 0; j < nvalid; ++j) {
             double bd = inf;
             uint32_t bi = PLAN_NONE;
@@ -1221,8 +1239,9 @@ __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, con
             select_list<K>(tv, L, nent, (int)j, lane_f64(qx, j), lane_f64(qy, j), lane_f64(qz, j), lane_f64(T, j), bd,
                         bi, first);
             finish_query<K>(tv, bd, bi, wv * qpw + j, lane_u32(self, j), qpts || qidx, mode, kernel, rho_in,
-                            rho_out, densest, idx_out, d2_out);
+                            rho_out, densest, idx_out, d2_out, (int)j, d_tm, d_num);
         }
+        if (mode == 0 && kernel == 0 && valid) median_density(tv, d_tm, d_num, q, self, qpts || qidx, rho_out);
     } else {
         double *bd2 = sd2 + (size_t)blockIdx.x * qpw * K;
         uint32_t *bix = sidx + (size_t)blockIdx.x * qpw * K;
@@ -1257,12 +1276,14 @@ __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, con
             }
             __syncthreads();
         }
+        double d_tm = 0, d_num = 0;
         for (uint32_t j = 0; j < nvalid; ++j) {
             const double bd = lane < K ? bd2[j * K + lane] : inf;
             const uint32_t bi = lane < K ? bix[j * K + lane] : PLAN_NONE;
             finish_query<K>(tv, bd, bi, wv * qpw + j, lane_u32(self, j), qpts || qidx, mode, kernel, rho_in,
-                            rho_out, densest, idx_out, d2_out);
+                            rho_out, densest, idx_out, d2_out, (int)j, d_tm, d_num);
         }
+        if (mode == 0 && kernel == 0 && valid) median_density(tv, d_tm, d_num, q, self, qpts || qidx, rho_out);
     }
 }
 
