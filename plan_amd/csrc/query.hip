@@ -802,13 +802,17 @@ __device__ __forceinline__ void merge64(double &bd, uint32_t &bi, double cd, uin
  * and either the candidate prefix over them (spref, general path) or up to
  * 64 staged candidates (sd, si) waiting to be sorted (leaf path). */
 template <int CAP> struct SelectScratch {
-    uint16_t sent[CAP];
+    union {
+        uint16_t sent[CAP];
+        uint8_t start[64];
+    };
     union {
         uint32_t spref[CAP + 1];
         struct {
             double sd[64];
             uint32_t si[64];
         };
+        uint2 ent[64];
     };
 };
 /* LDS of one select wave: node cache, the include list of the walk and the
@@ -847,14 +851,17 @@ __device__ __forceinline__ void feed64(double d, uint32_t id, double &bd, uint32
 }
 
 /* Feed the candidates of query bit j of the recorded list to the running
- * best-64 (bd, bi).  The query's entries are compacted into sent in one
- * ballot pass.  Entries are leaves of at most 8 particles nearly always:
- * then 8 entries x 8 slots are measured per round, the candidates within T
+ * best-64 (bd, bi).  One ballot pass over the list finds the query's entries
+ * (their ids to sent; the first 64 as (begin, end) to ent).  With at most 64
+ * entries (nearly every query) their candidates are laid out densely over
+ * the lanes, 64 per round: a prefix sum of the entry sizes gives each
+ * entry's first position, a ballot over the marked first positions gives the
+ * entry of each position, and the entry's first particle comes from its
+ * lane.  The first 64 candidates within T are sorted at once; later ones
  * that can still enter the kq best (below the current kq-th key) are staged
- * in LDS, and every 64 staged are sorted and merged.  A query with a larger
- * entry (a node the sphere contains, or a level-32 leaf) takes its
- * candidates 64 at a time instead, each lane locating its particle by
- * binary search over the entries' candidate prefix. */
+ * in LDS and sorted and merged 64 at a time.  A query with more entries
+ * takes its candidates 64 at a time by binary search over the entries'
+ * candidate prefix instead. */
 template <int K, class LDS>
 __device__ __forceinline__ void select_list(const TreeView &tv, LDS &L, uint32_t nent, int j, double qx,
                                             double qy, double qz, double T, double &bd, uint32_t &bi, bool &first)
@@ -864,50 +871,84 @@ __device__ __forceinline__ void select_list(const TreeView &tv, LDS &L, uint32_t
     SelectScratch<ListCap<K>::value> &S = L.ss;
     nent = __builtin_amdgcn_readfirstlane(nent);
     uint32_t M = 0;
-    bool big = false;
     __syncthreads();
     for (uint32_t e0 = 0; e0 < nent; e0 += 64) {
         const uint32_t e = e0 + (uint32_t)lane;
         bool has = false;
-        uint32_t sz = 0;
+        uint32_t b = 0, en = 0;
         if (e < nent) {
             const ListEnt le = L.lst[e];
             has = (le.mask >> j) & 1ull;
-            sz = le.end - le.begin;
+            b = le.begin;
+            en = le.end;
         }
-        const uint64_t b = __ballot(has);
-        big = big || __ballot(has && sz > 8u) != 0ull;
-        if (has) S.sent[M + mbcnt64(b)] = (uint16_t)e;
-        M += (uint32_t)__popcll(b);
+        const uint64_t bm = __ballot(has);
+        const uint32_t rk = M + mbcnt64(bm);
+        if (has) {
+            S.sent[rk] = (uint16_t)e;
+            if (rk < 64u) S.ent[rk] = make_uint2(b, en);
+        }
+        M += (uint32_t)__popcll(bm);
     }
     __syncthreads();
-    if (!big) {
+    if (M <= 64u) {
+        /* lane r < M holds entry r: its size, first candidate position p0
+         * and base (particle index = base + position, modulo 2^32); node
+         * ranges are never empty */
+        uint32_t b0 = 0, sz = 0;
+        if ((uint32_t)lane < M) {
+            const uint2 en = S.ent[lane];
+            b0 = en.x;
+            sz = en.y - en.x;
+        }
+        const uint32_t incl = wave_incl_scan(sz);
+        const uint32_t C = lane_u32(incl, 63);
+        const uint32_t p0 = incl - sz;
+        const uint32_t base = b0 - p0;
+        const bool starts = (uint32_t)lane < M && sz > 0u;
         uint32_t ns = 0; /* staged candidates */
-        for (uint32_t x0 = 0; x0 < M; x0 += 8) {
-            const uint32_t x = x0 + ((uint32_t)lane >> 3);
+        for (uint32_t k0 = 0; k0 < C; k0 += 64) {
+            __syncthreads();
+            S.start[lane] = 0;
+            __syncthreads();
+            if (starts && p0 >= k0 && p0 - k0 < 64u) S.start[p0 - k0] = 1;
+            __syncthreads();
+            const uint64_t sm = __ballot(S.start[lane] != 0);
+            const uint32_t before = (uint32_t)__popcll(__ballot(starts && p0 < k0));
+            /* the entry holding position k0 + lane: the entries starting at
+             * or below it, less one */
+            int rk = (int)(before + mbcnt64(sm) + (uint32_t)((sm >> lane) & 1ull)) - 1;
+            rk = rk < 0 ? 0 : (rk >= (int)M ? (int)M - 1 : rk);
+            const uint32_t pb = (uint32_t)__shfl((int)base, rk, 64);
+            const uint32_t c = k0 + (uint32_t)lane;
             double d = inf;
             uint32_t id = PLAN_NONE;
-            bool v = false;
-            if (x < M) {
-                const ListEnt le = L.lst[S.sent[x]];
-                const uint32_t p = le.begin + ((uint32_t)lane & 7u);
-                if (p < le.end) {
-                    const double4 q = tv.pm[p];
-                    d = dist2(qx, qy, qz, q.x, q.y, q.z);
-                    id = p;
-                    v = d <= T;
+            if (c < C) {
+                const uint32_t p = pb + c;
+                /* always below n: keeps a wrong rank from reading outside
+                 * the particle arrays */
+                if (p < tv.n) {
+                    const double4 x = tv.pm[p];
+                    const double dd = dist2(qx, qy, qz, x.x, x.y, x.z);
+                    if (dd <= T) {
+                        d = dd;
+                        id = p;
+                    }
                 }
             }
-            if (!first) {
-                /* only keys below the current kq-th can enter the kq best */
-                const double kd = lane_f64(bd, (int)tv.kq - 1);
-                const uint32_t ki = lane_u32(bi, (int)tv.kq - 1);
-                v = v && key_less(d, id, kd, ki);
+            if (first) {
+                feed64(d, id, bd, bi, first);
+                continue;
             }
-            const uint64_t b = __ballot(v);
-            const uint32_t nv = (uint32_t)__popcll(b);
+            /* only keys below the current kq-th can enter the kq best */
+            const double kd = lane_f64(bd, (int)tv.kq - 1);
+            const uint32_t ki = lane_u32(bi, (int)tv.kq - 1);
+            const bool v = id != PLAN_NONE && key_less(d, id, kd, ki);
+            const uint64_t vm = __ballot(v);
+            const uint32_t nv = (uint32_t)__popcll(vm);
             if (nv == 0) continue;
-            const uint32_t off = ns + mbcnt64(b);
+            const uint32_t off = ns + mbcnt64(vm);
+            __syncthreads();
             if (v && off < 64u) {
                 S.sd[off] = d;
                 S.si[off] = id;
