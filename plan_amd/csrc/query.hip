@@ -1194,21 +1194,28 @@ __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, con
     bool filled = false;
     if constexpr (FUSE && !RESUME) {
         /* the radius schedule r0, 2 r0, ... until |S(r)| >= kq
-         * (src/tree.hpp:3206-3223): each round walks every query at its
-         * current radius; the queries short of kq candidates double theirs.
-         * The list of the last round is the one selected from. */
-        const uint64_t V = __ballot(valid);
+         * (src/tree.hpp:3206-3223): the first round walks every query at r0;
+         * each later round walks only the queries still short of kq
+         * candidates, at their doubled radius, after taking their bits out of
+         * the previous round's entries, so the list holds each query's
+         * entries of its last round. */
+        uint64_t U = __ballot(valid);
+        nent = 0;
         for (int round = 0; round < 2200; ++round) {
-            nent = 0;
-            if (!walk_dense<CAP>(tv, L.nc, L.dw, L.lst, nent, qx, qy, qz, r, V)) {
+            const uint32_t e0 = nent;
+            if (!walk_dense<CAP>(tv, L.nc, L.dw, L.lst, nent, qx, qy, qz, r, U)) {
                 if (lane == 0) ovf[wv] = 1u;
                 return;
             }
             __syncthreads();
-            const uint32_t sc = list_count(L.lst, 0u, nent);
-            const bool need = valid && sc < tv.kq;
-            if (!__ballot(need)) break;
+            const uint32_t sc = list_count(L.lst, e0, nent);
+            const bool need = ((U >> lane) & 1ull) && sc < tv.kq;
+            const uint64_t N = __ballot(need);
+            if (!N) break;
+            for (uint32_t e = e0 + (uint32_t)lane; e < nent; e += 64) L.lst[e].mask &= ~N;
+            __syncthreads();
             if (need) r = __dmul_rn(r, 2.0);
+            U = N;
         }
         filled = true;
         if (valid) {
