@@ -248,17 +248,18 @@ __device__ __forceinline__ void query_setup(const TreeView &tv, uint32_t q, cons
 }
 
 /* One entry of a wave's include list: a node whose particles some lanes take
- * whole (contained node or intersecting leaf), with the mask of those lanes. */
+ * whole (contained node or intersecting leaf), with the mask of those lanes
+ * (12 bytes; the node's particle range is read from the node array when the
+ * entry is expanded). */
 struct ListEnt {
-    uint32_t begin, end;
-    uint64_t mask;
+    uint32_t node, mlo, mhi;
+    __device__ __forceinline__ uint64_t mask() const { return ((uint64_t)mhi << 32) | mlo; }
 };
 /* include-list capacity (entries) of one wave: larger K walks larger spheres.
  * The list is most of a wave's LDS, so the capacity trades occupancy against
- * the waves that overflow it and are redone in smaller pieces (at 16.8M
- * particles, 384 entries for K = 64 beat 256, 320, 448 and 512). */
+ * the waves that overflow it and are redone in smaller pieces. */
 template <int K> struct ListCap {
-    static constexpr int value = K >= 64 ? 384 : 256;
+    static constexpr int value = K >= 64 ? 512 : 320;
 };
 
 /* LDS of the group-wise walk: a stack of sibling groups still to visit
@@ -282,6 +283,7 @@ template <int STACK> struct DenseWalkLdsT {
     uint8_t rk[64];
     uint64_t inc[8];
     uint64_t desc[8];
+    uint32_t scnt[64]; /* list walks: particles taken per query (packed steps) */
 };
 using DenseWalkLds = DenseWalkLdsT<WALK_STACK>;
 
@@ -389,10 +391,10 @@ __device__ WalkEnd walk_dense_run(const TreeView &tv, NodeCache &nc, DenseWalkLd
                 const int act = classify_sphere(g, leaf, qx, qy, qz, r, tv.to_diag, mine);
                 const uint64_t tk = __ballot(mine && act <= 1);
                 const uint64_t dn = __ballot(mine && act == 2);
-                if constexpr (MODE == WALK_COUNT) {
+                if constexpr (MODE == WALK_COUNT || MODE == WALK_LIST) {
                     const uint32_t sz = nc.t[c].end - nc.t[c].begin;
                     if (mine && act <= 1) sc += sz;
-                    if (mine && act == 0) cc += sz;
+                    if (MODE == WALK_COUNT && mine && act == 0) cc += sz;
                 }
                 if ((uint32_t)lane == c) {
                     I = tk;
@@ -433,6 +435,7 @@ __device__ WalkEnd walk_dense_run(const TreeView &tv, NodeCache &nc, DenseWalkLd
                     }
                 } else {
                     if (pv && act <= 1) atomicOr((unsigned long long *)&D.inc[c], 1ull << j);
+                    if (MODE == WALK_LIST && pv && act <= 1) atomicAdd(&D.scnt[j], nc.t[c].end - nc.t[c].begin);
                     if (MODE == WALK_COUNT && pv && act == 0) atomicOr((unsigned long long *)&cont[c], 1ull << j);
                 }
                 if (pv && act == 2) atomicOr((unsigned long long *)&D.desc[c], 1ull << j);
@@ -463,7 +466,7 @@ __device__ WalkEnd walk_dense_run(const TreeView &tv, NodeCache &nc, DenseWalkLd
                 ++sp; /* the group is decided again by the next call */
                 return WalkEnd::list_full;
             }
-            if (I != 0ull) lst[nent + mbcnt64(mi)] = ListEnt{t.begin, t.end, I};
+            if (I != 0ull) lst[nent + mbcnt64(mi)] = ListEnt{fd + (uint32_t)lane, (uint32_t)I, (uint32_t)(I >> 32)};
             nent += ni;
         }
         const uint64_t md = __ballot(Dm != 0ull);
@@ -486,6 +489,7 @@ __device__ WalkEnd walk_dense_run(const TreeView &tv, NodeCache &nc, DenseWalkLd
 template <int STACK> __device__ __forceinline__ uint32_t walk_dense_start(DenseWalkLdsT<STACK> &D, uint64_t M0)
 {
     __syncthreads();
+    D.scnt[threadIdx.x] = 0;
     if (threadIdx.x == 0) {
         D.sfd[0] = 0;
         D.snd[0] = 1;
@@ -494,14 +498,20 @@ template <int STACK> __device__ __forceinline__ uint32_t walk_dense_start(DenseW
     return M0 ? 1u : 0u;
 }
 
-/* the whole walk into one list; false when the list or the stack is full */
+/* the whole walk into one list; false when the list or the stack is full.
+ * cnt: per lane, the particles of the entries recorded for its query (the
+ * size of S(r)) */
 template <int CAP>
 __device__ bool walk_dense(const TreeView &tv, NodeCache &nc, DenseWalkLds &D, ListEnt *lst, uint32_t &nent, double qx,
-                           double qy, double qz, double r, uint64_t M0)
+                           double qy, double qz, double r, uint64_t M0, uint32_t &cnt)
 {
     uint32_t sp = walk_dense_start(D, M0), sc = 0, cc = 0;
-    return walk_dense_run<CAP, WALK_STACK, WALK_LIST>(tv, nc, D, nullptr, lst, nent, qx, qy, qz, r, sp, sc, cc) ==
-           WalkEnd::done;
+    const bool ok =
+        walk_dense_run<CAP, WALK_STACK, WALK_LIST>(tv, nc, D, nullptr, lst, nent, qx, qy, qz, r, sp, sc, cc) ==
+        WalkEnd::done;
+    __syncthreads();
+    cnt = sc + D.scnt[threadIdx.x];
+    return ok;
 }
 
 /* LDS of the radius-schedule and tight-radius kernels */
@@ -620,18 +630,6 @@ __global__ __launch_bounds__(64) void k_point_tight(TreeView tv, uint32_t nq, co
         }
     }
     if (valid) rt_out[q] = ok ? r : 0.0;
-}
-
-/* |S(r)| of each query from the include-list entries [e0, e1) */
-__device__ __forceinline__ uint32_t list_count(const ListEnt *lst, uint32_t e0, uint32_t e1)
-{
-    const int lane = threadIdx.x;
-    uint32_t s = 0;
-    for (uint32_t e = e0; e < e1; ++e) {
-        const ListEnt le = lst[e];
-        if ((le.mask >> lane) & 1ull) s += le.end - le.begin;
-    }
-    return s;
 }
 
 __global__ void k_ovf_compact(const uint32_t *__restrict__ ovf, const uint32_t *__restrict__ pos, uint32_t n,
@@ -799,21 +797,13 @@ __device__ __forceinline__ void merge64(double &bd, uint32_t &bi, double cd, uin
 }
 
 /* Selection scratch of the query being selected: its list entries (sent),
- * and either the candidate prefix over them (spref, general path) or up to
- * 64 staged candidates (sd, si) waiting to be sorted (leaf path). */
+ * the marks of first candidate positions (start) and up to 64 staged
+ * candidates (sd, si) waiting to be sorted. */
 template <int CAP> struct SelectScratch {
-    union {
-        uint16_t sent[CAP];
-        uint8_t start[64];
-    };
-    union {
-        uint32_t spref[CAP + 1];
-        struct {
-            double sd[64];
-            uint32_t si[64];
-        };
-        uint2 ent[64];
-    };
+    uint16_t sent[CAP];
+    uint8_t start[64];
+    double sd[64];
+    uint32_t si[64];
 };
 /* LDS of one select wave: node cache, the include list of the walk and the
  * selection scratch (sharing its space with the walk's stack) */
@@ -825,7 +815,7 @@ template <int CAP> struct SelectLds {
         DenseWalkLds dw;
     };
 };
-static_assert(sizeof(SelectLds<256>) <= 6144, "a K <= 32 select wave should fit 6 KB of LDS");
+static_assert(sizeof(SelectLds<ListCap<32>::value>) <= 6144, "a K <= 32 select wave should fit 6 KB of LDS");
 /* the few waves that walk in several fills keep the walk's stack beside the
  * selection's arrays, so the walk resumes after each selection */
 template <int CAP> struct ResumeLds {
@@ -852,16 +842,15 @@ __device__ __forceinline__ void feed64(double d, uint32_t id, double &bd, uint32
 
 /* Feed the candidates of query bit j of the recorded list to the running
  * best-64 (bd, bi).  One ballot pass over the list finds the query's entries
- * (their ids to sent; the first 64 as (begin, end) to ent).  With at most 64
- * entries (nearly every query) their candidates are laid out densely over
- * the lanes, 64 per round: a prefix sum of the entry sizes gives each
- * entry's first position, a ballot over the marked first positions gives the
- * entry of each position, and the entry's first particle comes from its
- * lane.  The first 64 candidates within T are sorted at once; later ones
- * that can still enter the kq best (below the current kq-th key) are staged
- * in LDS and sorted and merged 64 at a time.  A query with more entries
- * takes its candidates 64 at a time by binary search over the entries'
- * candidate prefix instead. */
+ * (their list positions to sent).  They are expanded 64 entries at a time
+ * (nearly every query has fewer): lane r reads entry r's particle range from
+ * the node array, a prefix sum of the sizes gives each entry's first
+ * candidate position, and the candidates are laid out densely over the
+ * lanes, 64 per round (a ballot over the marked first positions gives the
+ * entry of each position; the entry's first particle comes from its lane).
+ * The first 64 candidates within T are sorted at once; later ones that can
+ * still enter the kq best (below the current kq-th key) are staged in LDS
+ * and sorted and merged 64 at a time. */
 template <int K, class LDS>
 __device__ __forceinline__ void select_list(const TreeView &tv, LDS &L, uint32_t nent, int j, double qx,
                                             double qy, double qz, double T, double &bd, uint32_t &bi, bool &first)
@@ -875,38 +864,29 @@ __device__ __forceinline__ void select_list(const TreeView &tv, LDS &L, uint32_t
     for (uint32_t e0 = 0; e0 < nent; e0 += 64) {
         const uint32_t e = e0 + (uint32_t)lane;
         bool has = false;
-        uint32_t b = 0, en = 0;
-        if (e < nent) {
-            const ListEnt le = L.lst[e];
-            has = (le.mask >> j) & 1ull;
-            b = le.begin;
-            en = le.end;
-        }
+        if (e < nent) has = (L.lst[e].mask() >> j) & 1ull;
         const uint64_t bm = __ballot(has);
-        const uint32_t rk = M + mbcnt64(bm);
-        if (has) {
-            S.sent[rk] = (uint16_t)e;
-            if (rk < 64u) S.ent[rk] = make_uint2(b, en);
-        }
+        if (has) S.sent[M + mbcnt64(bm)] = (uint16_t)e;
         M += (uint32_t)__popcll(bm);
     }
     __syncthreads();
-    if (M <= 64u) {
-        /* lane r < M holds entry r: its size, first candidate position p0
-         * and base (particle index = base + position, modulo 2^32); node
-         * ranges are never empty */
+    uint32_t ns = 0; /* staged candidates */
+    for (uint32_t g0 = 0; g0 < M; g0 += 64) {
+        /* lane r < Mg holds entry g0 + r: its first particle b0, size, first
+         * candidate position p0 and base (particle index = base + position,
+         * modulo 2^32); node ranges are never empty */
+        const uint32_t Mg = min(M - g0, 64u);
         uint32_t b0 = 0, sz = 0;
-        if ((uint32_t)lane < M) {
-            const uint2 en = S.ent[lane];
-            b0 = en.x;
-            sz = en.y - en.x;
+        if ((uint32_t)lane < Mg) {
+            const NodeTop t = tv.ntop[L.lst[S.sent[g0 + lane]].node];
+            b0 = t.begin;
+            sz = t.end - t.begin;
         }
         const uint32_t incl = wave_incl_scan(sz);
         const uint32_t C = lane_u32(incl, 63);
         const uint32_t p0 = incl - sz;
         const uint32_t base = b0 - p0;
-        const bool starts = (uint32_t)lane < M && sz > 0u;
-        uint32_t ns = 0; /* staged candidates */
+        const bool starts = (uint32_t)lane < Mg && sz > 0u;
         for (uint32_t k0 = 0; k0 < C; k0 += 64) {
             __syncthreads();
             S.start[lane] = 0;
@@ -918,7 +898,7 @@ __device__ __forceinline__ void select_list(const TreeView &tv, LDS &L, uint32_t
             /* the entry holding position k0 + lane: the entries starting at
              * or below it, less one */
             int rk = (int)(before + mbcnt64(sm) + (uint32_t)((sm >> lane) & 1ull)) - 1;
-            rk = rk < 0 ? 0 : (rk >= (int)M ? (int)M - 1 : rk);
+            rk = rk < 0 ? 0 : (rk >= (int)Mg ? (int)Mg - 1 : rk);
             const uint32_t pb = (uint32_t)__shfl((int)base, rk, 64);
             const uint32_t c = k0 + (uint32_t)lane;
             double d = inf;
@@ -966,65 +946,11 @@ __device__ __forceinline__ void select_list(const TreeView &tv, LDS &L, uint32_t
             }
             ns = ns + nv - 64u;
         }
-        if (ns > 0) {
-            __syncthreads();
-            const bool h = (uint32_t)lane < ns;
-            feed64(h ? S.sd[lane] : inf, h ? S.si[lane] : PLAN_NONE, bd, bi, first);
-        }
-        __syncthreads();
-        return;
     }
-    /* general path: the candidate prefix over the query's entries */
-    uint32_t np = 0;
-    for (uint32_t x = (uint32_t)lane; x < M; x += 64) np += L.lst[S.sent[x]].end - L.lst[S.sent[x]].begin;
-    const uint32_t np_in = wave_incl_scan(np);
-    const uint32_t C = lane_u32(np_in, 63);
-    {
-        /* lane l holds the entries l, l + 64, ...: prefix of its own entries
-         * in lane order first, then the chunks in turn */
+    if (ns > 0) {
         __syncthreads();
-        uint32_t run = 0;
-        for (uint32_t x0 = 0; x0 < M; x0 += 64) {
-            const uint32_t x = x0 + (uint32_t)lane;
-            const uint32_t sz = x < M ? L.lst[S.sent[x]].end - L.lst[S.sent[x]].begin : 0u;
-            const uint32_t in = wave_incl_scan(sz);
-            if (x < M) S.spref[x] = run + in - sz;
-            run += lane_u32(in, 63);
-        }
-        if (lane == 0) S.spref[M] = C;
-    }
-    __syncthreads();
-    uint32_t steps = 1;
-    while (steps * 2 < M) steps *= 2;
-    for (uint32_t base = 0; base < C; base += 64) {
-        const uint32_t c = base + lane;
-        double d = inf;
-        uint32_t id = PLAN_NONE;
-        if (c < C) {
-            /* largest lo < M with spref[lo] <= c (spref ascending, spref[0] = 0) */
-            uint32_t lo = 0;
-            for (uint32_t step = steps; step > 0; step >>= 1) {
-                const uint32_t t = lo + step;
-                lo = (t < M && S.spref[t] <= c) ? t : lo;
-            }
-            const uint32_t p = L.lst[S.sent[lo]].begin + (c - S.spref[lo]);
-            const double4 x = tv.pm[p];
-            const double dd = dist2(qx, qy, qz, x.x, x.y, x.z);
-            if (dd <= T) {
-                d = dd;
-                id = p;
-            }
-        }
-        if (!first) {
-            const double kd = lane_f64(bd, (int)tv.kq - 1);
-            const uint32_t ki = lane_u32(bi, (int)tv.kq - 1);
-            if (id != PLAN_NONE && !key_less(d, id, kd, ki)) {
-                d = inf;
-                id = PLAN_NONE;
-            }
-        }
-        if (!__ballot(id != PLAN_NONE)) continue;
-        feed64(d, id, bd, bi, first);
+        const bool h = (uint32_t)lane < ns;
+        feed64(h ? S.sd[lane] : inf, h ? S.si[lane] : PLAN_NONE, bd, bi, first);
     }
     __syncthreads();
 }
@@ -1197,23 +1123,51 @@ __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, con
          * (src/tree.hpp:3206-3223): the first round walks every query at r0;
          * each later round walks only the queries still short of kq
          * candidates, at their doubled radius, after taking their bits out of
-         * the previous round's entries, so the list holds each query's
-         * entries of its last round. */
-        uint64_t U = __ballot(valid);
+         * the previous round's entries (and dropping entries left with no
+         * query), so the list holds each query's entries of its last round.
+         * When such a round fills the list, the wave starts the list again
+         * and walks every query at its current radius in each round. */
+        const uint64_t V = __ballot(valid);
+        uint64_t U = V;
+        bool whole = false;
         nent = 0;
         for (int round = 0; round < 2200; ++round) {
+            if (whole) nent = 0;
             const uint32_t e0 = nent;
-            if (!walk_dense<CAP>(tv, L.nc, L.dw, L.lst, nent, qx, qy, qz, r, U)) {
+            const uint64_t W = whole ? V : U;
+            uint32_t sc = 0;
+            if (!walk_dense<CAP>(tv, L.nc, L.dw, L.lst, nent, qx, qy, qz, r, W, sc)) {
+                if (!whole && round > 0) {
+                    whole = true;
+                    continue;
+                }
                 if (lane == 0) ovf[wv] = 1u;
                 return;
             }
             __syncthreads();
-            const uint32_t sc = list_count(L.lst, e0, nent);
-            const bool need = ((U >> lane) & 1ull) && sc < tv.kq;
+            const bool need = ((W >> lane) & 1ull) && sc < tv.kq;
             const uint64_t N = __ballot(need);
             if (!N) break;
-            for (uint32_t e = e0 + (uint32_t)lane; e < nent; e += 64) L.lst[e].mask &= ~N;
-            __syncthreads();
+            if (!whole) {
+                /* in place: an entry only moves down, past those already read */
+                uint32_t w = e0;
+                for (uint32_t x0 = e0; x0 < nent; x0 += 64) {
+                    const uint32_t e = x0 + (uint32_t)lane;
+                    ListEnt le{0u, 0u, 0u};
+                    if (e < nent) {
+                        le = L.lst[e];
+                        le.mlo &= ~(uint32_t)N;
+                        le.mhi &= ~(uint32_t)(N >> 32);
+                    }
+                    const bool kp = (le.mlo | le.mhi) != 0u;
+                    const uint64_t keep = __ballot(kp);
+                    __syncthreads();
+                    if (kp) L.lst[w + mbcnt64(keep)] = le;
+                    w += (uint32_t)__popcll(keep);
+                }
+                nent = w;
+                __syncthreads();
+            }
             if (need) r = __dmul_rn(r, 2.0);
             U = N;
         }
