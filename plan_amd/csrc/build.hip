@@ -305,7 +305,8 @@ __global__ void k_gather_particles(const uint4 *__restrict__ skeys, const uint32
                                    const int32_t *__restrict__ pprop, const uint32_t *__restrict__ pid,
                                    const double *__restrict__ mass_of_type, uint32_t ntypes, double4 *__restrict__ pm,
                                    double4 *__restrict__ va, uint32_t *__restrict__ oid, uint32_t *__restrict__ sinkcnt,
-                                   uint32_t *__restrict__ sinkoff, uint4 *__restrict__ ukeys)
+                                   uint32_t *__restrict__ sinkoff, uint4 *__restrict__ ukeys, double m0,
+                                   uint32_t *__restrict__ nonuniform)
 {
     uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
@@ -335,6 +336,7 @@ __global__ void k_gather_particles(const uint4 *__restrict__ skeys, const uint32
         v.z = __ddiv_rn(vz, ms);
     }
     pm[r] = make_double4(p.x, p.y, p.z, m);
+    if (m != m0) *nonuniform = 1u; /* any thread may set it; all write 1 */
     va[r] = make_double4(v.x, v.y, v.z, p.w);
     oid[r] = pid[src];
     sinkcnt[r] = e - s;
@@ -684,17 +686,22 @@ void dev_build_tree(DevTree &t, const DevParticles &ps, const plan_derived &d, b
     t.sinkcnt = a.get_t<uint32_t>("sinkcnt", n);
     t.sinkoff = a.get_t<uint32_t>("sinkoff", n);
     t.keys = a.get_t<uint4>("ukeys", n);
+    HIPCHK(hipMemsetAsync(tot + 1, 0, 8, s));
     hipLaunchKernelGGL(k_gather_particles, dim3(ceil_div(n, BLK)), dim3(BLK), 0, s, skeys, start, n, T, ps.pos,
                        ps.vel, ps.prop, ps.id, mot, (uint32_t)PLAN_MAX_TYPES, t.pm, t.va, t.oid, t.sinkcnt,
-                       t.sinkoff, t.keys);
+                       t.sinkoff, t.keys, d.mass_per_particle[0], tot + 2);
     LAUNCHCHK();
     HIPCHK(hipMemsetAsync(t.rho, 0, (size_t)n * 8, s));
-    t.num_sinks = 0;
     if (n != T) {
-        HIPCHK(hipMemsetAsync(tot + 1, 0, 4, s));
         hipLaunchKernelGGL(k_count_sinks, dim3(ceil_div(n, BLK)), dim3(BLK), 0, s, t.sinkcnt, n, tot + 1);
         LAUNCHCHK();
-        t.num_sinks = read_u32(tot + 1, s);
+    }
+    {
+        uint32_t h[2];
+        HIPCHK(hipMemcpyAsync(h, tot + 1, sizeof(h), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        t.num_sinks = h[0];
+        t.mass_uniform = h[1] == 0u;
     }
 
     /* node array straight from the sorted keys (grow-only node arrays) */

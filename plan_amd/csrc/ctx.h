@@ -16,6 +16,7 @@ struct DevTree {
     uint32_t num_nodes = 0;
     uint32_t num_leaves = 0;
     uint32_t num_sinks = 0;
+    bool mass_uniform = false;    /* every tree particle has species 0's mass */
     int depth = 0;
     double root_center[3] = {0, 0, 0};
     double half_width = 0;
@@ -129,6 +130,7 @@ struct TreeView {
     /* msum[k]: k masses m0 summed one by one from zero (k <= 64), or null */
     const double *msum = nullptr;
     double m0 = 0;
+    bool mass_uniform = false; /* every tree particle's mass is m0 */
     /* densities to recompute with the C library's pow: count, capacity, and
      * per entry the output index, x and the mass sum (null: none kept) */
     uint32_t *pw_count = nullptr;

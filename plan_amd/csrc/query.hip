@@ -47,6 +47,7 @@ TreeView tree_view(const DevTree &t, const plan_derived &d)
     v.cell_volume = d.cell_volume;
     v.margin_abs = 2.5 * t.epsilon;
     v.list_limit = t.list_limit;
+    v.mass_uniform = t.mass_uniform;
     return v;
 }
 
@@ -674,6 +675,21 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
     return v;
 }
 
+/* inclusive sum over the 64 lanes with DPP: shifts by 1, 2, 4, 8 within
+ * rows of 16 lanes (lanes shifted in from outside the row add 0), then the
+ * last lane of row 0 added to row 1 and of row 2 to row 3 (row_bcast:15),
+ * and the last lane of row 1 to rows 2 and 3 (row_bcast:31) */
+__device__ __forceinline__ uint32_t wave_incl_sum_dpp(uint32_t v)
+{
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);
+    return v;
+}
+
 /* lane l reads lane l ^ J: DPP for J = 1, 2 (quad permutes) and 8 (row
  * rotate by 8), swizzle for 4 and 16, bpermute for 32 */
 template <int J> __device__ __forceinline__ uint32_t xor_lane(uint32_t v)
@@ -882,7 +898,7 @@ __device__ __forceinline__ void select_list(const TreeView &tv, LDS &L, uint32_t
             b0 = t.begin;
             sz = t.end - t.begin;
         }
-        const uint32_t incl = wave_incl_scan(sz);
+        const uint32_t incl = wave_incl_sum_dpp(sz);
         const uint32_t C = lane_u32(incl, 63);
         const uint32_t p0 = incl - sz;
         const uint32_t base = b0 - p0;
@@ -1012,8 +1028,10 @@ __device__ __forceinline__ void finish_query(const TreeView &tv, double bd, uint
         const uint64_t in = __ballot(lane > kq / 2 && lane < kq && !(bd > m));
         const int is = in ? 63 - __clzll((long long)in) : kq / 2;
         const double tm = lane_f64(bd, is);
-        const double w = (lane <= is) ? tv.pm[bi].w : 0.0;
-        if (tv.msum && !__ballot(lane <= is && w != tv.m0)) {
+        if (tv.msum && tv.mass_uniform) {
+            /* every mass is m0: the sum of i* + 1 of them is the table's */
+            rho = tv.msum[is + 1];
+        } else if (const double w = (lane <= is) ? tv.pm[bi].w : 0.0; tv.msum && !__ballot(lane <= is && w != tv.m0)) {
             /* i* + 1 equal masses: their sum in that order is the table's */
             rho = tv.msum[is + 1];
         } else {
