@@ -112,6 +112,17 @@ static void sort_by_distance_buckets(std::vector<PD> &v, std::vector<PD> &tmp, s
 
 namespace {
 
+/* Loops over a clump's members visit the 80-byte member records in
+ * distance order, scattered over the clump's record ranges: the record
+ * PF_AHEAD steps ahead is prefetched (both cache lines it may span). */
+constexpr size_t PF_AHEAD = 12;
+static inline void prefetch_rec(const MemberRec *P, uint32_t h)
+{
+    const char *a = reinterpret_cast<const char *>(P + h);
+    __builtin_prefetch(a);
+    __builtin_prefetch(a + 64);
+}
+
 struct Ops {
     const MemberRec *P;
     const ClumpSettings &cs;
@@ -142,8 +153,10 @@ struct Ops {
     {
         c.total_mass = 0;
         for (int k = 0; k < 3; ++k) c.com[k] = c.vel[k] = 0;
-        for (uint32_t h : c.indices) {
-            const MemberRec &r = P[h];
+        const size_t n = c.indices.size();
+        for (size_t i = 0; i < n; ++i) {
+            if (i + PF_AHEAD < n) prefetch_rec(P, c.indices[i + PF_AHEAD]);
+            const MemberRec &r = P[c.indices[i]];
             c.total_mass += r.m;
             c.com[0] += r.x * r.m;
             c.com[1] += r.y * r.m;
@@ -178,12 +191,26 @@ struct Ops {
         if (c.inner > 0 && c.outer / c.inner > 2.5) c.one10th = std::min(c.outer, c.two_sigma);
         else c.one10th = std::min(c.one10th, c.two_sigma);
     }
+    /* v[i].second = distance of member v[i].first to c */
+    void set_distances(std::vector<PD> &v, const double *c) const
+    {
+        const size_t n = v.size();
+        for (size_t i = 0; i < n; ++i) {
+            if (i + PF_AHEAD < n) prefetch_rec(P, v[i + PF_AHEAD].first);
+            v[i].second = dist_to(v[i].first, c);
+        }
+    }
     /* SortParticles (src/tree.hpp:4446-4512) */
     void sort_particles(HClump &c, bool nearly_sorted = false) const
     {
         kinematics(c);
         c.particles.reserve(c.particles.size() + c.indices.size());
-        for (uint32_t h : c.indices) c.particles.push_back(PD(h, dist_to(h, c.com)));
+        const size_t n = c.indices.size();
+        for (size_t i = 0; i < n; ++i) {
+            if (i + PF_AHEAD < n) prefetch_rec(P, c.indices[i + PF_AHEAD]);
+            const uint32_t h = c.indices[i];
+            c.particles.push_back(PD(h, dist_to(h, c.com)));
+        }
         sort_by_distance(c.particles, nearly_sorted);
         for (size_t i = 0; i < c.particles.size(); ++i) c.indices[i] = c.particles[i].first;
         radii(c);
@@ -304,7 +331,7 @@ struct Ops {
         a.total_mass = tot;
         const size_t na = a.particles.size();
         a.particles.insert(a.particles.end(), b.particles.begin(), b.particles.end());
-        for (auto &it : a.particles) it.second = dist_to(it.first, a.com);
+        set_distances(a.particles, a.com);
         sort_union(a.particles, na);
         a.indices.resize(a.particles.size());
         for (size_t i = 0; i < a.particles.size(); ++i) a.indices[i] = a.particles[i].first;
@@ -338,7 +365,7 @@ struct Ops {
             a.total_mass = tot;
             a.particles.insert(a.particles.end(), b->particles.begin(), b->particles.end());
         }
-        for (auto &it : a.particles) it.second = dist_to(it.first, a.com);
+        set_distances(a.particles, a.com);
         if (!sort_union(a.particles, na, true)) {
             a = a0;
             for (const HClump *b : bs) {
@@ -364,6 +391,7 @@ struct Ops {
         const double G = cs.G, sh = cs.q * cs.Omega;
         size_t i = c.particles.size() - 1;
         while (i != 0) {
+            if (i > PF_AHEAD) prefetch_rec(P, c.particles[i - PF_AHEAD].first);
             const MemberRec &q = P[c.particles[i].first];
             double tm = c.total_mass - q.m;
             double tc[3], tv[3];
@@ -405,7 +433,10 @@ struct Ops {
         double quarter = 0.25 * c.hill;
         double hu = ((c.total_mass * c.hill) * c.hill) * cs.Omega;
         double s1 = cs.q * cs.Omega;
-        for (const auto &it : c.particles) {
+        const size_t n = c.particles.size();
+        for (size_t i = 0; i < n; ++i) {
+            const auto &it = c.particles[i];
+            if (i + PF_AHEAD < n) prefetch_rec(P, c.particles[i + PF_AHEAD].first);
             const MemberRec &q = P[it.first];
             const double *qp = &q.x, *qv = &q.vx;
             double dr[3], dv[3], j[3];
