@@ -227,7 +227,21 @@ struct MortonFrame {
     double boxmin[3], scale[3];
 };
 
-__global__ void k_morton(const double4 *__restrict__ pos, uint32_t T, MortonFrame f, uint4 *__restrict__ keys)
+/* A particle's fields in one 64-byte record, so that the gather in key
+ * order (random positions) reads one cache line per particle instead of one
+ * per field array. */
+struct __align__(64) GatherRec {
+    double x, y, z, w;
+    double vx, vy, vz;
+    int32_t prop;
+    uint32_t id;
+};
+static_assert(sizeof(GatherRec) == 64, "one 64-byte line per particle");
+
+/* keys, and the particles packed into gather records (sequential pass) */
+__global__ void k_morton(const double4 *__restrict__ pos, const double4 *__restrict__ vel,
+                         const int32_t *__restrict__ prop, const uint32_t *__restrict__ pid, uint32_t T, MortonFrame f,
+                         uint4 *__restrict__ keys, GatherRec *__restrict__ rec)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= T) return;
@@ -236,6 +250,18 @@ __global__ void k_morton(const double4 *__restrict__ pos, uint32_t T, MortonFram
     uint32_t iy = double_to_grid(__dmul_rn(__dsub_rn(p.y, f.boxmin[1]), f.scale[1]));
     uint32_t iz = double_to_grid(__dmul_rn(__dsub_rn(p.z, f.boxmin[2]), f.scale[2]));
     keys[i] = morton_key(ix, iy, iz, i);
+    const double4 v = vel[i];
+    GatherRec r;
+    r.x = p.x;
+    r.y = p.y;
+    r.z = p.z;
+    r.w = p.w;
+    r.vx = v.x;
+    r.vy = v.y;
+    r.vz = v.z;
+    r.prop = prop[i];
+    r.id = pid[i];
+    rec[i] = r;
 }
 
 /* Keys sorted stably by their high 64 bits (words y, z) are in input order
@@ -301,9 +327,7 @@ __global__ void k_count_sinks(const uint32_t *__restrict__ sinkcnt, uint32_t n, 
  * one sink particle (first of the run represents; mass and momentum summed
  * in run order) */
 __global__ void k_gather_particles(const uint4 *__restrict__ skeys, const uint32_t *__restrict__ start, uint32_t n,
-                                   uint32_t T, const double4 *__restrict__ ppos, const double4 *__restrict__ pvel,
-                                   const int32_t *__restrict__ pprop, const uint32_t *__restrict__ pid,
-                                   const double *__restrict__ mass_of_type, uint32_t ntypes, double4 *__restrict__ pm,
+                                   uint32_t T, const GatherRec *__restrict__ rec, const double *__restrict__ mass_of_type, uint32_t ntypes, double4 *__restrict__ pm,
                                    double4 *__restrict__ va, uint32_t *__restrict__ oid, uint32_t *__restrict__ sinkcnt,
                                    uint32_t *__restrict__ sinkoff, uint4 *__restrict__ ukeys, double m0,
                                    uint32_t *__restrict__ nonuniform)
@@ -314,20 +338,20 @@ __global__ void k_gather_particles(const uint4 *__restrict__ skeys, const uint32
     uint32_t e = (r + 1 < n) ? start[r + 1] : T;
     uint4 k = skeys[s];
     uint32_t src = k.w;
-    double4 p = ppos[src];
-    double4 v = pvel[src];
-    int32_t ty = pprop[src];
+    const GatherRec g = rec[src];
+    double4 p = make_double4(g.x, g.y, g.z, g.w);
+    double4 v = make_double4(g.vx, g.vy, g.vz, 0.0);
+    int32_t ty = g.prop;
     double m = mass_of_type[(ty >= 0 && (uint32_t)ty < ntypes) ? ty : 0];
     if (e - s > 1) {
         double ms = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
         for (uint32_t j = s; j < e; ++j) {
-            uint32_t q = skeys[j].w;
-            int32_t tq = pprop[q];
+            const GatherRec &gq = rec[skeys[j].w];
+            int32_t tq = gq.prop;
             double mq = mass_of_type[(tq >= 0 && (uint32_t)tq < ntypes) ? tq : 0];
-            double4 vq = pvel[q];
-            vx = __dadd_rn(vx, __dmul_rn(mq, vq.x));
-            vy = __dadd_rn(vy, __dmul_rn(mq, vq.y));
-            vz = __dadd_rn(vz, __dmul_rn(mq, vq.z));
+            vx = __dadd_rn(vx, __dmul_rn(mq, gq.vx));
+            vy = __dadd_rn(vy, __dmul_rn(mq, gq.vy));
+            vz = __dadd_rn(vz, __dmul_rn(mq, gq.vz));
             ms = __dadd_rn(ms, mq);
         }
         m = ms;
@@ -338,7 +362,7 @@ __global__ void k_gather_particles(const uint4 *__restrict__ skeys, const uint32
     pm[r] = make_double4(p.x, p.y, p.z, m);
     if (m != m0) *nonuniform = 1u; /* any thread may set it; all write 1 */
     va[r] = make_double4(v.x, v.y, v.z, p.w);
-    oid[r] = pid[src];
+    oid[r] = g.id;
     sinkcnt[r] = e - s;
     sinkoff[r] = s;
     ukeys[r] = make_uint4(k.x, k.y, k.z, s);
@@ -637,7 +661,9 @@ void dev_build_tree(DevTree &t, const DevParticles &ps, const plan_derived &d, b
     /* keys and sort */
     uint4 *k0 = a.get_t<uint4>("keys_a", T);
     uint4 *k1 = a.get_t<uint4>("keys_b", T);
-    hipLaunchKernelGGL(k_morton, dim3(ceil_div(T, BLK)), dim3(BLK), 0, s, ps.pos, T, f, k0);
+    GatherRec *grec = a.get_t<GatherRec>("gather_rec", T);
+    hipLaunchKernelGGL(k_morton, dim3(ceil_div(T, BLK)), dim3(BLK), 0, s, ps.pos, ps.vel, ps.prop, ps.id, T, f, k0,
+                       grec);
     LAUNCHCHK();
     /* sort by the high 64 bits, then each run of equal high halves by the
      * low word */
@@ -687,8 +713,8 @@ void dev_build_tree(DevTree &t, const DevParticles &ps, const plan_derived &d, b
     t.sinkoff = a.get_t<uint32_t>("sinkoff", n);
     t.keys = a.get_t<uint4>("ukeys", n);
     HIPCHK(hipMemsetAsync(tot + 1, 0, 8, s));
-    hipLaunchKernelGGL(k_gather_particles, dim3(ceil_div(n, BLK)), dim3(BLK), 0, s, skeys, start, n, T, ps.pos,
-                       ps.vel, ps.prop, ps.id, mot, (uint32_t)PLAN_MAX_TYPES, t.pm, t.va, t.oid, t.sinkcnt,
+    hipLaunchKernelGGL(k_gather_particles, dim3(ceil_div(n, BLK)), dim3(BLK), 0, s, skeys, start, n, T, grec, mot,
+                       (uint32_t)PLAN_MAX_TYPES, t.pm, t.va, t.oid, t.sinkcnt,
                        t.sinkoff, t.keys, d.mass_per_particle[0], tot + 2);
     LAUNCHCHK();
     HIPCHK(hipMemsetAsync(t.rho, 0, (size_t)n * 8, s));
