@@ -1357,15 +1357,29 @@ void run_clump_pipeline(HostMembers &hm, const ClumpSettings &cs, ClumpServices 
     const uint32_t nc = (uint32_t)hm.cpeak.size();
     /* naive clumps in key order, members in the serial walk's discovery order
      * (src/tree.hpp:3766-3809) */
-    for (uint32_t c = 0; c < nc; ++c) {
-        auto it = cl.emplace_hint(cl.end(), hm.cpeak[c], HClump());
-        HClump &h = it->second;
-        h.peak_index = hm.cpeak[c];
-        uint32_t s = hm.cstart[c], e = hm.cstart[c + 1];
-        h.indices.resize(e - s);
-        for (uint32_t i = s; i < e; ++i) h.indices[i - s] = i;
+    {
+        /* the map in key order, then each clump's member list and
+         * SortParticles in parallel, the largest first */
+        std::vector<HClump *> hv(nc);
+        for (uint32_t c = 0; c < nc; ++c) {
+            auto it = cl.emplace_hint(cl.end(), hm.cpeak[c], HClump());
+            it->second.peak_index = hm.cpeak[c];
+            hv[c] = &it->second;
+        }
+        std::vector<uint32_t> order(nc);
+        for (uint32_t c = 0; c < nc; ++c) order[c] = c;
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+            return hm.cstart[x + 1] - hm.cstart[x] > hm.cstart[y + 1] - hm.cstart[y];
+        });
+        parallel_for((long)nc, [&](long k) {
+            const uint32_t c = order[k];
+            HClump &h = *hv[c];
+            const uint32_t s = hm.cstart[c], e = hm.cstart[c + 1];
+            h.indices.resize(e - s);
+            for (uint32_t i = s; i < e; ++i) h.indices[i - s] = i;
+            ops.sort_particles(h);
+        });
     }
-    for_each_clump_parallel(cl, [&](HClump &h) { ops.sort_particles(h); });
     std::vector<uint32_t> del;
     if (cs.diffuse < 0.525)
         for (uint32_t k : dispersed_keys(cl, cs.diffuse, false)) cl.erase(k);
