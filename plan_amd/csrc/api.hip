@@ -13,7 +13,10 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <atomic>
 #include <condition_variable>
+#include <memory>
+#include <thread>
 #include <exception>
 #include <mutex>
 #include <vector>
@@ -54,6 +57,8 @@ struct plan_ctx {
     uint32_t density_pow_checks = 0; /* densities checked against the C library's pow */
     uint32_t density_pow_fixes = 0;  /* densities redone with the C library's pow */
     hipEvent_t ev[8];
+    /* the member records reach the host in chunks, one event each */
+    hipEvent_t rec_ev[8];
 };
 
 namespace plan {
@@ -400,6 +405,7 @@ int plan_ctx_create(plan_ctx **out, int device)
         HIPCHK(hipSetDevice(device));
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
+        for (auto &e : c->rec_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         if (scratch_init(device) != PLAN_OK) throw std::runtime_error("scratch symbol address");
     } catch (const std::exception &e) {
         delete c;
@@ -420,6 +426,7 @@ void plan_ctx_destroy(plan_ctx *c)
     free_particles(c->cps);
     free_particles(c->saved);
     for (auto &e : c->ev) (void)hipEventDestroy(e);
+    for (auto &e : c->rec_ev) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -736,6 +743,14 @@ int plan_find_planetesimals(plan_ctx *c, plan_catalog **out)
         /* member table to the host */
         hipStream_t s = c->stream;
         HostMembers hm;
+        std::thread rec_waiter;
+        struct Join {
+            std::thread &t;
+            ~Join()
+            {
+                if (t.joinable()) t.join();
+            }
+        } join_waiter{rec_waiter};
         const uint32_t na = c->hop.na, nc = c->hop.nclumps;
         {
             /* pinned, grow-only staging for the member table */
@@ -757,14 +772,43 @@ int plan_find_planetesimals(plan_ctx *c, plan_catalog **out)
             hipLaunchKernelGGL(k_clump_peaks, dim3(ceil_div(nc, 256)), dim3(256), 0, s, c->hop.order, c->hop.cstart,
                                nc, c->hop.a_idx, cpk);
             LAUNCHCHK();
-            HIPCHK(hipMemcpyAsync(hm.rec.data(), c->hop.members, (size_t)na * sizeof(MemberRec), hipMemcpyDeviceToHost,
-                                  s));
             HIPCHK(hipMemcpyAsync(hm.sinkcnt.data(), c->hop.msink, (size_t)na * 4, hipMemcpyDeviceToHost, s));
             HIPCHK(hipMemcpyAsync(hm.cstart.data(), c->hop.cstart, (size_t)nc * 4, hipMemcpyDeviceToHost, s));
             HIPCHK(hipMemcpyAsync(hm.cpeak.data(), cpk, (size_t)nc * 4, hipMemcpyDeviceToHost, s));
             HIPCHK(hipMemcpyAsync(hm.sorted_tidx.data(), c->hop.a_idx, (size_t)na * 4, hipMemcpyDeviceToHost, s));
             HIPCHK(hipMemcpyAsync(hm.rec_of_sorted.data(), roa, (size_t)na * 4, hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
+            /* the records in chunks: the naive clumps' first pass starts on
+             * the clumps whose records have arrived while the rest stream */
+            constexpr uint32_t NCH = sizeof(c->rec_ev) / sizeof(c->rec_ev[0]);
+            const uint32_t step = (na + NCH - 1) / NCH;
+            std::vector<uint32_t> ends;
+            for (uint32_t k = 0; k < NCH && k * step < na; ++k) {
+                const uint32_t b = k * step, e = std::min(na, b + step);
+                HIPCHK(hipMemcpyAsync(hm.rec.data() + b, c->hop.members + b, (size_t)(e - b) * sizeof(MemberRec),
+                                      hipMemcpyDeviceToHost, s));
+                HIPCHK(hipEventRecord(c->rec_ev[k], s));
+                ends.push_back(e);
+            }
+            /* one thread waits for the chunks in turn and publishes how many
+             * records have arrived; the clump workers poll that count */
+            auto ready = std::make_shared<std::atomic<uint32_t>>(0u);
+            auto failed = std::make_shared<std::atomic<bool>>(false);
+            rec_waiter = std::thread([c, ends, ready, failed]() {
+                for (size_t k = 0; k < ends.size(); ++k) {
+                    if (hipEventSynchronize(c->rec_ev[k]) != hipSuccess) {
+                        failed->store(true);
+                        return;
+                    }
+                    ready->store(ends[k], std::memory_order_release);
+                }
+            });
+            hm.wait_records = [ready, failed](uint32_t upto) {
+                while (ready->load(std::memory_order_acquire) < upto) {
+                    if (failed->load()) throw std::runtime_error("member record copy to the host failed");
+                    std::this_thread::yield();
+                }
+            };
         }
         hm.cstart[nc] = na;
         if (c->tidx_map.size() < c->tree.n) c->tidx_map.resize(c->tree.n);

@@ -1359,7 +1359,9 @@ void run_clump_pipeline(HostMembers &hm, const ClumpSettings &cs, ClumpServices 
      * (src/tree.hpp:3766-3809) */
     {
         /* the map in key order, then each clump's member list and
-         * SortParticles in parallel, the largest first */
+         * SortParticles in parallel: in record order while the records
+         * still arrive (each clump waits for its own), else the largest
+         * first */
         std::vector<HClump *> hv(nc);
         for (uint32_t c = 0; c < nc; ++c) {
             auto it = cl.emplace_hint(cl.end(), hm.cpeak[c], HClump());
@@ -1368,17 +1370,20 @@ void run_clump_pipeline(HostMembers &hm, const ClumpSettings &cs, ClumpServices 
         }
         std::vector<uint32_t> order(nc);
         for (uint32_t c = 0; c < nc; ++c) order[c] = c;
-        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-            return hm.cstart[x + 1] - hm.cstart[x] > hm.cstart[y + 1] - hm.cstart[y];
-        });
+        if (!hm.wait_records)
+            std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+                return hm.cstart[x + 1] - hm.cstart[x] > hm.cstart[y + 1] - hm.cstart[y];
+            });
         parallel_for((long)nc, [&](long k) {
             const uint32_t c = order[k];
             HClump &h = *hv[c];
             const uint32_t s = hm.cstart[c], e = hm.cstart[c + 1];
+            if (hm.wait_records) hm.wait_records(e);
             h.indices.resize(e - s);
             for (uint32_t i = s; i < e; ++i) h.indices[i - s] = i;
             ops.sort_particles(h);
         });
+        if (hm.wait_records) hm.wait_records(hm.cstart[nc]);
     }
     std::vector<uint32_t> del;
     if (cs.diffuse < 0.525)

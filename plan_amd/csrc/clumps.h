@@ -4,6 +4,7 @@
 #include "hop.h"
 #include "../../include/plan_params.h"
 
+#include <functional>
 #include <map>
 #include <vector>
 
@@ -82,6 +83,9 @@ struct HostMembers {
     const uint32_t *tidx_map = nullptr;
     size_t tidx_map_n = 0;
     uint32_t rec_of_tidx(uint32_t tidx) const;
+    /* when set, records may still be arriving: wait_records(e) returns once
+     * records [0, e) are in rec */
+    std::function<void(uint32_t)> wait_records;
 };
 
 struct ClumpStats {
