@@ -913,9 +913,8 @@ __device__ __forceinline__ void select_list(const TreeView &tv, LDS &L, uint32_t
             const uint32_t before = (uint32_t)__popcll(__ballot(starts && p0 < k0));
             /* the entry holding position k0 + lane: the entries starting at
              * or below it, less one */
-            int rk = (int)(before + mbcnt64(sm) + (uint32_t)((sm >> lane) & 1ull)) - 1;
-            rk = rk < 0 ? 0 : (rk >= (int)Mg ? (int)Mg - 1 : rk);
-            const uint32_t pb = (uint32_t)__shfl((int)base, rk, 64);
+            const uint32_t rk = before + mbcnt64(sm) + (uint32_t)((sm >> lane) & 1ull) - 1u;
+            const uint32_t pb = (uint32_t)__shfl((int)base, (int)(rk & 63u), 64);
             const uint32_t c = k0 + (uint32_t)lane;
             double d = inf;
             uint32_t id = PLAN_NONE;
