@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libplan_hip.so")
 BUILD = os.path.join(HERE, "_build")
-SOURCES = ["scan_sort.hip", "build.hip", "query.hip", "hop.hip", "api.hip", "clumps.cpp", "boundness.hip", "lis.hip", "writers.hip", "analyses.hip"]
+SOURCES = ["scan_sort.hip", "build.hip", "query.hip", "hop.hip", "api.hip", "clumps.cpp", "boundness.hip", "lis.hip", "writers.hip", "analyses.hip", "host_tree.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PLAN_OFFLOAD_ARCH", "gfx950")
 

@@ -6,6 +6,7 @@
 #include "ctx.h"
 #include "hop.h"
 #include "boundness.h"
+#include "host_tree.h"
 
 #include <chrono>
 #include <cmath>
@@ -62,6 +63,10 @@ struct plan_ctx {
 };
 
 namespace plan {
+
+/* clump trees of at most this many centres are built and searched on the
+ * host (HostTree), larger ones on the device */
+constexpr uint32_t HOST_CLUMP_TREE_MAX = 65536;
 
 /* NumericalParameters after ReadLisFile (src/tree.hpp:1597-1612) and
  * CalculateNewParameters (src/global.cpp:28-47). */
@@ -277,6 +282,25 @@ struct Services : ClumpServices {
         uint32_t n = (uint32_t)com_peak.size();
         peaks.assign(queries.size(), {});
         if (n == 0) return;
+        if (n <= HOST_CLUMP_TREE_MAX) {
+            /* a few thousand clump centres: the same tree and searches on the
+             * host, without the device round trips */
+            HostTree ht;
+            ht.build(com_peak, c->tree.root_center, c->tree.half_width, c->d);
+            const long nq = (long)queries.size();
+#pragma omp parallel
+            {
+                std::vector<uint32_t> l;
+#pragma omp for schedule(dynamic, 16)
+                for (long qi = 0; qi < nq; ++qi) {
+                    const double4 &q = queries[qi];
+                    ht.ball(q.x, q.y, q.z, q.w, l);
+                    peaks[qi].reserve(l.size());
+                    for (uint32_t j : l) peaks[qi].push_back((uint32_t)ht.payload(j));
+                }
+            }
+            return;
+        }
         ensure_particles(c->cps, n);
         std::vector<int32_t> prop(n, 0);
         std::vector<uint32_t> ids(n);
