@@ -249,6 +249,17 @@ int plan_last_density_radius_ms(plan_ctx *ctx, double *ms);
  * waves, resumable multi-fill walks, frontier regrowth).  Results do not
  * change, only speed. */
 int plan_set_knn_list_limit(plan_ctx *ctx, uint32_t max_entries);
+/* Test hook, no C API counterpart in the reference: the merge passes' clump
+ * tree (BuildClumpTree's BuildTree over n points with the given root box,
+ * src/tree.hpp:5251-5284) built on the host, and RecursiveBallSearch
+ * (src/tree.hpp:3275-3293) for nq queries (x, y, z, radius).  pts4 holds
+ * (x, y, z, payload) per point; query q's results are the payloads of the
+ * particles found, in the walk's order, at out[offsets[q] .. offsets[q+1])
+ * (offsets has nq + 1 entries).  Returns PLAN_E_ARG when the results exceed
+ * cap (offsets[nq] then holds the count needed).  No device is used. */
+int plan_host_tree_balls(const plan_params *params, uint32_t n, const double *pts4, const double *root_center,
+                         double half_width, const double *queries4, uint32_t nq, uint32_t *offsets, double *out,
+                         uint32_t cap);
 /* num_hop for the next plan_find_planetesimals of the loaded snapshot (1..64).
  * The reference keeps H in its global parameters and its automatic choice
  * overwrites it (src/tree.hpp:3583-3586), so in a series the H a snapshot

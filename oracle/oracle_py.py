@@ -66,6 +66,8 @@ def lib():
         L.orc_knn_candidates.argtypes = [vp, _d, C.c_uint32, C.c_double, _u32, _d, C.c_uint32]
         L.orc_ball.restype = C.c_uint32
         L.orc_ball.argtypes = [vp, _d, C.c_double, _u32, C.c_uint32]
+        L.orc_point_tree_balls.restype = C.c_uint32
+        L.orc_point_tree_balls.argtypes = [vp, C.c_uint32, _d, _d, C.c_double, _d, C.c_uint32, _u32, _d, C.c_uint32]
         L.orc_find_planetesimals.argtypes = [vp]
         L.orc_density_vs_scale.restype = C.c_uint32
         L.orc_density_vs_scale.argtypes = [vp, C.c_uint32, _d, _d, P(dtype=np.uint64, flags="C_CONTIGUOUS"), _d]
@@ -114,6 +116,23 @@ def derive(params: dict, n: int) -> PlanDerived:
     d = PlanDerived()
     lib().orc_derive(C.byref(p), n, C.byref(d))
     return d
+
+
+def point_tree_balls(params: dict, pts4, root_center, half_width, queries4):
+    """BuildClumpTree's tree over pts4 (x, y, z, payload) with the given root
+    box and one ball search per query (x, y, z, r): per query the payloads
+    found, in traversal order."""
+    p = make_params(params)
+    pts4 = np.ascontiguousarray(pts4, dtype=np.float64).reshape(-1, 4)
+    q4 = np.ascontiguousarray(queries4, dtype=np.float64).reshape(-1, 4)
+    rc = np.ascontiguousarray(root_center, dtype=np.float64)
+    off = np.zeros(len(q4) + 1, np.uint32)
+    cap = 1 << 20
+    out = np.zeros(cap)
+    n = lib().orc_point_tree_balls(C.byref(p), len(pts4), pts4.ravel(), rc, float(half_width), q4.ravel(), len(q4),
+                                   off, out, cap)
+    assert n <= cap
+    return [out[off[i]:off[i + 1]].copy() for i in range(len(q4))]
 
 
 def make_ghosts(params: dict, pos, vel, dens, prop, ids):

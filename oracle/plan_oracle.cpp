@@ -1670,6 +1670,42 @@ uint32_t orc_ball(const orc_tree *h, const double *c, double r, uint32_t *out, u
     return (uint32_t)v.size();
 }
 
+/* BuildClumpTree's tree (src/tree.hpp:5251-5284: BuildTree over the given
+ * points with the given root box, no position check) and ball searches
+ * (3275-3299): for each query (x, y, z, r) the payloads (the points' density
+ * field) of the particles found, in traversal order, at out[offsets[q] ..
+ * offsets[q+1]).  Returns the total count (only cap entries are written). */
+uint32_t orc_point_tree_balls(const plan_params *pp, uint32_t n, const double *pts4, const double *root_center,
+                              double half_width, const double *q4, uint32_t nq, uint32_t *offsets, double *out,
+                              uint32_t cap)
+{
+    plan_derived d;
+    derive(pp, n, &d);
+    std::vector<Par> P(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        for (int k = 0; k < 3; ++k) { P[i].pos[k] = pts4[4 * i + k]; P[i].vel[k] = 0.0; }
+        P[i].density = pts4[4 * i + 3];
+        P[i].id = i;
+        P[i].property_index = 0;
+    }
+    Tree CT;
+    for (int k = 0; k < 3; ++k) CT.root_center[k] = root_center[k];
+    CT.half_width = half_width;
+    CT.build(pp, &d, P, false);
+    uint32_t k = 0;
+    std::vector<uint32_t> v;
+    for (uint32_t q = 0; q < nq; ++q) {
+        offsets[q] = k;
+        CT.ball(q4 + 4 * q, q4[4 * q + 3], v);
+        for (uint32_t j : v) {
+            if (k < cap) out[k] = CT.p[j].ath_density;
+            ++k;
+        }
+    }
+    offsets[nq] = k;
+    return k;
+}
+
 /* Run FindPlanetesimals on a built tree. */
 void orc_find_planetesimals(orc_tree *h)
 {

@@ -52,7 +52,7 @@ EXPORTS = [
     "plan_last_density_radius_ms", "plan_boundness_load", "plan_boundness_load_catalog", "plan_boundness_run",
     "plan_boundness_result", "plan_set_knn_list_limit", "plan_read_lis", "plan_set_particles_lis",
     "plan_get_particle_set", "plan_write_peaks", "plan_write_clump_particles", "plan_write_basic_results",
-    "plan_density_vs_scale", "plan_knn_query_noalloc", "plan_set_num_hop",
+    "plan_density_vs_scale", "plan_knn_query_noalloc", "plan_set_num_hop", "plan_host_tree_balls",
 ]
 
 
@@ -100,6 +100,7 @@ def lib():
     L.plan_boundness_run.argtypes = [vp, C.c_int, C.c_uint32, C.POINTER(PlanBoundnessStats)]
     L.plan_boundness_result.argtypes = [vp, vp, C.POINTER(PlanBoundnessClump)]
     L.plan_set_knn_list_limit.argtypes = [vp, C.c_uint32]
+    L.plan_host_tree_balls.argtypes = [vp, C.c_uint32, vp, vp, C.c_double, vp, C.c_uint32, vp, vp, C.c_uint32]
     L.plan_set_num_hop.argtypes = [vp, C.c_uint32]
     L.plan_read_lis.argtypes = [vp, C.POINTER(PlanParams), C.POINTER(C.c_char_p), C.c_uint32,
                                 C.POINTER(C.c_double), C.POINTER(C.c_uint32)]

@@ -1402,6 +1402,36 @@ int plan_set_num_hop(plan_ctx *c, uint32_t num_hop)
     return PLAN_OK;
 }
 
+int plan_host_tree_balls(const plan_params *params, uint32_t n, const double *pts4, const double *root_center,
+                         double half_width, const double *queries4, uint32_t nq, uint32_t *offsets, double *out,
+                         uint32_t cap)
+{
+    if (!params || (n && !pts4) || !root_center || (nq && !queries4) || !offsets) return PLAN_E_ARG;
+    try {
+        plan_derived d;
+        derive_params(params, n, &d);
+        std::vector<double4> pts(n);
+        for (uint32_t i = 0; i < n; ++i)
+            pts[i] = make_double4(pts4[4 * i], pts4[4 * i + 1], pts4[4 * i + 2], pts4[4 * i + 3]);
+        HostTree ht;
+        ht.build(pts, root_center, half_width, d);
+        std::vector<uint32_t> l;
+        uint32_t k = 0;
+        for (uint32_t q = 0; q < nq; ++q) {
+            offsets[q] = k;
+            if (n) ht.ball(queries4[4 * q], queries4[4 * q + 1], queries4[4 * q + 2], queries4[4 * q + 3], l);
+            for (uint32_t j : l) {
+                if (k < cap && out) out[k] = ht.payload(j);
+                ++k;
+            }
+        }
+        offsets[nq] = k;
+        return k > cap ? PLAN_E_ARG : PLAN_OK;
+    } catch (...) {
+        return PLAN_E_INTERNAL;
+    }
+}
+
 int plan_set_knn_list_limit(plan_ctx *c, uint32_t max_entries)
 {
     if (!c) return PLAN_E_ARG;
