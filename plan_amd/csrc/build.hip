@@ -264,18 +264,20 @@ __global__ void k_morton(const double4 *__restrict__ pos, const double4 *__restr
     rec[i] = r;
 }
 
-/* Keys sorted stably by their high 64 bits (words y, z) are in input order
- * inside each run of equal high halves; sorting every such run by the low
- * word, stably, gives the order of a stable sort by all 96 bits.  Runs hold
- * particles within ~1e-7 of the box width of each other and are short; a run
- * longer than RUN_FIX_MAX is reported (err) and the caller sorts by the full
- * key instead. */
+/* Keys sorted stably by their high 48 bits (word z and the top half of
+ * word y, the first 16 octree levels) are in input order inside each run of
+ * equal high parts; sorting every such run by the low 48 bits, stably, gives
+ * the order of a stable sort by all 96 bits.  Runs hold particles within
+ * 2^-16 of the root box width of each other and are short (about 0.2% of the
+ * particles of the densest benchmark clump core share a cell); a run longer
+ * than RUN_FIX_MAX is reported (err) and the caller sorts by the full key
+ * instead. */
 constexpr uint32_t RUN_FIX_MAX = 64;
 __global__ void k_high_run_starts(const uint4 *__restrict__ keys, uint32_t T, uint32_t *__restrict__ start)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= T) return;
-    start[i] = (i == 0) ? 1u : ((keys[i].y != keys[i - 1].y || keys[i].z != keys[i - 1].z) ? 1u : 0u);
+    start[i] = (i == 0) ? 1u : (((keys[i].y >> 16) != (keys[i - 1].y >> 16) || keys[i].z != keys[i - 1].z) ? 1u : 0u);
 }
 __global__ void k_high_run_fix(uint4 *__restrict__ keys, uint32_t T, const uint32_t *__restrict__ start,
                                uint32_t *__restrict__ err)
@@ -289,10 +291,13 @@ __global__ void k_high_run_fix(uint4 *__restrict__ keys, uint32_t T, const uint3
         atomicOr(err, 1u);
         return;
     }
+    /* low 48 bits: the bottom half of word y, then word x */
+    auto low = [](const uint4 &k) { return ((uint64_t)(k.y & 0xffffu) << 32) | k.x; };
     for (uint32_t a = i + 1; a < e; ++a) {
         const uint4 v = keys[a];
+        const uint64_t lv = low(v);
         uint32_t b = a;
-        while (b > i && keys[b - 1].x > v.x) {
+        while (b > i && low(keys[b - 1]) > lv) {
             keys[b] = keys[b - 1];
             --b;
         }
@@ -665,13 +670,13 @@ void dev_build_tree(DevTree &t, const DevParticles &ps, const plan_derived &d, b
     hipLaunchKernelGGL(k_morton, dim3(ceil_div(T, BLK)), dim3(BLK), 0, s, ps.pos, ps.vel, ps.prop, ps.id, T, f, k0,
                        grec);
     LAUNCHCHK();
-    /* sort by the high 64 bits, then each run of equal high halves by the
-     * low word */
+    /* sort by the high 48 bits, then each run of equal high parts by the
+     * low 48 bits */
     uint32_t *flags = a.get_t<uint32_t>("flags", T + 1);
     uint32_t *ridx = a.get_t<uint32_t>("ridx", T + 1);
     uint32_t *tot = a.get_t<uint32_t>("tot", 8);
     HIPCHK(hipMemsetAsync(tot + 1, 0, 4, s));
-    radix_sort_96(a, &k0, &k1, T, 3, s, 4);
+    radix_sort_96(a, &k0, &k1, T, 3, s, 6);
     hipLaunchKernelGGL(k_high_run_starts, dim3(ceil_div(T, BLK)), dim3(BLK), 0, s, k0, T, flags);
     LAUNCHCHK();
     hipLaunchKernelGGL(k_high_run_fix, dim3(ceil_div(T, BLK)), dim3(BLK), 0, s, k0, T, flags, tot + 1);
