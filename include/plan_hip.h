@@ -211,7 +211,7 @@ int plan_device_memory(plan_ctx *ctx, uint64_t *bytes);
 /* wall-clock milliseconds of the last pipeline stages: ghosts, build, density,
  * hop, clumps, total (hipEvent timing on the context stream; clumps = host). */
 int plan_stage_times(plan_ctx *ctx, double *ms, int n);
-/* counters of the last FindPlanetesimals, up to n of these 46 values:
+/* counters of the last FindPlanetesimals, up to n of these 48 values:
  *  0 particles above threshold, 1 HOP groups, 2 naive clumps, 3 clumps after
  *  merging, 4 merge passes, 5 pair candidates, 6 saddle queries, 7 rebinding
  *  ball searches, 8 member-table copy ms, 9 rebinding reruns,
@@ -236,7 +236,9 @@ int plan_stage_times(plan_ctx *ctx, double *ms, int n);
  *  44 densities of the density pass whose x^1.5 lay so near a rounding
  *  midpoint (about 0.013 ulp) that they were checked on the host against the C library's
  *  pow, which the reference calls (src/tree.hpp:3489), 45 those of them
- *  where that pow rounds the other way and which were redone with it */
+ *  where that pow rounds the other way and which were redone with it,
+ *  46 host ms building the plan_catalog, 47 host ms of the whole clump logic
+ *  (merging to ranking) */
 int plan_clump_stats(plan_ctx *ctx, double *values, int n);
 /* device milliseconds of the last density pass's gather kernel (the K-nearest
  * selection and density, one launch) and of its radius-schedule kernel */

@@ -49,6 +49,8 @@ struct plan_ctx {
     ClumpStats cstats;
     uint32_t hop_na = 0, hop_groups = 0;
     double d2h_ms = 0;
+    double clump_logic_ms = 0; /* run_clump_pipeline of the last FindPlanetesimals */
+    double catalog_ms = 0;     /* building its plan_catalog */
     HostPinned hpin;
     std::vector<uint32_t> tidx_map; /* tree index -> member record (grow-only) */
     double density_kernel_ms = 0;
@@ -884,7 +886,10 @@ int plan_find_planetesimals(plan_ctx *c, plan_catalog **out)
         double Mp_max = 0, Mp_tot = 0;
         ClumpStats st;
         c->d2h_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
+        const auto tc0 = std::chrono::steady_clock::now();
         run_clump_pipeline(hm, cs, svc, cl, Mp_max, Mp_tot, st);
+        const auto tc1 = std::chrono::steady_clock::now();
+        c->clump_logic_ms = std::chrono::duration<double, std::milli>(tc1 - tc0).count();
         c->cstats = st;
         c->hop_na = na;
         c->hop_groups = nc;
@@ -902,15 +907,16 @@ int plan_find_planetesimals(plan_ctx *c, plan_catalog **out)
         cat->clump_mass_threshold = cs.clump_mass_threshold;
         cat->Mp_max = Mp_max;
         cat->Mp_tot = Mp_tot;
+        /* clump fields and member offsets in key order, then every clump's
+         * members (records scattered in distance order) in parallel */
+        std::vector<const HClump *> qs;
         uint32_t off = 0, i = 0;
         for (auto &it : cl) {
             const HClump &q = it.second;
             plan_clump &o = cat->clumps[i++];
+            qs.push_back(&q);
             o.peak_index = it.first;
             o.num_members = (uint32_t)q.particles.size();
-            uint32_t np = 0;
-            for (uint32_t h : q.indices) np += hm.sinkcnt[h];
-            o.num_particles = np;
             o.member_offset = off;
             o.total_mass = q.total_mass;
             for (int k = 0; k < 3; ++k) {
@@ -925,13 +931,31 @@ int plan_find_planetesimals(plan_ctx *c, plan_catalog **out)
             o.max_distance = q.particles.empty() ? 0.0 : q.particles.back().second;
             uint32_t kr = hm.rec_of_tidx(it.first);
             o.peak_density = kr == PLAN_NONE ? 0.0 : hm.rec[kr].rho;
-            for (auto &m : q.particles) {
-                cat->members[off] = hm.rec[m.first].tidx;
-                cat->member_original_ids[off] = hm.rec[m.first].oid;
-                ++off;
+            off += o.num_members;
+        }
+        const long ncl = (long)qs.size();
+#pragma omp parallel for schedule(dynamic, 1)
+        for (long k = 0; k < ncl; ++k) {
+            const HClump &q = *qs[k];
+            plan_clump &o = cat->clumps[k];
+            const size_t nm = q.particles.size();
+            uint32_t np = 0, w = o.member_offset;
+            for (size_t j = 0; j < nm; ++j) {
+                if (j + 12 < nm) {
+                    __builtin_prefetch(&hm.rec[q.particles[j + 12].first].rho);
+                    __builtin_prefetch(&hm.sinkcnt[q.particles[j + 12].first]);
+                }
+                const uint32_t h = q.particles[j].first;
+                const MemberRec &r = hm.rec[h];
+                cat->members[w] = r.tidx;
+                cat->member_original_ids[w] = r.oid;
+                np += hm.sinkcnt[h];
+                ++w;
             }
+            o.num_particles = np;
         }
         auto t2 = std::chrono::steady_clock::now();
+        c->catalog_ms = std::chrono::duration<double, std::milli>(t2 - tc1).count();
         c->times[4] = std::chrono::duration<double, std::milli>(t2 - t1).count();
         c->times[5] = std::chrono::duration<double, std::milli>(t2 - t0).count();
         *out = cat;
@@ -1355,7 +1379,7 @@ int plan_stage_times(plan_ctx *c, double *ms, int n)
 int plan_clump_stats(plan_ctx *c, double *v, int n)
 {
     if (!c || !v) return PLAN_E_ARG;
-    double a[46] = {(double)c->hop_na, (double)c->hop_groups, (double)c->cstats.naive, (double)c->cstats.after_merge,
+    double a[48] = {(double)c->hop_na, (double)c->hop_groups, (double)c->cstats.naive, (double)c->cstats.after_merge,
                     (double)c->cstats.merge_passes, (double)c->cstats.pair_candidates,
                     (double)c->cstats.saddle_queries, (double)c->cstats.rebind_searches, c->d2h_ms,
                     (double)c->cstats.rebind_reruns};
@@ -1382,7 +1406,9 @@ int plan_clump_stats(plan_ctx *c, double *v, int n)
     a[43] = c->cstats.key0_erased;
     a[44] = c->density_pow_checks;
     a[45] = c->density_pow_fixes;
-    for (int i = 0; i < n && i < 46; ++i) v[i] = a[i];
+    a[46] = c->catalog_ms;
+    a[47] = c->clump_logic_ms;
+    for (int i = 0; i < n && i < 48; ++i) v[i] = a[i];
     return PLAN_OK;
 }
 
