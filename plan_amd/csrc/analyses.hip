@@ -205,7 +205,7 @@ void dev_density_vs_scale(const DevTree &t, const plan_params &pp, const plan_de
     LAUNCHCHK();
     uint32_t he[4];
     HIPCHK(hipMemcpyAsync(he, ex, 16, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(plan::stream_wait(s));
     if (he[1] || he[3]) throw std::runtime_error("density_vs_scale: tree deeper than 40 levels");
     double zlo, zhi;
     HIPCHK(hipMemcpy(&zlo, reinterpret_cast<const char *>(t.pm + he[0]) + 16, 8, hipMemcpyDeviceToHost));
@@ -264,7 +264,7 @@ void dev_density_vs_scale(const DevTree &t, const plan_params &pp, const plan_de
     }
     std::vector<uint32_t> h(4 * (size_t)L);
     HIPCHK(hipMemcpyAsync(h.data(), acc, 16 * (size_t)L, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(plan::stream_wait(s));
     for (int i = 0; i < L; ++i) {
         const double best = (double)h[4 * i];
         res.sums[i] = (uint64_t)h[4 * i + 2] | ((uint64_t)h[4 * i + 3] << 32);

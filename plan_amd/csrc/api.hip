@@ -268,7 +268,7 @@ struct Services : ClumpServices {
         dev_knn(c->tree, c->K, 0, c->kernel, nq, nullptr, q, e, c->tree.rho, c->d, ko, s);
         rho.resize(nq);
         HIPCHK(hipMemcpyAsync(rho.data(), r, nq * sizeof(double), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(plan::stream_wait(s));
     }
 
     void run_balls(const DevTree &t, const std::vector<double4> &queries, const double *rho, double rho_min,
@@ -317,7 +317,7 @@ struct Services : ClumpServices {
         run_balls(c->ctree, queries, nullptr, 0.0, 0, idx);
         std::vector<double4> va(c->ctree.n);
         HIPCHK(hipMemcpyAsync(va.data(), c->ctree.va, c->ctree.n * sizeof(double4), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(plan::stream_wait(s));
         for (size_t qi = 0; qi < queries.size(); ++qi) {
             peaks[qi].reserve(idx[qi].size());
             for (uint32_t j : idx[qi]) peaks[qi].push_back((uint32_t)va[j].w);
@@ -430,8 +430,8 @@ int plan_ctx_create(plan_ctx **out, int device)
     try {
         HIPCHK(hipSetDevice(device));
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-        for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
-        for (auto &e : c->rec_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        for (auto &e : c->ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventBlockingSync));
+        for (auto &e : c->rec_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming | hipEventBlockingSync));
         if (scratch_init(device) != PLAN_OK) throw std::runtime_error("scratch symbol address");
     } catch (const std::exception &e) {
         delete c;
@@ -445,7 +445,7 @@ void plan_ctx_destroy(plan_ctx *c)
 {
     if (!c) return;
     (void)hipSetDevice(c->device);
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->stream) (void)plan::stream_wait(c->stream);
     free_tree_nodes(c->tree);
     free_tree_nodes(c->ctree);
     free_particles(c->ps);
@@ -505,7 +505,7 @@ int plan_set_particles(plan_ctx *c, const plan_params *params, uint32_t N, const
         HIPCHK(hipMemcpyAsync(c->ps.id, id, (size_t)4 * N, hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(k_f64_to_set, dim3(ceil_div(N, 256)), dim3(256), 0, s, dp, dv, dd, N, c->ps.pos, c->ps.vel);
         LAUNCHCHK();
-        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(plan::stream_wait(s));
         return PLAN_OK;
     });
 }
@@ -532,7 +532,7 @@ int plan_set_particles_f32(plan_ctx *c, const plan_params *params, uint32_t N, c
         HIPCHK(hipMemcpyAsync(c->ps.id, id, (size_t)4 * N, hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(k_f32_to_set, dim3(ceil_div(N, 256)), dim3(256), 0, s, dp, dv, dd, N, c->ps.pos, c->ps.vel);
         LAUNCHCHK();
-        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(plan::stream_wait(s));
         return PLAN_OK;
     });
 }
@@ -621,7 +621,7 @@ int plan_get_particle_set(plan_ctx *c, uint32_t cap, double *pos, double *vel, d
         HIPCHK(hipMemcpyAsync(V.data(), c->ps.vel, (size_t)n * 32, hipMemcpyDeviceToHost, s));
         if (prop) HIPCHK(hipMemcpyAsync(prop, c->ps.prop, (size_t)n * 4, hipMemcpyDeviceToHost, s));
         if (id) HIPCHK(hipMemcpyAsync(id, c->ps.id, (size_t)n * 4, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(plan::stream_wait(s));
         for (uint32_t i = 0; i < n; ++i) {
             if (pos) {
                 pos[3 * i] = P[i].x;
@@ -747,6 +747,12 @@ int plan_hop(plan_ctx *c, uint32_t H, double threshold, uint32_t *num_above, uin
     });
 }
 
+static std::mutex &host_phase_mutex()
+{
+    static std::mutex m;
+    return m;
+}
+
 int plan_find_planetesimals(plan_ctx *c, plan_catalog **out)
 {
     return guarded(c, [&]() -> int {
@@ -765,6 +771,10 @@ int plan_find_planetesimals(plan_ctx *c, plan_catalog **out)
         if (r) return r;
         r = run_hop(c, H, thr);
         if (r) return r;
+        /* the host part (member table, clump logic, catalog) uses every host
+         * core: contexts take turns at it (two at once would oversubscribe
+         * the cores), while another context's kernels keep the GPU busy */
+        std::unique_lock<std::mutex> host_turn(host_phase_mutex());
         auto t1 = std::chrono::steady_clock::now();
         /* member table to the host */
         hipStream_t s = c->stream;
@@ -803,7 +813,7 @@ int plan_find_planetesimals(plan_ctx *c, plan_catalog **out)
             HIPCHK(hipMemcpyAsync(hm.cpeak.data(), cpk, (size_t)nc * 4, hipMemcpyDeviceToHost, s));
             HIPCHK(hipMemcpyAsync(hm.sorted_tidx.data(), c->hop.a_idx, (size_t)na * 4, hipMemcpyDeviceToHost, s));
             HIPCHK(hipMemcpyAsync(hm.rec_of_sorted.data(), roa, (size_t)na * 4, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
+            HIPCHK(plan::stream_wait(s));
             /* the records in chunks: the naive clumps' first pass starts on
              * the clumps whose records have arrived while the rest stream */
             constexpr uint32_t NCH = sizeof(c->rec_ev) / sizeof(c->rec_ev[0]);
@@ -1140,7 +1150,7 @@ int plan_get_hop(plan_ctx *c, uint32_t *densest, uint32_t *peak)
         hipLaunchKernelGGL(k_hop_full, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, c->hop.apos, n,
                            c->hop.densest, c->hop.root, c->hop.a_idx, dn, pk);
         LAUNCHCHK();
-        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(plan::stream_wait(c->stream));
         if (densest) HIPCHK(hipMemcpy(densest, dn, (size_t)n * 4, hipMemcpyDeviceToHost));
         if (peak) HIPCHK(hipMemcpy(peak, pk, (size_t)n * 4, hipMemcpyDeviceToHost));
         return PLAN_OK;
@@ -1187,7 +1197,7 @@ int plan_knn_query(plan_ctx *c, uint32_t nq, const uint32_t *pidx, const double 
             LAUNCHCHK();
             dev_knn(c->tree, K, 2, 0, nq, nullptr, q4, e2, c->tree.rho, c->d, ko, s);
         }
-        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(plan::stream_wait(s));
         if (idx) HIPCHK(hipMemcpy(idx, di, (size_t)nq * K * 4, hipMemcpyDeviceToHost));
         if (d2) HIPCHK(hipMemcpy(d2, dd, (size_t)nq * K * 8, hipMemcpyDeviceToHost));
         if (rstop) HIPCHK(hipMemcpy(rstop, dr, (size_t)nq * 8, hipMemcpyDeviceToHost));
@@ -1263,7 +1273,7 @@ int plan_knn_query_noalloc(plan_ctx *c, uint32_t nq, const uint32_t *pidx, const
         LAUNCHCHK();
         uint32_t nbad = 0;
         HIPCHK(hipMemcpyAsync(&nbad, bad, 4, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(plan::stream_wait(s));
         if (nbad)
             return fail(c, PLAN_E_ARG, pidx ? "particle index out of range"
                                             : "point queries need a positive start radius");
@@ -1280,7 +1290,7 @@ int plan_knn_query_noalloc(plan_ctx *c, uint32_t nq, const uint32_t *pidx, const
             LAUNCHCHK();
             dev_knn(c->tree, K, 2, 0, nq, nullptr, q4, e2, c->tree.rho, c->d, ko, s, &a);
         }
-        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(plan::stream_wait(s));
         return PLAN_OK;
     });
 }
@@ -1337,7 +1347,7 @@ int plan_keep_input(plan_ctx *c)
         HIPCHK(hipMemcpyAsync(c->saved.vel, c->ps.vel, (size_t)N * 32, hipMemcpyDeviceToDevice, s));
         HIPCHK(hipMemcpyAsync(c->saved.prop, c->ps.prop, (size_t)N * 4, hipMemcpyDeviceToDevice, s));
         HIPCHK(hipMemcpyAsync(c->saved.id, c->ps.id, (size_t)N * 4, hipMemcpyDeviceToDevice, s));
-        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(plan::stream_wait(s));
         c->saved.N = c->saved.T = N;
         return PLAN_OK;
     });
@@ -1354,7 +1364,7 @@ int plan_restore_input(plan_ctx *c)
         HIPCHK(hipMemcpyAsync(c->ps.vel, c->saved.vel, (size_t)N * 32, hipMemcpyDeviceToDevice, s));
         HIPCHK(hipMemcpyAsync(c->ps.prop, c->saved.prop, (size_t)N * 4, hipMemcpyDeviceToDevice, s));
         HIPCHK(hipMemcpyAsync(c->ps.id, c->saved.id, (size_t)N * 4, hipMemcpyDeviceToDevice, s));
-        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(plan::stream_wait(s));
         c->ps.N = c->ps.T = N;
         c->have_particles = true;
         c->have_tree = c->have_density = c->have_hop = false;

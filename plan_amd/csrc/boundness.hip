@@ -570,7 +570,7 @@ void BoundnessEngine::alloc_run_buffers(hipStream_t s)
     scan_exclusive_u32(a_, inb_, sb_, nm_ ? nm_ : 1, s, nullptr);
     scan_exclusive_u32(a_, tiles_, toff_, nc_ ? nc_ : 1, s, nullptr);
     shape_idx_ = -1;
-    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(plan::stream_wait(s));
 }
 
 /* reset members [m0,m1) and clumps [c0,c1) to "all bound" */
@@ -658,7 +658,7 @@ uint32_t BoundnessEngine::read_active(hipStream_t s)
 {
     uint32_t v = 0;
     HIPCHK(hipMemcpyAsync(&v, misc_ + 2, 4, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(plan::stream_wait(s));
     return v;
 }
 
@@ -708,8 +708,8 @@ void BoundnessEngine::run_fp32_batched(uint32_t max_iter, hipStream_t s)
 void BoundnessEngine::autotune(hipStream_t s)
 {
     hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventCreateWithFlags(&e0, hipEventBlockingSync));
+    HIPCHK(hipEventCreateWithFlags(&e1, hipEventBlockingSync));
     float best = 1e30f;
     int bi = 0;
     const uint32_t cend = std::min<uint32_t>(nc_, std::max<uint32_t>(1, nc_ / 4));
@@ -743,8 +743,8 @@ void BoundnessEngine::run(int form, uint32_t max_iter, hipStream_t s, plan_bound
     unsigned long long zero = 0;
     HIPCHK(hipMemcpyAsync(pairs_, &zero, 8, hipMemcpyHostToDevice, s));
     hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventCreateWithFlags(&e0, hipEventBlockingSync));
+    HIPCHK(hipEventCreateWithFlags(&e1, hipEventBlockingSync));
     if (form == 1 && shape_idx_ < 0) autotune(s);
     HIPCHK(hipMemcpyAsync(pairs_, &zero, 8, hipMemcpyHostToDevice, s));
     HIPCHK(hipEventRecord(e0, s));
@@ -759,7 +759,7 @@ void BoundnessEngine::run(int form, uint32_t max_iter, hipStream_t s, plan_bound
         /* clumps whose total energy came within 1e-3 of zero: redo in binary64 */
         std::vector<uint32_t> nz(nc_);
         HIPCHK(hipMemcpyAsync(nz.data(), nearz_, nc_ * 4, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(plan::stream_wait(s));
         for (uint32_t c = 0; c < nc_; ++c)
             if (nz[c]) refit.push_back(c);
         if (!refit.empty()) run_fp64_per_clump(refit, max_iter, s);
@@ -793,7 +793,7 @@ void BoundnessEngine::download(uint8_t *bound, plan_boundness_clump *out, hipStr
         HIPCHK(hipMemcpyAsync(eo.data(), eout_, eo.size() * 8, hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpyAsync(tot.data(), tot_, tot.size() * 8, hipMemcpyDeviceToHost, s));
     }
-    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(plan::stream_wait(s));
     if (bound)
         for (uint32_t i = 0; i < nm_; ++i) bound[i] = fl[i] == B_IN ? 1 : 0;
     if (out)

@@ -106,7 +106,7 @@ static uint32_t read_u32(const uint32_t *dp, hipStream_t s)
 {
     uint32_t v;
     HIPCHK(hipMemcpyAsync(&v, dp, 4, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(plan::stream_wait(s));
     return v;
 }
 
@@ -137,7 +137,7 @@ void dev_make_ghosts(DevParticles &ps, const plan_params &pp, const plan_derived
     scan_exclusive_u32(a, cy, oy, N, s, tot + 1);
     uint32_t tt[2];
     HIPCHK(hipMemcpyAsync(tt, tot, 8, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(plan::stream_wait(s));
     const uint32_t GX = tt[0], GYO = tt[1];
     /* capacity: the y images of the x images are bounded by 2 GX */
     uint64_t need = (uint64_t)N + GX + 2ull * GX + GYO;
@@ -156,7 +156,7 @@ void dev_make_ghosts(DevParticles &ps, const plan_params &pp, const plan_derived
         HIPCHK(hipMemcpyAsync(nv, ps.vel, (size_t)N * sizeof(double4), hipMemcpyDeviceToDevice, s));
         HIPCHK(hipMemcpyAsync(npr, ps.prop, (size_t)N * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
         HIPCHK(hipMemcpyAsync(nid, ps.id, (size_t)N * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
-        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(plan::stream_wait(s));
         HIPCHK(hipFree(ps.pos));
         HIPCHK(hipFree(ps.vel));
         HIPCHK(hipFree(ps.prop));
@@ -618,7 +618,7 @@ void dev_build_tree(DevTree &t, const DevParticles &ps, const plan_derived &d, b
         LAUNCHCHK();
         std::vector<double> hb((size_t)nb * 6);
         HIPCHK(hipMemcpyAsync(hb.data(), bb, hb.size() * 8, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(plan::stream_wait(s));
         double pmin[3], pmax[3];
         for (int k = 0; k < 3; ++k) {
             pmin[k] = hb[k];
@@ -689,7 +689,7 @@ void dev_build_tree(DevTree &t, const DevParticles &ps, const plan_derived &d, b
     scan_exclusive_u32(a, flags, ridx, T, s, tot);
     uint32_t h2[2];
     HIPCHK(hipMemcpyAsync(h2, tot, sizeof(h2), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(plan::stream_wait(s));
     if (h2[1]) {
         /* a long run of equal high halves: stable sort by the full key (equal
          * full keys are in input order already) */
@@ -699,7 +699,7 @@ void dev_build_tree(DevTree &t, const DevParticles &ps, const plan_derived &d, b
         LAUNCHCHK();
         scan_exclusive_u32(a, flags, ridx, T, s, tot);
         HIPCHK(hipMemcpyAsync(h2, tot, sizeof(h2), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(plan::stream_wait(s));
     }
     const uint32_t n = h2[0];
     t.n = n;
@@ -730,7 +730,7 @@ void dev_build_tree(DevTree &t, const DevParticles &ps, const plan_derived &d, b
     {
         uint32_t h[2];
         HIPCHK(hipMemcpyAsync(h, tot + 1, sizeof(h), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(plan::stream_wait(s));
         t.num_sinks = h[0];
         t.mass_uniform = h[1] == 0u;
     }
@@ -749,7 +749,7 @@ void dev_build_tree(DevTree &t, const DevParticles &ps, const plan_derived &d, b
     HIPCHK(hipMemsetAsync(lptr + 33, 0, 4, s));
     uint32_t hl[33];
     HIPCHK(hipMemcpyAsync(hl, lptr, sizeof(hl), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(plan::stream_wait(s));
     const uint32_t nn = hl[32];
     if (t.node_cap < (size_t)nn) {
         if (t.ngeo) HIPCHK(hipFree(t.ngeo));

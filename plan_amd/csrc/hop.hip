@@ -134,7 +134,7 @@ static uint32_t read_u32(const uint32_t *dp, hipStream_t s)
 {
     uint32_t v;
     HIPCHK(hipMemcpyAsync(&v, dp, 4, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(plan::stream_wait(s));
     return v;
 }
 

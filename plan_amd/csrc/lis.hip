@@ -68,7 +68,7 @@ static uint32_t lis_read_u32(const uint32_t *dp, hipStream_t s)
 {
     uint32_t v = 0;
     HIPCHK(hipMemcpyAsync(&v, dp, 4, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(plan::stream_wait(s));
     return v;
 }
 
@@ -111,7 +111,7 @@ public:
     {
         for (int b = 0; b < 2; ++b) {
             HIPCHK(hipHostMalloc(&buf_[b], CHUNK, hipHostMallocDefault));
-            HIPCHK(hipEventCreateWithFlags(&ev_[b], hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&ev_[b], hipEventDisableTiming | hipEventBlockingSync));
         }
     }
     ~Uploader()
@@ -213,7 +213,7 @@ std::string lis_upload_records(const char *const *paths, uint32_t num_paths, con
             return std::string(paths[p]) + " holds fewer records than its header states";
         dst += bytes;
     }
-    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(plan::stream_wait(s));
     return "";
 }
 

@@ -1325,7 +1325,7 @@ static void launch_select(const TreeView &tv, Arena &a, uint32_t nb, const uint3
     scan_exclusive_u32(a, ovf, opos, nw, s, otot);
     uint32_t novf = 0;
     HIPCHK(hipMemcpyAsync(&novf, otot, 4, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(plan::stream_wait(s));
     if (novf) {
         uint32_t *waves = a.get_t<uint32_t>("knn_owaves", novf);
         hipLaunchKernelGGL(k_ovf_compact, dim3(ceil_div(nw, 256)), dim3(256), 0, s, ovf, opos, nw, waves);
@@ -1347,7 +1347,7 @@ static void launch_select(const TreeView &tv, Arena &a, uint32_t nb, const uint3
             LAUNCHCHK();
             scan_exclusive_u32(a, ovf2, opos2, nw2, s, otot);
             HIPCHK(hipMemcpyAsync(&nr, otot, 4, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
+            HIPCHK(plan::stream_wait(s));
             if (nr) {
                 waves = a.get_t<uint32_t>("knn_owaves2", nr);
                 hipLaunchKernelGGL(k_ovf_compact, dim3(ceil_div(nw2, 256)), dim3(256), 0, s, ovf2, opos2, nw2, waves);
@@ -1480,12 +1480,12 @@ void dev_knn(const DevTree &t, uint32_t K, int mode, int kernel, uint32_t nq, co
          * value the kernel used are sent back and their densities redone */
         uint32_t cnt = 0;
         HIPCHK(hipMemcpyAsync(&cnt, tv.pw_count, 4, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(plan::stream_wait(s));
         if (cnt > tv.pw_cap) throw std::runtime_error("KNN: density re-check list overflow");
         if (cnt) {
             const double *x = (const double *)const_cast<DevTree &>(t).host_stage.get((size_t)cnt * 8);
             HIPCHK(hipMemcpyAsync((void *)x, tv.pw_x, (size_t)cnt * 8, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
+            HIPCHK(plan::stream_wait(s));
             std::vector<uint32_t> bad_slot;
             std::vector<double> bad_p;
 #pragma omp parallel
@@ -1517,7 +1517,7 @@ void dev_knn(const DevTree &t, uint32_t K, int mode, int kernel, uint32_t nq, co
                 hipLaunchKernelGGL(k_pow_fix, dim3(ceil_div(nb, 256)), dim3(256), 0, s, ds, dp, nb, tv.pw_idx,
                                    tv.pw_num, out.rho);
                 LAUNCHCHK();
-                HIPCHK(hipStreamSynchronize(s));
+                HIPCHK(plan::stream_wait(s));
             }
             if (out.npow) *out.npow += nb;
         }
@@ -1766,7 +1766,7 @@ void dev_ball_lists(const DevTree &t, Arena &a, const std::vector<double4> &quer
         }
         uint32_t h[40];
         HIPCHK(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(plan::stream_wait(s));
         if (h[37]) {
             /* the counts up to the overflowing level are exact requests;
              * grow at least 4x so a later level that was not reached fits
@@ -1802,7 +1802,7 @@ void dev_ball_lists(const DevTree &t, Arena &a, const std::vector<double4> &quer
         scan_exclusive_u32(a, sizes, pre, nt, s, pre + nt);
         uint32_t ncand = 0;
         HIPCHK(hipMemcpyAsync(&ncand, pre + nt, 4, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(plan::stream_wait(s));
         uint32_t *flag = a.get_t<uint32_t>("bl_flag", (size_t)ncand + 1);
         uint32_t *pidx = a.get_t<uint32_t>("bl_pidx", (size_t)ncand + 1);
         uint32_t *opos = a.get_t<uint32_t>("bl_opos", (size_t)ncand + 1);
@@ -1813,7 +1813,7 @@ void dev_ball_lists(const DevTree &t, Arena &a, const std::vector<double4> &quer
         scan_exclusive_u32(a, flag, opos, ncand, s, opos + ncand);
         uint32_t total = 0;
         HIPCHK(hipMemcpyAsync(&total, opos + ncand, 4, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(plan::stream_wait(s));
         uint32_t *res = a.get_t<uint32_t>("bl_out", (size_t)total + 1);
         hipLaunchKernelGGL(k_bfs_compact, dim3(gb), dim3(256), 0, s, flag, opos, pidx, ncand, res);
         LAUNCHCHK();
@@ -1824,7 +1824,7 @@ void dev_ball_lists(const DevTree &t, Arena &a, const std::vector<double4> &quer
         std::vector<uint32_t> hres(total);
         HIPCHK(hipMemcpyAsync(htq.data(), tq, (size_t)nt * sizeof(uint2), hipMemcpyDeviceToHost, s));
         if (total) HIPCHK(hipMemcpyAsync(hres.data(), res, (size_t)total * 4, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(plan::stream_wait(s));
         /* tasks are grouped by query: a query's results run from its first
          * task's output position to the next query's */
         for (uint32_t k = 0; k < nt;) {

@@ -80,7 +80,7 @@ void catalog_members(const DevTree &t, const DevParticles &ps, const plan_catalo
     LAUNCHCHK();
     std::vector<uint32_t> cnt(m);
     HIPCHK(hipMemcpyAsync(cnt.data(), dc, (size_t)m * 4, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(plan::stream_wait(s));
     for (uint32_t i = 0; i < m; ++i) out.off[i + 1] = out.off[i] + cnt[i];
     const uint32_t e = out.off[m];
     uint32_t *doff = a.get_t<uint32_t>("wr_off", m);
@@ -95,7 +95,7 @@ void catalog_members(const DevTree &t, const DevParticles &ps, const plan_catalo
     HIPCHK(hipMemcpyAsync(out.pm.data(), dpm, (size_t)m * 32, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(out.va.data(), dva, (size_t)m * 32, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(out.exp.data(), dex, (size_t)e * 32, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(plan::stream_wait(s));
 }
 
 /* clumps in the order of peaks_and_masses (src/tree.hpp:4291-4299):
