@@ -721,13 +721,21 @@ static void to_records(const HostMembers &hm, const std::vector<uint32_t> &tidx,
     }
 }
 
+/* A member's terms of the kinematic sums (src/tree.hpp:4446-4460): m, x m,
+ * y m, z m, vx m, vy m, vz m, computed once when it joins; a sum over the
+ * members then reads one cache line per member. */
+struct alignas(64) KinRow {
+    double t[8];
+};
+
 /* A clump's members during rebinding, structure of arrays in the order they
  * joined (the members at the start, then each claim), with the current
  * distance order `ord` over those positions.  Only mass, centre and velocity
  * feed back into the loop, so member lists and radii are rebuilt once at the
  * end. */
 struct RbLocal {
-    std::vector<double> x, y, z, m, vx, vy, vz, rho, d;
+    std::vector<double> x, y, z, m, rho, d;
+    std::vector<KinRow> kin;
     std::vector<uint32_t> h, ord;
     std::vector<PD> v, tmp;
     std::vector<uint32_t> cnt;
@@ -736,7 +744,8 @@ struct RbLocal {
     /* room for the members plus the superset they are claimed from */
     void reserve(size_t n)
     {
-        for (auto *a : {&x, &y, &z, &m, &vx, &vy, &vz, &rho}) a->reserve(n);
+        for (auto *a : {&x, &y, &z, &m, &rho}) a->reserve(n);
+        kin.reserve(n);
         h.reserve(n);
     }
     void push(const MemberRec &r, uint32_t hh)
@@ -745,9 +754,7 @@ struct RbLocal {
         y.push_back(r.y);
         z.push_back(r.z);
         m.push_back(r.m);
-        vx.push_back(r.vx);
-        vy.push_back(r.vy);
-        vz.push_back(r.vz);
+        kin.push_back(KinRow{{r.m, r.x * r.m, r.y * r.m, r.z * r.m, r.vx * r.m, r.vy * r.m, r.vz * r.m, 0.0}});
         rho.push_back(r.rho);
         h.push_back(hh);
     }
@@ -794,25 +801,45 @@ static DistFn dist_fn()
     return cpu_has_avx512() ? dist_avx512 : __builtin_cpu_supports("avx2") ? dist_avx2 : dist_base;
 }
 
+/* out[k] = sum of the kinematic rows' term k over ord[0, ns) and then the
+ * positions ns .. n - 1, each term accumulated in that order from 0 (the
+ * seven sums of SortParticles); one vector add per member with AVX-512,
+ * whose lanes round as the scalar adds do */
+#define PLAN_KIN_SUM_ARGS const KinRow *__restrict K, const uint32_t *__restrict ord, size_t ns, size_t n, double *out
+static void kin_sum_base(PLAN_KIN_SUM_ARGS)
+{
+    double s[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (size_t i = 0; i < n; ++i) {
+        const double *t = K[i < ns ? ord[i] : i].t;
+        for (int k = 0; k < 7; ++k) s[k] += t[k];
+    }
+    for (int k = 0; k < 7; ++k) out[k] = s[k];
+}
+__attribute__((target("avx512f"))) static void kin_sum_avx512(PLAN_KIN_SUM_ARGS)
+{
+    typedef double v8d __attribute__((vector_size(64)));
+    v8d acc = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (size_t i = 0; i < ns; ++i) acc += *reinterpret_cast<const v8d *>(K[ord[i]].t);
+    for (size_t p = ns; p < n; ++p) acc += *reinterpret_cast<const v8d *>(K[p].t);
+    for (int k = 0; k < 7; ++k) out[k] = acc[k];
+}
+#undef PLAN_KIN_SUM_ARGS
+typedef void (*KinSumFn)(const KinRow *, const uint32_t *, size_t, size_t, double *);
+static KinSumFn kin_sum_fn()
+{
+    return cpu_has_avx512() ? kin_sum_avx512 : kin_sum_base;
+}
+
 /* SortParticles after claims (src/tree.hpp:4431-4512 via 4705-4709):
  * kinematics summed in member order (the last sorted order, then the claims
  * in claim order), distances to the new centre, sort. */
 static void rb_resort(HClump &c, RbLocal &L)
 {
     const size_t n = L.size(), ns = L.n_sorted;
-    double M = 0, cx = 0, cy = 0, cz = 0, ux = 0, uy = 0, uz = 0;
-    auto add = [&](uint32_t p) {
-        const double mp = L.m[p];
-        M += mp;
-        cx += L.x[p] * mp;
-        cy += L.y[p] * mp;
-        cz += L.z[p] * mp;
-        ux += L.vx[p] * mp;
-        uy += L.vy[p] * mp;
-        uz += L.vz[p] * mp;
-    };
-    for (size_t i = 0; i < ns; ++i) add(L.ord[i]);
-    for (size_t p = ns; p < n; ++p) add((uint32_t)p);
+    static const KinSumFn kin_sum = kin_sum_fn();
+    double sum[7];
+    kin_sum(L.kin.data(), L.ord.data(), ns, n, sum);
+    const double M = sum[0], cx = sum[1], cy = sum[2], cz = sum[3], ux = sum[4], uy = sum[5], uz = sum[6];
     c.total_mass = M;
     c.com[0] = cx / M;
     c.com[1] = cy / M;
