@@ -58,55 +58,87 @@ uint32_t HostMembers::rec_of_tidx(uint32_t tidx) const
     return rec_of_sorted[it - sorted_tidx.begin()];
 }
 
-/* std::sort(v) by distance for non-negative distances, in expected linear
- * time: bucket by distance (a monotone map, so buckets are ordered), then
- * sort inside each bucket.  Without distance ties the sorted order is
- * unique, so this is std::sort's result; with a tie std::sort runs on the
- * original sequence, which fixes libstdc++'s tie order. */
-static void sort_by_distance_buckets(std::vector<PD> &v, std::vector<PD> &tmp, std::vector<uint32_t> &cnt)
+/* std::sort(v) by distance for non-negative distances, in linear time: the
+ * entries become (22-bit fixed-point distance << 32 | position) items,
+ * sorted by two 11-bit LSD radix passes; the key is monotone in the
+ * distance and the passes are stable, so only runs of equal keys can be out
+ * of distance order, and an insertion pass over those runs finishes the
+ * sort.  Without distance ties the sorted order is unique, so this is
+ * std::sort's result; with a tie (or a distance that is not a number)
+ * std::sort runs on the original sequence, which fixes libstdc++'s tie
+ * order.  tmp, a, b: scratch. */
+static void sort_by_distance_keys(std::vector<PD> &v, std::vector<PD> &tmp, std::vector<uint64_t> &a,
+                                  std::vector<uint64_t> &b)
 {
     const size_t n = v.size();
     double dmax = 0;
     for (const PD &p : v) dmax = std::max(dmax, p.second);
-    if (n < 64 || !(dmax > 0) || !std::isfinite(dmax)) {
+    if (n < 64 || n >= (1ull << 32) || !(dmax > 0) || !std::isfinite(dmax)) {
         std::sort(v.begin(), v.end(), pd_less);
         return;
     }
-    const size_t nb = n;
-    const double scale = (double)nb / dmax;
-    cnt.assign(nb + 1, 0);
-    auto bucket = [&](double d) {
-        const double b = d * scale;
-        return b >= (double)(nb - 1) ? nb - 1 : (size_t)b;
-    };
-    for (const PD &p : v) cnt[bucket(p.second) + 1]++;
-    for (size_t b = 0; b < nb; ++b) cnt[b + 1] += cnt[b];
-    tmp.resize(n);
-    for (const PD &p : v) tmp[cnt[bucket(p.second)]++] = p;
-    /* cnt[b] is now the end of bucket b */
-    size_t s = 0;
-    for (size_t b = 0; b < nb; ++b) {
-        const size_t e = cnt[b];
-        if (e - s > 32) {
-            std::sort(tmp.begin() + s, tmp.begin() + e, pd_less);
+    constexpr uint32_t KMAX = (1u << 22) - 1, DIG = 2048;
+    const double scale = (double)KMAX / dmax;
+    a.resize(n);
+    b.resize(n);
+    uint32_t h0[DIG] = {}, h1[DIG] = {};
+    bool bad = false;
+    for (size_t i = 0; i < n; ++i) {
+        const double dd = v[i].second;
+        bad |= !(dd >= 0);
+        const double f = dd * scale;
+        const uint32_t k = f >= (double)KMAX ? KMAX : f > 0 ? (uint32_t)f : 0u;
+        b[i] = (uint64_t)k << 32 | i;
+        h0[k & (DIG - 1)]++;
+        h1[k >> 11]++;
+    }
+    if (bad) {
+        std::sort(v.begin(), v.end(), pd_less);
+        return;
+    }
+    for (uint32_t d = 0, s0 = 0, s1 = 0; d < DIG; ++d) {
+        const uint32_t t0 = h0[d], t1 = h1[d];
+        h0[d] = s0;
+        h1[d] = s1;
+        s0 += t0;
+        s1 += t1;
+    }
+    for (size_t i = 0; i < n; ++i) {
+        const uint64_t x = b[i];
+        a[h0[(x >> 32) & (DIG - 1)]++] = x;
+    }
+    for (size_t i = 0; i < n; ++i) {
+        const uint64_t x = a[i];
+        b[h1[x >> 43]++] = x;
+    }
+    uint64_t *q = b.data();
+    auto dist = [&](uint64_t x) { return v[(uint32_t)x].second; };
+    for (size_t r0 = 0; r0 < n;) {
+        size_t r1 = r0 + 1;
+        while (r1 < n && (q[r1] >> 32) == (q[r0] >> 32)) ++r1;
+        if (r1 - r0 > 32) {
+            std::sort(q + r0, q + r1, [&](uint64_t x, uint64_t y) { return dist(x) < dist(y); });
         } else {
-            for (size_t i = s + 1; i < e; ++i) {
-                PD x = tmp[i];
+            for (size_t i = r0 + 1; i < r1; ++i) {
+                const uint64_t x = q[i];
+                const double xd = dist(x);
                 size_t j = i;
-                while (j > s && x.second < tmp[j - 1].second) {
-                    tmp[j] = tmp[j - 1];
+                while (j > r0 && xd < dist(q[j - 1])) {
+                    q[j] = q[j - 1];
                     --j;
                 }
-                tmp[j] = x;
+                q[j] = x;
             }
         }
-        s = e;
+        r0 = r1;
     }
     for (size_t i = 1; i < n; ++i)
-        if (tmp[i].second == tmp[i - 1].second) {
+        if ((q[i] >> 32) == (q[i - 1] >> 32) && dist(q[i]) == dist(q[i - 1])) {
             std::sort(v.begin(), v.end(), pd_less);
             return;
         }
+    tmp.resize(n);
+    for (size_t i = 0; i < n; ++i) tmp[i] = v[(uint32_t)q[i]];
     v.swap(tmp);
 }
 
@@ -223,9 +255,9 @@ struct Ops {
     static void sort_by_distance(std::vector<PD> &v, bool nearly_sorted)
     {
         static thread_local std::vector<PD> tmp;
-        static thread_local std::vector<uint32_t> cnt;
+        static thread_local std::vector<uint64_t> ka, kb;
         if (!nearly_sorted || v.size() < 32) {
-            sort_by_distance_buckets(v, tmp, cnt);
+            sort_by_distance_keys(v, tmp, ka, kb);
             return;
         }
         static thread_local std::vector<PD> orig;
@@ -253,7 +285,7 @@ struct Ops {
                 }
         if (!ok) {
             v.swap(orig);
-            sort_by_distance_buckets(v, tmp, cnt);
+            sort_by_distance_keys(v, tmp, ka, kb);
         }
     }
     /* std::sort of v = A ++ B by distance, where A (the first na entries) is
@@ -270,7 +302,7 @@ struct Ops {
             return true;
         }
         static thread_local std::vector<PD> a, b, m, tmp;
-        static thread_local std::vector<uint32_t> cnt;
+        static thread_local std::vector<uint64_t> ka, kb;
         a.assign(v.begin(), v.begin() + na);
         b.assign(v.begin() + na, v.end());
         size_t budget = 16 * a.size() + 1024, moves = 0;
@@ -288,10 +320,10 @@ struct Ops {
             }
             a[j] = x;
         }
-        if (far) sort_by_distance_buckets(a, tmp, cnt);
+        if (far) sort_by_distance_keys(a, tmp, ka, kb);
         bool ok = true;
         {
-            sort_by_distance_buckets(b, tmp, cnt);
+            sort_by_distance_keys(b, tmp, ka, kb);
             m.resize(v.size());
             std::merge(a.begin(), a.end(), b.begin(), b.end(), m.begin(), pd_less);
             for (size_t i = 1; i < m.size(); ++i)
@@ -738,7 +770,7 @@ struct RbLocal {
     std::vector<KinRow> kin;
     std::vector<uint32_t> h, ord;
     std::vector<PD> v, tmp;
-    std::vector<uint32_t> cnt;
+    std::vector<uint64_t> ka, kb;
     size_t n_sorted = 0;
     size_t size() const { return h.size(); }
     /* room for the members plus the superset they are claimed from */
@@ -854,7 +886,7 @@ static void rb_resort(HClump &c, RbLocal &L)
     L.v.resize(n);
     for (size_t i = 0; i < ns; ++i) L.v[i] = PD(L.ord[i], D[L.ord[i]]);
     for (size_t p = ns; p < n; ++p) L.v[p] = PD((uint32_t)p, D[p]);
-    sort_by_distance_buckets(L.v, L.tmp, L.cnt);
+    sort_by_distance_keys(L.v, L.tmp, L.ka, L.kb);
     L.ord.resize(n);
     for (size_t i = 0; i < n; ++i) L.ord[i] = L.v[i].first;
     L.n_sorted = n;
