@@ -158,6 +158,74 @@ def boundness_bench(ctx, args, params) -> dict:
     return out
 
 
+def series_bench(snap, steps: int, contexts: int = 2) -> dict:
+    """The same steps as a series of snapshots on `contexts` device contexts
+    at once (the reference's snapshot loop, src/main.cpp:56-79): one
+    snapshot's host clump logic overlaps the next one's kernels.  Resident:
+    each context restores its own device copy of the snapshot per step;
+    with_upload: each step starts from the float32 snapshot in host memory.
+    Every context runs one untimed step first."""
+    import threading
+
+    import plan_amd
+
+    ctxs = [plan_amd.Context(0) for _ in range(contexts)]
+    try:
+        for c in ctxs:
+            c.set_particles(snap.params, snap.pos, snap.vel, snap.density, snap.property_index, snap.id)
+            c.keep_input()
+        out = {"contexts": contexts, "snapshots": steps}
+        for mode in ("resident", "with_upload"):
+            def one(c):
+                if mode == "resident":
+                    c.restore_input()
+                else:
+                    c.set_particles(snap.params, snap.pos, snap.vel, snap.density, snap.property_index, snap.id)
+                c.make_ghosts()
+                c.build_tree(True)
+                return len(c.find_planetesimals()["clumps"])
+
+            for c in ctxs:
+                one(c)
+            lock = threading.Lock()
+            todo = [steps]
+            found = []
+            errors = []
+            stages = []
+
+            def worker(c):
+                try:
+                    while True:
+                        with lock:
+                            if todo[0] == 0:
+                                return
+                            todo[0] -= 1
+                        found.append(one(c))
+                        st = c.stage_times()
+                        st["clump_logic"] = c.clump_stats()["ms_clump_logic"]
+                        stages.append(st)
+                except BaseException as e:  # reported after the join
+                    errors.append(e)
+
+            th = [threading.Thread(target=worker, args=(c,)) for c in ctxs]
+            t0 = time.perf_counter()
+            for t in th:
+                t.start()
+            for t in th:
+                t.join()
+            dt = time.perf_counter() - t0
+            if errors:
+                raise errors[0]
+            out[mode] = {"value": snap.n * steps / dt, "unit": "particles/s", "ms_per_snapshot": dt / steps * 1e3,
+                         "clumps_found": sorted(set(found)),
+                         "stage_ms": {k: float(np.mean([x[k] for x in stages])) for k in stages[0]}}
+            log(f"[bench] series {mode}: {out[mode]}")
+        return out
+    finally:
+        for c in ctxs:
+            c.close()
+
+
 def cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as f:
@@ -207,6 +275,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-boundness", action="store_true")
     ap.add_argument("--boundness-clumps", type=int, default=5000)
+    ap.add_argument("--no-series", action="store_true")
+    ap.add_argument("--series-contexts", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -269,6 +339,9 @@ def main():
     achieved = DENSITY_BYTES_PER_PARTICLE * launch_units / (k_ms * 1e-3) / 1e9
     traffic = measured_traffic(launch_units)
     mean_st = {k: float(np.mean([s[k] for s in stages])) for k in stages[0]}
+    series = None
+    if not args.no_series:
+        series = series_bench(snap, args.steps, args.series_contexts)
     boundness = None
     if not args.no_boundness:
         boundness = boundness_bench(ctx, args, snap.params)
@@ -302,6 +375,7 @@ def main():
             "value": args.n / (up_total / args.steps), "unit": "particles/s", "ms_per_step": up_total / args.steps * 1e3,
             "note": "each step starts from the float32 snapshot in host memory (plan_set_particles_f32 over PCIe)",
         },
+        "series": series,
         "roofline": {
             "kernel": "k_knn_select<32,false> (density pass: walk, K-nearest selection, median density)",
             "bound": "hbm",

@@ -238,7 +238,12 @@ int plan_stage_times(plan_ctx *ctx, double *ms, int n);
  *  pow, which the reference calls (src/tree.hpp:3489), 45 those of them
  *  where that pow rounds the other way and which were redone with it,
  *  46 host ms building the plan_catalog, 47 host ms of the whole clump logic
- *  (merging to ranking) */
+ *  (merging to ranking),
+ *  48-51 the slowest clump's rebinding: ms scanning its live candidate
+ *  records, ms evaluating candidate energies and claiming, live records at
+ *  its start, candidate energies evaluated,
+ *  52-54 all rebinding runs: summed ms of live-record scans, of energy
+ *  evaluation and of member re-sorts */
 int plan_clump_stats(plan_ctx *ctx, double *values, int n);
 /* device milliseconds of the last density pass's gather kernel (the K-nearest
  * selection and density, one launch) and of its radius-schedule kernel */

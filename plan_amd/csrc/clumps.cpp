@@ -17,6 +17,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <exception>
 #include <limits>
 #include <stdexcept>
@@ -738,7 +739,9 @@ struct Rebind {
     std::vector<uint32_t> claimed;
     uint32_t exact_searches = 0;
     uint32_t iters = 0;
-    double ms = 0, ms_exact = 0, ms_sort = 0;
+    double ms = 0, ms_exact = 0, ms_sort = 0, ms_scan = 0, ms_energy = 0;
+    /* live records at the start; candidate energies evaluated */
+    uint64_t live0 = 0, evals = 0;
     std::chrono::steady_clock::time_point t0, t1; /* start and end of the run */
 };
 
@@ -1051,6 +1054,58 @@ static EnergyBlockFn energy_block_fn()
     return f;
 }
 
+/* Screening of candidate energies without divisions or roots: with
+ * h = |u|^2 / 2 (u as in rebind_energy) and q the squared distance, the
+ * energy h / (M + m) - G / sqrt(q) is positive exactly when
+ * h^2 q > (G (M + m))^2.  Both sides are formed in binary64 (relative error
+ * below 1e-14); when the left exceeds the right by a relative 2e-9, the
+ * energy's own binary64 evaluation (a few roundings of 2^-53 on each term)
+ * is positive too: the record is unbound and would not be claimed.
+ * need[j] = 1 for a record in the ball (F bit 0) without that proof (also
+ * when a side leaves 1e-290 .. 1e290 or is not a number).  Records [0, n)
+ * of the live arrays, contiguous. */
+#define PLAN_ENERGY_SCREEN_BODY                                                   \
+    for (size_t j = 0; j < n; ++j) {                                              \
+        const double a = S.cx - x[j];                                             \
+        const double d0 = (S.vx - 0.0 * a) - vx[j];                               \
+        const double d1 = (S.vy - S.sh * a) - vy[j];                              \
+        const double d2 = (S.vz - 0.0 * a) - vz[j];                               \
+        const double t1 = S.cy - y[j], t2 = S.cz - z[j];                          \
+        const double h = 0.5 * (d0 * d0 + d1 * d1 + d2 * d2);                     \
+        const double q = a * a + t1 * t1 + t2 * t2;                               \
+        const double gt = S.G * (S.M + m[j]);                                     \
+        const double lhs = h * h * q, rhs = gt * gt;                              \
+        const bool sure = (rhs > 1e-290) & (lhs < 1e290) & (lhs * (1.0 - 1e-9) > rhs * (1.0 + 1e-9)); \
+        need[j] = (uint8_t)((F[j] & 1) & !sure);                                  \
+    }
+#define PLAN_ENERGY_SCREEN_ARGS                                                                                     \
+    const EnergyState &S, const double *__restrict x, const double *__restrict y, const double *__restrict z,    \
+        const double *__restrict vx, const double *__restrict vy, const double *__restrict vz,                    \
+        const double *__restrict m, const uint8_t *__restrict F, size_t n, uint8_t *__restrict need
+static void energy_screen_base(PLAN_ENERGY_SCREEN_ARGS)
+{
+    PLAN_ENERGY_SCREEN_BODY
+}
+__attribute__((target("avx2"))) static void energy_screen_avx2(PLAN_ENERGY_SCREEN_ARGS)
+{
+    PLAN_ENERGY_SCREEN_BODY
+}
+__attribute__((target("avx512f,avx512bw,avx512vl"))) static void energy_screen_avx512(PLAN_ENERGY_SCREEN_ARGS)
+{
+    PLAN_ENERGY_SCREEN_BODY
+}
+#undef PLAN_ENERGY_SCREEN_BODY
+#undef PLAN_ENERGY_SCREEN_ARGS
+typedef void (*EnergyScreenFn)(const EnergyState &, const double *, const double *, const double *, const double *,
+                               const double *, const double *, const double *, const uint8_t *, size_t, uint8_t *);
+static EnergyScreenFn energy_screen_fn()
+{
+    static const EnergyScreenFn f = cpu_has_avx512()                 ? energy_screen_avx512
+                                    : __builtin_cpu_supports("avx2") ? energy_screen_avx2
+                                                                     : energy_screen_base;
+    return f;
+}
+
 /* One iteration's scan of the live records: squared distance s to the clump
  * centre, bit 0 = within the ball (s <= r2) and near c0, bit 1 = in the band
  * around the sphere; returns the number of band records.  The same
@@ -1165,6 +1220,7 @@ static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
             if (!flags[h] && !mine.count(h)) live.push(hm.rec[h], h, Ops::dist_pts(rb.c0, &hm.rec[h].x));
     };
     rebuild();
+    rb.live0 = live.size();
     RbLocal L;
     L.reserve(c.indices.size() + rb.sup.size());
     for (uint32_t h : c.indices) L.push(hm.rec[h], h);
@@ -1175,11 +1231,12 @@ static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
     std::vector<uint32_t> cand, cpos;
     std::vector<uint8_t> flag;
     size_t ndead = 0;
-    constexpr size_t BLK = 32;
+    constexpr size_t SCREEN = 128;
     const double inf = std::numeric_limits<double>::infinity();
-    double eb[BLK];
     const EnergyBlockFn energy_block = energy_block_fn();
+    const EnergyScreenFn energy_screen = energy_screen_fn();
     const LiveScanFn live_scan = live_scan_fn();
+    alignas(8) uint8_t need[SCREEN];
     size_t before;
     auto claim = [&](uint32_t h, const double *qp, const double *qv, double qm) {
         c.total_mass = c.total_mass + qm;
@@ -1198,6 +1255,7 @@ static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
         bool exact = dc + r + margin > rb.R;
         cand.clear();
         cpos.clear();
+        const auto tsc = std::chrono::steady_clock::now();
         if (!exact) {
             /* a member farther than this from c0 is outside the ball by more
              * than the margin (triangle inequality, generous rounding slack) */
@@ -1223,6 +1281,8 @@ static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
                 for (size_t k = 0; k < nl; ++k)
                     if (F[k] & 1) cpos.push_back((uint32_t)k);
         }
+        const auto ten = std::chrono::steady_clock::now();
+        rb.ms_scan += std::chrono::duration<double, std::milli>(ten - tsc).count();
         if (exact) {
             auto te = std::chrono::steady_clock::now();
             std::vector<double4> qs{make_double4(c.com[0], c.com[1], c.com[2], r),
@@ -1244,35 +1304,55 @@ static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
             rebuild();
             ndead = 0;
         } else if (!cpos.empty()) {
-            size_t i = 0;
-            const size_t nc = cpos.size();
-            while (i < nc) {
-                const size_t e = std::min(nc, i + BLK);
+            /* the live records in order, SCREEN at a time: those in the ball
+             * that the screen cannot prove unbound get the exact energy
+             * against the current state; after a claim (which changes the
+             * state) the screen restarts at the next record */
+            const uint8_t *F = flag.data();
+            size_t k = cpos.front();
+            const size_t kend = (size_t)cpos.back() + 1;
+            while (k < kend) {
+                const size_t e = std::min(kend, k + SCREEN);
                 const EnergyState es{c.total_mass, c.com[0], c.com[1], c.com[2], c.vel[0], c.vel[1], c.vel[2], sh, cs.G};
-                energy_block(es, live.x.data(), live.y.data(), live.z.data(), live.vx.data(), live.vy.data(),
-                             live.vz.data(), live.m.data(), cpos.data() + i, e - i, eb);
+                energy_screen(es, live.x.data() + k, live.y.data() + k, live.z.data() + k, live.vx.data() + k,
+                              live.vy.data() + k, live.vz.data() + k, live.m.data() + k, F + k, e - k, need);
+                const size_t ne = e - k;
                 size_t f = e;
-                for (size_t j = i; j < e; ++j)
-                    if (eb[j - i] < 0) {
-                        f = j;
-                        break;
+                for (size_t t0 = 0; t0 < ne; t0 += 8) {
+                    uint64_t w;
+                    std::memcpy(&w, need + t0, 8);
+                    if (t0 + 8 > ne) w &= ~0ull >> (8 * (t0 + 8 - ne));
+                    while (w) {
+                        const size_t t = t0 + (size_t)(__builtin_ctzll(w) >> 3);
+                        w &= ~(0xffull << (8 * (t - t0)));
+                        const uint32_t kk = (uint32_t)(k + t);
+                        double ex;
+                        energy_block(es, live.x.data(), live.y.data(), live.z.data(), live.vx.data(),
+                                     live.vy.data(), live.vz.data(), live.m.data(), &kk, 1, &ex);
+                        rb.evals++;
+                        if (ex < 0) {
+                            f = k + t;
+                            break;
+                        }
                     }
+                    if (f != e) break;
+                }
                 if (f == e) {
-                    i = e;
+                    k = e;
                     continue;
                 }
-                const uint32_t k = cpos[f];
-                const double qp[3] = {live.x[k], live.y[k], live.z[k]};
-                const double qv[3] = {live.vx[k], live.vy[k], live.vz[k]};
-                claim(live.h[k], qp, qv, live.m[k]);
-                live.lsd[k] = inf; /* beyond every bound: skipped from now on */
+                const double qp[3] = {live.x[f], live.y[f], live.z[f]};
+                const double qv[3] = {live.vx[f], live.vy[f], live.vz[f]};
+                claim(live.h[f], qp, qv, live.m[f]);
+                live.lsd[f] = inf; /* beyond every bound: skipped from now on */
                 ++ndead;
-                i = f + 1;
+                k = f + 1;
             }
             if (ndead > live.size() / 4) {
                 live.compact();
                 ndead = 0;
             }
+            rb.ms_energy += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ten).count();
         }
         if (L.size() > L.n_sorted) {
             auto ts = std::chrono::steady_clock::now();
@@ -1385,7 +1465,14 @@ static void rebind_all(std::map<uint32_t, HClump> &cl, const HostMembers &hm, co
             st.rebind_max_exact = rb[i].exact_searches;
             st.rebind_max_exact_ms = rb[i].ms_exact;
             st.rebind_max_sort_ms = rb[i].ms_sort;
+            st.rebind_max_scan_ms = rb[i].ms_scan;
+            st.rebind_max_energy_ms = rb[i].ms_energy;
+            st.rebind_max_live = (double)rb[i].live0;
+            st.rebind_max_evals = (double)rb[i].evals;
         }
+        st.rebind_scan_total_ms += rb[i].ms_scan;
+        st.rebind_energy_total_ms += rb[i].ms_energy;
+        st.rebind_sort_total_ms += rb[i].ms_sort;
     }
     part(2);
 }

@@ -100,6 +100,11 @@ struct ClumpStats {
     uint32_t key0_erased = 0;
     uint32_t rebind_iters = 0, rebind_max_iters = 0, rebind_max_n = 0, rebind_max_exact = 0;
     double rebind_max_ms = 0, rebind_max_exact_ms = 0, rebind_max_sort_ms = 0;
+    /* the longest run's live-record scans, energy blocks, initial live
+     * records and energy evaluations; scans, energies and re-sorts summed
+     * over all runs */
+    double rebind_max_scan_ms = 0, rebind_max_energy_ms = 0, rebind_max_live = 0, rebind_max_evals = 0;
+    double rebind_scan_total_ms = 0, rebind_energy_total_ms = 0, rebind_sort_total_ms = 0;
     /* rebinding runs: summed time, summed exact-search time (waits included),
      * schedule position of the longest run */
     double rebind_work_ms = 0, rebind_exact_total_ms = 0;

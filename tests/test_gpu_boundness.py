@@ -197,3 +197,59 @@ def test_precision_form_on_a_snapshot_catalog_with_seeded_virial_clumps():
         print(f"  seeded clump {c}: n={off[c + 1] - off[c]} E/|E_pot| {tot / abs(oo['E_pot'][c]):+.2e}")
     assert err.max() <= 1e-6
     assert st["refit_clumps"] >= 1
+
+
+def _check_fp32_against_oracle(b, ob, oo, label):
+    """the batched FP32 form against the oracle: bound sets equal up to
+    near-zero members, per-clump energy errors within 1e-6, reproducible"""
+    ctx = _ctx()
+    ctx.boundness_load(b)
+    st = ctx.boundness_run(ctx.FP32)
+    gb, go = ctx.boundness_result()
+    diff = gb != ob
+    if diff.any():
+        nearz = near_zero_members(b, ob, 1e-6)
+        assert np.all(nearz[diff]), np.flatnonzero(diff & ~nearz)[:10]
+    same = go["n_bound"] == oo["n_bound"]
+    ok = same & (oo["n_bound"] > 1)
+    rel_pot = np.abs(go["E_pot"][ok] - oo["E_pot"][ok]) / np.abs(oo["E_pot"][ok])
+    rel_kin = np.abs(go["E_kin"][ok] - oo["E_kin"][ok]) / np.abs(oo["E_kin"][ok])
+    rel = np.maximum(rel_pot, rel_kin)
+    print(f"[{label}] clumps={b.num_clumps} members={len(b.mass)} same bound count {same.sum()}; per-clump "
+          f"relative energy error p50 {np.median(rel):.2e} p99 {np.percentile(rel, 99):.2e} max {rel.max():.2e}; "
+          f"refit {st['refit_clumps']}, replays {st['graph_replays']}, shape {st['tile_shape']}", flush=True)
+    assert rel.max() <= 1e-6
+    st2 = ctx.boundness_run(ctx.FP32)
+    gb2, go2 = ctx.boundness_result()
+    assert np.array_equal(gb2, gb)
+    assert np.array_equal(go2["E_pot"], go["E_pot"])
+    return st, go, ok
+
+
+def test_fp32_batched_5000_clumps_matches_oracle():
+    """The batching workload's schedule at its full clump count: 5,000
+    Plummer clumps (32..8,000 members, dN/dlnN ~ N^-0.6, 0-30% interlopers,
+    2.35M members, three seeded at zero total energy) in one persistent,
+    graph-captured launch, every clump against the FP64 oracle."""
+    import oracle_py as O
+    from plan_amd import synth
+
+    b = synth.boundness_batch(n_clumps=5000, n_min=32, n_max=8000, seed=2, virial_targets={7: 1.0, 100: 1.0, 2000: 1.0})
+    O.set_threads(16)
+    ob, oo = O.boundness(b.offsets, b.pos, b.vel, b.mass, b.G, b.Omega, b.q)
+    st, go, ok = _check_fp32_against_oracle(b, ob, oo, "boundness fp32, 5000 clumps")
+    assert ok.sum() >= 0.99 * b.num_clumps
+    assert st["refit_clumps"] >= 1
+
+
+def test_fp32_batched_40k_member_clump_matches_oracle():
+    """One 40,000-member clump (plus interlopers): the largest tile counts
+    the FP64 oracle checks in seconds."""
+    import oracle_py as O
+    from plan_amd import synth
+
+    b = synth.boundness_batch(n_clumps=1, n_min=40000, n_max=40000, seed=3)
+    O.set_threads(16)
+    ob, oo = O.boundness(b.offsets, b.pos, b.vel, b.mass, b.G, b.Omega, b.q)
+    st, go, ok = _check_fp32_against_oracle(b, ob, oo, "boundness fp32, 40k members")
+    assert ok.all()
