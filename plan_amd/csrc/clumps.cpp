@@ -1228,7 +1228,7 @@ static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
     for (size_t i = 0; i < L.size(); ++i) L.ord[i] = (uint32_t)i;
     L.n_sorted = L.size();
     const size_t n0 = L.size();
-    std::vector<uint32_t> cand, cpos;
+    std::vector<uint32_t> cand;
     std::vector<uint8_t> flag;
     size_t ndead = 0;
     constexpr size_t SCREEN = 128;
@@ -1254,7 +1254,7 @@ static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
         const double dc = Ops::dist_pts(c.com, rb.c0);
         bool exact = dc + r + margin > rb.R;
         cand.clear();
-        cpos.clear();
+        size_t klo = live.size(), khi = 0; /* records in the ball lie in [klo, khi) */
         const auto tsc = std::chrono::steady_clock::now();
         if (!exact) {
             /* a member farther than this from c0 is outside the ball by more
@@ -1277,9 +1277,45 @@ static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
             if (nband)
                 for (size_t k = 0; k < nl && !exact; ++k)
                     if ((F[k] & 2) && std::fabs(std::sqrt(S[k]) - r) <= margin) exact = true;
-            if (!exact)
-                for (size_t k = 0; k < nl; ++k)
-                    if (F[k] & 1) cpos.push_back((uint32_t)k);
+            if (!exact) {
+                /* first and last record in the ball (flag bit 0), 8 flags
+                 * at a time */
+                constexpr uint64_t B0 = 0x0101010101010101ull;
+                size_t k = 0;
+                for (; k + 8 <= nl; k += 8) {
+                    uint64_t w;
+                    std::memcpy(&w, F + k, 8);
+                    if (w & B0) {
+                        klo = k + (size_t)(__builtin_ctzll(w & B0) >> 3);
+                        break;
+                    }
+                }
+                if (klo == nl)
+                    for (; k < nl; ++k)
+                        if (F[k] & 1) {
+                            klo = k;
+                            break;
+                        }
+                if (klo < nl) {
+                    /* F[klo] bit 0 is set, so the walk back stops above klo */
+                    size_t t = nl;
+                    while (t > klo) {
+                        if (t >= klo + 8) {
+                            uint64_t w;
+                            std::memcpy(&w, F + t - 8, 8);
+                            if (!(w & B0)) {
+                                t -= 8;
+                                continue;
+                            }
+                            t = t - 8 + (size_t)((63 - __builtin_clzll(w & B0)) >> 3) + 1;
+                            break;
+                        }
+                        if (F[t - 1] & 1) break;
+                        --t;
+                    }
+                    khi = t;
+                }
+            }
         }
         const auto ten = std::chrono::steady_clock::now();
         rb.ms_scan += std::chrono::duration<double, std::milli>(ten - tsc).count();
@@ -1293,7 +1329,6 @@ static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
             to_records(hm, l[0], cand);
             to_records(hm, l[1], rb.sup);
             rb.ms_exact += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - te).count();
-            cpos.clear();
             for (int k = 0; k < 3; ++k) rb.c0[k] = c.com[k];
             rb.R = 2.0 * r;
             for (uint32_t h : cand) {
@@ -1303,14 +1338,14 @@ static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
             }
             rebuild();
             ndead = 0;
-        } else if (!cpos.empty()) {
+        } else if (klo < khi) {
             /* the live records in order, SCREEN at a time: those in the ball
              * that the screen cannot prove unbound get the exact energy
              * against the current state; after a claim (which changes the
              * state) the screen restarts at the next record */
             const uint8_t *F = flag.data();
-            size_t k = cpos.front();
-            const size_t kend = (size_t)cpos.back() + 1;
+            size_t k = klo;
+            const size_t kend = khi;
             while (k < kend) {
                 const size_t e = std::min(kend, k + SCREEN);
                 const EnergyState es{c.total_mass, c.com[0], c.com[1], c.com[2], c.vel[0], c.vel[1], c.vel[2], sh, cs.G};
