@@ -60,8 +60,10 @@ uint32_t HostMembers::rec_of_tidx(uint32_t tidx) const
 }
 
 /* std::sort(v) by distance for non-negative distances, in linear time: the
- * entries become (22-bit fixed-point distance << 32 | position) items,
- * sorted by two 11-bit LSD radix passes; the key is monotone in the
+ * entries become (fixed-point distance << 32 | position) items, sorted by
+ * two LSD radix passes (22-bit keys and 11-bit digits; 16-bit keys and 8-bit
+ * digits below 4,096 entries, where the histograms' fixed cost would
+ * dominate); the key is monotone in the
  * distance and the passes are stable, so only runs of equal keys can be out
  * of distance order, and an insertion pass over those runs finishes the
  * sort.  Without distance ties the sorted order is unique, so this is
@@ -78,11 +80,13 @@ static void sort_by_distance_keys(std::vector<PD> &v, std::vector<PD> &tmp, std:
         std::sort(v.begin(), v.end(), pd_less);
         return;
     }
-    constexpr uint32_t KMAX = (1u << 22) - 1, DIG = 2048;
+    const uint32_t DB = n < 4096 ? 8u : 11u, DIG = 1u << DB, KMAX = (1u << (2 * DB)) - 1;
     const double scale = (double)KMAX / dmax;
     a.resize(n);
     b.resize(n);
-    uint32_t h0[DIG] = {}, h1[DIG] = {};
+    uint32_t h0[2048], h1[2048];
+    std::memset(h0, 0, DIG * sizeof(uint32_t));
+    std::memset(h1, 0, DIG * sizeof(uint32_t));
     bool bad = false;
     for (size_t i = 0; i < n; ++i) {
         const double dd = v[i].second;
@@ -91,7 +95,7 @@ static void sort_by_distance_keys(std::vector<PD> &v, std::vector<PD> &tmp, std:
         const uint32_t k = f >= (double)KMAX ? KMAX : f > 0 ? (uint32_t)f : 0u;
         b[i] = (uint64_t)k << 32 | i;
         h0[k & (DIG - 1)]++;
-        h1[k >> 11]++;
+        h1[k >> DB]++;
     }
     if (bad) {
         std::sort(v.begin(), v.end(), pd_less);
@@ -110,7 +114,7 @@ static void sort_by_distance_keys(std::vector<PD> &v, std::vector<PD> &tmp, std:
     }
     for (size_t i = 0; i < n; ++i) {
         const uint64_t x = a[i];
-        b[h1[x >> 43]++] = x;
+        b[h1[x >> (32 + DB)]++] = x;
     }
     uint64_t *q = b.data();
     auto dist = [&](uint64_t x) { return v[(uint32_t)x].second; };
