@@ -202,7 +202,10 @@ def series_bench(snap, steps: int, contexts: int = 2) -> dict:
                             todo[0] -= 1
                         found.append(one(c))
                         st = c.stage_times()
-                        st["clump_logic"] = c.clump_stats()["ms_clump_logic"]
+                        cst = c.clump_stats()
+                        for k in ("ms_clump_logic", "ms_sort", "ms_mergeA", "ms_rebind", "rebind_max_ms",
+                                  "ms_merge_saddle", "rebind_exact_total_ms", "d2h_ms"):
+                            st[k] = cst[k]
                         stages.append(st)
                 except BaseException as e:  # reported after the join
                     errors.append(e)

@@ -147,6 +147,82 @@ static void sort_by_distance_keys(std::vector<PD> &v, std::vector<PD> &tmp, std:
     v.swap(tmp);
 }
 
+/* The distance order of the positions seq[0, n) (distances D[p] >= 0, no
+ * pairs built): the same two LSD radix passes and run repair as
+ * sort_by_distance_keys on items (key << 32 | position), written to out.
+ * Without distance ties the order is unique, so it is std::sort's on the
+ * sequence of (position, distance) pairs; false (out unspecified) when two
+ * distances tie, a distance is not a non-negative number, or n is out of the
+ * radix range — the caller then sorts the pairs. */
+static bool radix_order(const double *D, const uint32_t *seq, size_t n, std::vector<uint64_t> &a,
+                        std::vector<uint64_t> &b, uint32_t *out)
+{
+    if (n < 64 || n >= (1ull << 32)) return false;
+    double dmax = 0;
+    bool bad = false;
+    for (size_t i = 0; i < n; ++i) {
+        const double dd = D[seq[i]];
+        bad |= !(dd >= 0);
+        dmax = std::max(dmax, dd);
+    }
+    if (bad || !(dmax > 0) || !std::isfinite(dmax)) return false;
+    const uint32_t DB = n < 4096 ? 8u : 11u, DIG = 1u << DB, KMAX = (1u << (2 * DB)) - 1;
+    const double scale = (double)KMAX / dmax;
+    a.resize(n);
+    b.resize(n);
+    uint32_t h0[2048], h1[2048];
+    std::memset(h0, 0, DIG * sizeof(uint32_t));
+    std::memset(h1, 0, DIG * sizeof(uint32_t));
+    for (size_t i = 0; i < n; ++i) {
+        const uint32_t p = seq[i];
+        const double f = D[p] * scale;
+        const uint32_t k = f >= (double)KMAX ? KMAX : f > 0 ? (uint32_t)f : 0u;
+        b[i] = (uint64_t)k << 32 | p;
+        h0[k & (DIG - 1)]++;
+        h1[k >> DB]++;
+    }
+    for (uint32_t d = 0, s0 = 0, s1 = 0; d < DIG; ++d) {
+        const uint32_t t0 = h0[d], t1 = h1[d];
+        h0[d] = s0;
+        h1[d] = s1;
+        s0 += t0;
+        s1 += t1;
+    }
+    for (size_t i = 0; i < n; ++i) {
+        const uint64_t x = b[i];
+        a[h0[(x >> 32) & (DIG - 1)]++] = x;
+    }
+    for (size_t i = 0; i < n; ++i) {
+        const uint64_t x = a[i];
+        b[h1[x >> (32 + DB)]++] = x;
+    }
+    uint64_t *q = b.data();
+    auto dist = [&](uint64_t x) { return D[(uint32_t)x]; };
+    for (size_t r0 = 0; r0 < n;) {
+        size_t r1 = r0 + 1;
+        while (r1 < n && (q[r1] >> 32) == (q[r0] >> 32)) ++r1;
+        if (r1 - r0 > 32) {
+            std::sort(q + r0, q + r1, [&](uint64_t x, uint64_t y) { return dist(x) < dist(y); });
+        } else {
+            for (size_t i = r0 + 1; i < r1; ++i) {
+                const uint64_t x = q[i];
+                const double xd = dist(x);
+                size_t j = i;
+                while (j > r0 && xd < dist(q[j - 1])) {
+                    q[j] = q[j - 1];
+                    --j;
+                }
+                q[j] = x;
+            }
+        }
+        r0 = r1;
+    }
+    for (size_t i = 1; i < n; ++i)
+        if ((q[i] >> 32) == (q[i - 1] >> 32) && dist(q[i]) == dist(q[i - 1])) return false;
+    for (size_t i = 0; i < n; ++i) out[i] = (uint32_t)q[i];
+    return true;
+}
+
 namespace {
 
 /* Loops over a clump's members visit the 80-byte member records in
@@ -775,7 +851,7 @@ struct alignas(64) KinRow {
 struct RbLocal {
     std::vector<double> x, y, z, m, rho, d;
     std::vector<KinRow> kin;
-    std::vector<uint32_t> h, ord;
+    std::vector<uint32_t> h, ord, seq;
     std::vector<PD> v, tmp;
     std::vector<uint64_t> ka, kb;
     size_t n_sorted = 0;
@@ -890,12 +966,17 @@ static void rb_resort(HClump &c, RbLocal &L)
     static const DistFn dist_to_centre = dist_fn();
     double *D = L.d.data();
     dist_to_centre(c.com[0], c.com[1], c.com[2], L.x.data(), L.y.data(), L.z.data(), n, D);
-    L.v.resize(n);
-    for (size_t i = 0; i < ns; ++i) L.v[i] = PD(L.ord[i], D[L.ord[i]]);
-    for (size_t p = ns; p < n; ++p) L.v[p] = PD((uint32_t)p, D[p]);
-    sort_by_distance_keys(L.v, L.tmp, L.ka, L.kb);
+    /* the sequence summed: the last order, then the claims in claim order */
+    L.seq.resize(n);
+    std::memcpy(L.seq.data(), L.ord.data(), ns * sizeof(uint32_t));
+    for (size_t p = ns; p < n; ++p) L.seq[p] = (uint32_t)p;
     L.ord.resize(n);
-    for (size_t i = 0; i < n; ++i) L.ord[i] = L.v[i].first;
+    if (!radix_order(D, L.seq.data(), n, L.ka, L.kb, L.ord.data())) {
+        L.v.resize(n);
+        for (size_t i = 0; i < n; ++i) L.v[i] = PD(L.seq[i], D[L.seq[i]]);
+        sort_by_distance_keys(L.v, L.tmp, L.ka, L.kb);
+        for (size_t i = 0; i < n; ++i) L.ord[i] = L.v[i].first;
+    }
     L.n_sorted = n;
 }
 
@@ -907,8 +988,7 @@ static void rb_finish(HClump &c, const RbLocal &L)
     c.particles.resize(n);
     c.indices.resize(n);
     for (size_t i = 0; i < n; ++i) {
-        c.particles[i] = L.v[i];
-        c.particles[i].first = L.h[L.v[i].first];
+        c.particles[i] = PD(L.h[L.ord[i]], L.d[L.ord[i]]);
         c.indices[i] = c.particles[i].first;
     }
     const double tenth = L.rho[L.ord[n / 2]] / 10.;
@@ -1235,7 +1315,7 @@ static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
     std::vector<uint32_t> cand;
     std::vector<uint8_t> flag;
     size_t ndead = 0;
-    constexpr size_t SCREEN = 128;
+    constexpr size_t SCREEN = 256;
     const double inf = std::numeric_limits<double>::infinity();
     const EnergyBlockFn energy_block = energy_block_fn();
     const EnergyScreenFn energy_screen = energy_screen_fn();
@@ -1343,15 +1423,17 @@ static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
             rebuild();
             ndead = 0;
         } else if (klo < khi) {
-            /* the live records in order, SCREEN at a time: those in the ball
-             * that the screen cannot prove unbound get the exact energy
-             * against the current state; after a claim (which changes the
-             * state) the screen restarts at the next record */
+            /* the live records in order, in chunks: those in the ball that
+             * the screen cannot prove unbound get the exact energy against
+             * the current state; after a claim (which changes the state) the
+             * screen restarts at the next record with a chunk of 16, and the
+             * chunk doubles (up to SCREEN) while no claim interrupts it, so
+             * little screening is redone after a claim */
             const uint8_t *F = flag.data();
-            size_t k = klo;
+            size_t k = klo, chunk = 16;
             const size_t kend = khi;
             while (k < kend) {
-                const size_t e = std::min(kend, k + SCREEN);
+                const size_t e = std::min(kend, k + chunk);
                 const EnergyState es{c.total_mass, c.com[0], c.com[1], c.com[2], c.vel[0], c.vel[1], c.vel[2], sh, cs.G};
                 energy_screen(es, live.x.data() + k, live.y.data() + k, live.z.data() + k, live.vx.data() + k,
                               live.vy.data() + k, live.vz.data() + k, live.m.data() + k, F + k, e - k, need);
@@ -1378,8 +1460,10 @@ static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
                 }
                 if (f == e) {
                     k = e;
+                    chunk = std::min(SCREEN, 2 * chunk);
                     continue;
                 }
+                chunk = 16;
                 const double qp[3] = {live.x[f], live.y[f], live.z[f]};
                 const double qv[3] = {live.vx[f], live.vy[f], live.vz[f]};
                 claim(live.h[f], qp, qv, live.m[f]);
