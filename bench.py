@@ -274,7 +274,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=16777216)
+    ap.add_argument("--n", type=int, default=None,
+                    help="particles (default 16,777,216 for the plain workload, 134,217,728 for large)")
+    ap.add_argument("--workload", choices=("plain", "large"), default="plain",
+                    help="plain: 200 clumps in 10%% of the particles, K=32; large: 2,000 clumps in 15%%, K=64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-boundness", action="store_true")
     ap.add_argument("--boundness-clumps", type=int, default=5000)
@@ -289,9 +292,17 @@ def main():
     import plan_amd
     from plan_amd import synth
 
-    log(f"[bench] generating snapshot n={args.n}")
+    if args.n is None:
+        args.n = 16777216 if args.workload == "plain" else 134217728
+    if args.workload == "large":
+        # one context holds about 108 GiB at this size: a second context for
+        # the series would not fit, and the CPU restatement would run minutes
+        args.no_series = True
+        args.no_cpu_baseline = True
+        args.no_boundness = True
+    log(f"[bench] generating {args.workload} snapshot n={args.n}")
     t0 = time.perf_counter()
-    snap = synth.config_c2(n=args.n)
+    snap = synth.config_c2(n=args.n) if args.workload == "plain" else synth.config_c5(n=args.n)
     log(f"[bench] generated in {time.perf_counter() - t0:.1f}s; uploading")
     ctx = plan_amd.Context(0)
     t0 = time.perf_counter()
@@ -355,7 +366,8 @@ def main():
         cpu = cpu_baseline(threads, snap)
         log(f"[bench] cpu baseline: {cpu}")
     out = {
-        "metric": "particles/s end-to-end clump finding @16.8M",
+        "metric": ("particles/s end-to-end clump finding @16.8M" if args.workload == "plain"
+                   else "particles/s end-to-end clump finding @134M"),
         "value": value,
         "unit": "particles/s",
         "n_gpus": 1,
@@ -368,8 +380,10 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"plain drop-in: {args.n} particles, 200 Plummer clumps (10%), box [-0.1,0.1]^3, "
-                        f"K=32, H=64, seed 0; ghosts+BuildTree+FindPlanetesimals per step",
+            "workload": (f"plain drop-in: {args.n} particles, 200 Plummer clumps (10%), box [-0.1,0.1]^3, K=32, "
+                         if args.workload == "plain" else
+                         f"large: {args.n} particles, 2,000 Plummer clumps (15%), box [-0.1,0.1]^3, K=64, ")
+                        + "H=64, seed 0; ghosts+BuildTree+FindPlanetesimals per step",
             "N": args.n, "T_with_ghosts": T, "tree_particles": nparticles_tree, "nodes": info.num_nodes,
             "clumps_found": len(cat["clumps"]), "parallelism": "single GPU",
             "stage_ms": mean_st, "h2d_upload_s": upload_s,
