@@ -1333,10 +1333,12 @@ static void launch_select(const TreeView &tv, Arena &a, uint32_t nb, const uint3
         uint32_t rq = qpw, nr = novf;
         if (qpw == 64) {
             /* a wave whose include list overflowed covers 64 neighbouring
-             * queries whose spheres span many nodes: redo it as 4 waves of
-             * 16 queries (a smaller union of spheres); only the waves that
-             * still overflow take the resumable path */
-            const uint32_t sub = 4, nw2 = nw * sub;
+             * queries whose spheres span many nodes: redo it as 8 waves of
+             * 8 queries (a smaller union of spheres; at 16.8M particles no
+             * density wave and 56 densest-neighbour queries of 1.69M still
+             * overflow, against 144 and 336 with 4 waves of 16); only the
+             * waves that still overflow take the resumable path */
+            const uint32_t sub = 8, nw2 = nw * sub;
             rq = 64 / sub;
             uint32_t *ovf2 = a.get_t<uint32_t>("knn_ovf2", (size_t)nw2 + 1);
             uint32_t *opos2 = a.get_t<uint32_t>("knn_opos2", (size_t)nw2 + 1);

@@ -274,7 +274,7 @@ def test_catalog_files(name, tmp_path):
 
 @pytest.mark.parametrize("name,limit", [("n65k", 32), ("n65k", 8), ("n32k_k48_h24", 8)])
 def test_overflow_paths(name, limit):
-    """Waves whose include list overflows are redone as 16-query waves and,
+    """Waves whose include list overflows are redone as 8-query waves and,
     if still too long, walked in several fills; ball searches whose node
     frontier outgrows its buffers are redone with buffers sized from the
     counts.  Capping both (a host test hook) sends the density,
