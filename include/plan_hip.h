@@ -242,8 +242,9 @@ int plan_stage_times(plan_ctx *ctx, double *ms, int n);
  *  48-51 the slowest clump's rebinding: ms scanning its live candidate
  *  records, ms evaluating candidate energies and claiming, live records at
  *  its start, candidate energies evaluated,
- *  52-54 all rebinding runs: summed ms of live-record scans, of energy
- *  evaluation and of member re-sorts */
+ *  52-55 all rebinding runs: summed ms of live-record scans, of energy
+ *  evaluation, of member re-sorts and of building each run's live set and
+ *  member arrays */
 int plan_clump_stats(plan_ctx *ctx, double *values, int n);
 /* device milliseconds of the last density pass's gather kernel (the K-nearest
  * selection and density, one launch) and of its radius-schedule kernel */

@@ -405,8 +405,8 @@ class Context:
         return dict(zip(("ghosts", "build", "density", "hop", "clumps", "find_total"), t.tolist()))
 
     def clump_stats(self) -> dict:
-        v = np.zeros(55)
-        self._chk(lib().plan_clump_stats(self._h, v, 55), "plan_clump_stats")
+        v = np.zeros(56)
+        self._chk(lib().plan_clump_stats(self._h, v, 56), "plan_clump_stats")
         names = ["num_above", "hop_groups", "naive", "after_merge", "merge_passes", "pair_candidates",
                  "saddle_queries", "rebind_searches", "d2h_ms", "rebind_reruns", "ms_sort", "ms_mergeA", "ms_mergeB",
                  "ms_rekey", "ms_unbind", "ms_rebind", "ms_filters", "ms_rip", "ms_J", "ms_rank",
@@ -420,7 +420,8 @@ class Context:
                  "density_resumed_queries", "hop_resumed_queries", "key0_erased",
                  "density_pow_checks", "density_pow_fixes", "ms_catalog", "ms_clump_logic",
                  "rebind_max_scan_ms", "rebind_max_energy_ms", "rebind_max_live", "rebind_max_evals",
-                 "rebind_scan_total_ms", "rebind_energy_total_ms", "rebind_sort_total_ms"]
+                 "rebind_scan_total_ms", "rebind_energy_total_ms", "rebind_sort_total_ms",
+                 "rebind_setup_total_ms"]
         return {k: float(x) for k, x in zip(names, v)}
 
     def keep_input(self):

@@ -1389,7 +1389,7 @@ int plan_stage_times(plan_ctx *c, double *ms, int n)
 int plan_clump_stats(plan_ctx *c, double *v, int n)
 {
     if (!c || !v) return PLAN_E_ARG;
-    double a[55] = {(double)c->hop_na, (double)c->hop_groups, (double)c->cstats.naive, (double)c->cstats.after_merge,
+    double a[56] = {(double)c->hop_na, (double)c->hop_groups, (double)c->cstats.naive, (double)c->cstats.after_merge,
                     (double)c->cstats.merge_passes, (double)c->cstats.pair_candidates,
                     (double)c->cstats.saddle_queries, (double)c->cstats.rebind_searches, c->d2h_ms,
                     (double)c->cstats.rebind_reruns};
@@ -1425,7 +1425,8 @@ int plan_clump_stats(plan_ctx *c, double *v, int n)
     a[52] = c->cstats.rebind_scan_total_ms;
     a[53] = c->cstats.rebind_energy_total_ms;
     a[54] = c->cstats.rebind_sort_total_ms;
-    for (int i = 0; i < n && i < 55; ++i) v[i] = a[i];
+    a[55] = c->cstats.rebind_setup_total_ms;
+    for (int i = 0; i < n && i < 56; ++i) v[i] = a[i];
     return PLAN_OK;
 }
 

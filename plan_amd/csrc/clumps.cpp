@@ -819,7 +819,7 @@ struct Rebind {
     std::vector<uint32_t> claimed;
     uint32_t exact_searches = 0;
     uint32_t iters = 0;
-    double ms = 0, ms_exact = 0, ms_sort = 0, ms_scan = 0, ms_energy = 0;
+    double ms = 0, ms_exact = 0, ms_sort = 0, ms_scan = 0, ms_energy = 0, ms_setup = 0;
     /* live records at the start; candidate energies evaluated */
     uint64_t live0 = 0, evals = 0;
     std::chrono::steady_clock::time_point t0, t1; /* start and end of the run */
@@ -1312,6 +1312,7 @@ static void rebind_one(HClump &c, Rebind &rb, const HostMembers &hm, const std::
     for (size_t i = 0; i < L.size(); ++i) L.ord[i] = (uint32_t)i;
     L.n_sorted = L.size();
     const size_t n0 = L.size();
+    rb.ms_setup = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     std::vector<uint32_t> cand;
     std::vector<uint8_t> flag;
     size_t ndead = 0;
@@ -1596,6 +1597,7 @@ static void rebind_all(std::map<uint32_t, HClump> &cl, const HostMembers &hm, co
         st.rebind_scan_total_ms += rb[i].ms_scan;
         st.rebind_energy_total_ms += rb[i].ms_energy;
         st.rebind_sort_total_ms += rb[i].ms_sort;
+        st.rebind_setup_total_ms += rb[i].ms_setup;
     }
     part(2);
 }

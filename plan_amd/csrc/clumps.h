@@ -105,6 +105,8 @@ struct ClumpStats {
      * over all runs */
     double rebind_max_scan_ms = 0, rebind_max_energy_ms = 0, rebind_max_live = 0, rebind_max_evals = 0;
     double rebind_scan_total_ms = 0, rebind_energy_total_ms = 0, rebind_sort_total_ms = 0;
+    /* all rebinding runs: summed ms building the live set and member arrays */
+    double rebind_setup_total_ms = 0;
     /* rebinding runs: summed time, summed exact-search time (waits included),
      * schedule position of the longest run */
     double rebind_work_ms = 0, rebind_exact_total_ms = 0;
