@@ -258,9 +258,12 @@ struct ListEnt {
 };
 /* include-list capacity (entries) of one wave: larger K walks larger spheres.
  * The list is most of a wave's LDS, so the capacity trades occupancy against
- * the waves that overflow it and are redone in smaller pieces. */
+ * the waves that overflow it and are redone in smaller pieces.  K = 64 (the
+ * 134M-particle density pass, HOP): 640 entries (10.2 KB, 16 waves per CU);
+ * with 384, 512, 576, 640 and 768 entries that density launch of 209.9M
+ * queries took 1093, 789, 765, 711 and 847 ms. */
 template <int K> struct ListCap {
-    static constexpr int value = K >= 64 ? 512 : 320;
+    static constexpr int value = K >= 64 ? 640 : 320;
 };
 
 /* LDS of the group-wise walk: a stack of sibling groups still to visit
