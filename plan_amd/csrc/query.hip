@@ -265,6 +265,12 @@ struct ListEnt {
 template <int K> struct ListCap {
     static constexpr int value = K >= 64 ? 640 : 320;
 };
+/* the densest-neighbour pass (H = 64 over the particles above threshold)
+ * keeps 512 entries at K = 64: at 16.8M particles its launch takes 4.68 ms
+ * with 512 and 4.97 ms with 640 */
+template <int K> struct HopCap {
+    static constexpr int value = K >= 64 ? 512 : ListCap<K>::value;
+};
 
 /* LDS of the group-wise walk: a stack of sibling groups still to visit
  * (first node, count, mask of the queries that descended into their
@@ -876,7 +882,7 @@ __device__ __forceinline__ void select_list(const TreeView &tv, LDS &L, uint32_t
 {
     const int lane = threadIdx.x & 63;
     const double inf = __longlong_as_double(0x7ff0000000000000ll);
-    SelectScratch<ListCap<K>::value> &S = L.ss;
+    auto &S = L.ss;
     nent = __builtin_amdgcn_readfirstlane(nent);
     uint32_t M = 0;
     __syncthreads();
@@ -1103,7 +1109,7 @@ __device__ __forceinline__ void median_density(const TreeView &tv, double tm, do
  * (ovf[wave]) and stops.  RESUME = true: the flagged waves (waves[block])
  * walk in several list fills; each query's K best so far are kept in HBM
  * (sd2/sidx, K per query) between fills. */
-template <int K, bool RESUME, bool FUSE>
+template <int K, bool RESUME, bool FUSE, int CAP = ListCap<K>::value>
 __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, const uint32_t *__restrict__ qidx,
                                                    const double4 *__restrict__ qpts, double *__restrict__ r_io,
                                                    uint32_t *__restrict__ c_io, const double *__restrict__ rt_in,
@@ -1114,7 +1120,6 @@ __global__ __launch_bounds__(64) void k_knn_select(TreeView tv, uint32_t nq, con
                                                    uint32_t *__restrict__ densest, uint32_t *__restrict__ idx_out,
                                                    double *__restrict__ d2_out, uint32_t qpw)
 {
-    constexpr int CAP = ListCap<K>::value;
     using Lds = std::conditional_t<RESUME, ResumeLds<CAP>, SelectLds<CAP>>;
     __shared__ Lds L;
     const int lane = threadIdx.x;
@@ -1321,8 +1326,13 @@ static void launch_select(const TreeView &tv, Arena &a, uint32_t nb, const uint3
     uint32_t *otot = a.get_t<uint32_t>("knn_otot", 4);
     HIPCHK(hipMemsetAsync(ovf, 0, (size_t)nw * 4, s));
     if (out.ev) HIPCHK(hipEventRecord(out.ev[1], s));
-    hipLaunchKernelGGL((k_knn_select<K, false, FUSE>), dim3(nw), dim3(64), 0, s, tv, nb, qi, qp, rbuf, cbuf, rt, q0,
-                       nullptr, ovf, nullptr, nullptr, mode, kernel, rho_in, rho_o, dn_o, idx_o, d2_o, qpw);
+    if (mode == 1)
+        hipLaunchKernelGGL((k_knn_select<K, false, FUSE, HopCap<K>::value>), dim3(nw), dim3(64), 0, s, tv, nb, qi, qp,
+                           rbuf, cbuf, rt, q0, nullptr, ovf, nullptr, nullptr, mode, kernel, rho_in, rho_o, dn_o, idx_o,
+                           d2_o, qpw);
+    else
+        hipLaunchKernelGGL((k_knn_select<K, false, FUSE>), dim3(nw), dim3(64), 0, s, tv, nb, qi, qp, rbuf, cbuf, rt,
+                           q0, nullptr, ovf, nullptr, nullptr, mode, kernel, rho_in, rho_o, dn_o, idx_o, d2_o, qpw);
     LAUNCHCHK();
     if (out.ev) HIPCHK(hipEventRecord(out.ev[2], s));
     scan_exclusive_u32(a, ovf, opos, nw, s, otot);
